@@ -1,0 +1,196 @@
+"""Case definitions that drive a lattice backend through the HiLeMMS call
+sequence of the reference Apps (DefineBlocks ... Partition,
+SetInitialMacrosVars, PreDefinedInitialCondition, SetTimeStep).
+
+A backend is any object with the method set of mplb_amd.LatticeCase
+(add_block, add_connection, set_collision, add_boundary, prepare,
+set_macros, init_feq, set_dt, set_accel, step, ...).  The same builder
+therefore sets up the HIP path and the CPU oracle identically.
+
+Unit mapping (as in Apps/2DCavity/lbm2d_cavity.cpp:76-77, 138 and
+Src/model_kernel.inc:429): meshSize = 1/(N-1) for N nodes per side,
+dt = meshSize/sqrt(3), a lattice relaxation time tau_L maps to
+TauRef = (tau_L - 1/2) dt, and a lattice speed U maps to the code
+velocity sqrt(3) U (CalcBGKFeq takes cu = CS xi . u with CS = sqrt(3)).
+"""
+import math
+
+import numpy as np
+
+from .enums import BoundaryScheme as BSch
+from .enums import BoundarySurface as BS
+from .enums import VertexType as VT
+
+CS = math.sqrt(3.0)
+NOISE_SEED = 0x4D504C42
+_MASK = (1 << 64) - 1
+
+
+def splitmix64(x):
+    """Counter-based splitmix64 (Steele, Lea & Flood 2014) on uint64 arrays."""
+    z = (np.asarray(x, dtype=np.uint64) + np.uint64(0x9E3779B97F4A7C15))
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def density_noise(shape, seed=NOISE_SEED, amplitude=1e-4):
+    """rho0 = 1 + amplitude*(2u-1), u = splitmix64(seed ^ linear_index) as a
+    53-bit fraction; linear index i + nx*(j + ny*k), i fastest."""
+    n = int(np.prod(shape))
+    with np.errstate(over="ignore"):
+        idx = np.arange(n, dtype=np.uint64) ^ np.uint64(seed)
+        h = splitmix64(idx)
+    u = (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return (1.0 + amplitude * (2.0 * u - 1.0)).reshape(shape)
+
+
+def tau_ref_for(tau_lattice, dt):
+    return (tau_lattice - 0.5) * dt
+
+
+def cavity_params(n, re, ulid):
+    mesh = 1.0 / (n - 1)
+    dt = mesh / CS
+    tau_l = 0.5 + 3.0 * ulid * n / re
+    return mesh, dt, tau_l
+
+
+def cavity2d(be, n, re=100.0, ulid=0.05,
+             order=("Left", "Right", "Top", "Bottom"), rho0=None):
+    """Apps/2DCavity/lbm2d_cavity.cpp:72-135: one D2Q9 block, EQMDiffuseRefl
+    walls (stationary) and a moving lid on Top, defined in `order`."""
+    mesh, dt, tau_l = cavity_params(n, re, ulid)
+    be.add_block(0, (n, n))
+    be.set_collision(tau_ref_for(tau_l, dt), force_type=0)
+    lid = CS * ulid
+    for s in order:
+        vel = (lid, 0.0) if s == "Top" else (0.0, 0.0)
+        be.add_boundary(0, BS[s], BSch.EQMDiffuseRefl, vel, VT.Wall)
+    be.prepare()
+    shp = (n, n)
+    be.set_macros(0, rho0 if rho0 is not None else np.ones(shp),
+                  np.zeros(shp), np.zeros(shp))
+    be.init_feq()
+    be.set_dt(dt)
+    return dict(dt=dt, tau_l=tau_l, mesh=mesh, lid=lid)
+
+
+def cavity3d(be, n, re=400.0, ulid=0.05, noise=True,
+             order=("Left", "Right", "Top", "Bottom", "Front", "Back"),
+             size=None):
+    """Apps/3DCavity/lbm3d_cavity.cpp:72-180 with the hard-coded BC order
+    (Left, Right, Top, Bottom, Front, Back); the lid moves along x on Top.
+    With noise, rho0 = 1 +- 1e-4 from splitmix64 (seeded, counter based)."""
+    mesh, dt, tau_l = cavity_params(n, re, ulid)
+    size = tuple(size) if size is not None else (n, n, n)
+    be.add_block(0, size)
+    be.set_collision(tau_ref_for(tau_l, dt), force_type=0)
+    lid = CS * ulid
+    for s in order:
+        vel = (lid, 0.0, 0.0) if s == "Top" else (0.0, 0.0, 0.0)
+        be.add_boundary(0, BS[s], BSch.EQMDiffuseRefl, vel, VT.Wall)
+    be.prepare()
+    shp = tuple(reversed(size))
+    rho = density_noise(shp) if noise else np.ones(shp)
+    z = np.zeros(shp)
+    be.set_macros(0, rho, z, z, z)
+    be.init_feq()
+    be.set_dt(dt)
+    return dict(dt=dt, tau_l=tau_l, mesh=mesh, lid=lid)
+
+
+def periodic_box(be, size, tau_l, rho, u, v, w=None, force=None):
+    """A block whose every face is a periodic self-link (Field::CreateHalos,
+    field.h:207-214: "the periodic boundary can be treated by setting the
+    neighbor to the block itself") with MDPeriodic nodes on the envelope."""
+    dim = len(size)
+    be.add_block(0, size)
+    pairs = [(BS.Left, BS.Right), (BS.Right, BS.Left), (BS.Top, BS.Bottom),
+             (BS.Bottom, BS.Top)]
+    if dim == 3:
+        pairs += [(BS.Front, BS.Back), (BS.Back, BS.Front)]
+    for a, b in pairs:
+        be.add_connection(0, a, 0, b, VT.MDPeriodic)
+    dt = 1.0
+    be.set_collision(tau_ref_for(tau_l, dt), force_type=1 if force else 0,
+                     force_macros=bool(force))
+    for a, _ in pairs:
+        be.add_boundary(0, a, BSch.None_, (), VT.MDPeriodic)
+    be.prepare()
+    if force:
+        be.set_accel(0, force)
+    be.set_macros(0, rho, u, v, w)
+    be.init_feq()
+    be.set_dt(dt)
+    return dict(dt=dt)
+
+
+def channel2d(be, nx, ny, tau_l, g, order=("Top", "Bottom")):
+    """Body-force driven channel: periodic in x, EQMDiffuseRefl walls at
+    Top and Bottom, BodyForce_1st with the *_Force velocity macros."""
+    be.add_block(0, (nx, ny))
+    be.add_connection(0, BS.Left, 0, BS.Right, VT.MDPeriodic)
+    be.add_connection(0, BS.Right, 0, BS.Left, VT.MDPeriodic)
+    dt = 1.0
+    be.set_collision(tau_ref_for(tau_l, dt), force_type=1, force_macros=True)
+    be.add_boundary(0, BS.Left, BSch.None_, (), VT.MDPeriodic)
+    be.add_boundary(0, BS.Right, BSch.None_, (), VT.MDPeriodic)
+    for s in order:
+        be.add_boundary(0, BS[s], BSch.EQMDiffuseRefl, (0.0, 0.0), VT.Wall)
+    be.prepare()
+    be.set_accel(0, (g, 0.0))
+    shp = (ny, nx)
+    be.set_macros(0, np.ones(shp), np.zeros(shp), np.zeros(shp))
+    be.init_feq()
+    be.set_dt(dt)
+    return dict(dt=dt)
+
+
+def lchannel(be, n=33, top_lid=-0.001, outlet_rho=1.0, tau_ref=0.01,
+             mesh=None):
+    """Apps/3DLChannel/lbm3d_L.cpp:72-190: three D3Q19 blocks (Top, Middle,
+    Right) joined by VirtualBoundary faces, a moving wall on Top of block 0,
+    an ExtrapolPressure1ST outlet on the Right of block 2 and
+    EQMDiffuseRefl walls elsewhere, in the App's definition order."""
+    mesh = mesh if mesh is not None else 1.0 / (n - 1)
+    m = n - 1
+    be.add_block(0, (n, m, n))
+    be.add_block(1, (n, n, n))
+    be.add_block(2, (m, n, n))
+    for frm, fs, to, ts in ((0, BS.Bottom, 1, BS.Top), (1, BS.Top, 0, BS.Bottom),
+                            (1, BS.Right, 2, BS.Left), (2, BS.Left, 1, BS.Right)):
+        be.add_connection(frm, fs, to, ts, VT.VirtualBoundary)
+    be.set_collision(tau_ref, force_type=0)
+    wall = (0.0, 0.0, 0.0)
+    E = BSch.EQMDiffuseRefl
+    VB = (BSch.None_, (), VT.VirtualBoundary)
+    be.add_boundary(0, BS.Left, E, wall, VT.Wall)
+    be.add_boundary(0, BS.Right, E, wall, VT.Wall)
+    be.add_boundary(0, BS.Top, E, (0.0, top_lid, 0.0), VT.Wall)
+    be.add_boundary(0, BS.Bottom, *VB)
+    be.add_boundary(0, BS.Front, E, wall, VT.Wall)
+    be.add_boundary(0, BS.Back, E, wall, VT.Wall)
+    be.add_boundary(1, BS.Left, E, wall, VT.Wall)
+    be.add_boundary(1, BS.Right, *VB)
+    be.add_boundary(1, BS.Top, *VB)
+    be.add_boundary(1, BS.RightTop, E, wall, VT.Wall)
+    be.add_boundary(1, BS.Bottom, E, wall, VT.Wall)
+    be.add_boundary(1, BS.Front, E, wall, VT.Wall)
+    be.add_boundary(1, BS.Back, E, wall, VT.Wall)
+    be.add_boundary(2, BS.Left, *VB)
+    be.add_boundary(2, BS.Right, BSch.ExtrapolPressure1ST, (outlet_rho,),
+                    VT.OutLet)
+    be.add_boundary(2, BS.Top, E, wall, VT.Wall)
+    be.add_boundary(2, BS.Bottom, E, wall, VT.Wall)
+    be.add_boundary(2, BS.Front, E, wall, VT.Wall)
+    be.add_boundary(2, BS.Back, E, wall, VT.Wall)
+    be.prepare()
+    for b, size in ((0, (n, m, n)), (1, (n, n, n)), (2, (m, n, n))):
+        shp = tuple(reversed(size))
+        z = np.zeros(shp)
+        be.set_macros(b, np.ones(shp), z, z, z)
+    be.init_feq()
+    dt = mesh / CS
+    be.set_dt(dt)
+    return dict(dt=dt)

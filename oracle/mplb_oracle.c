@@ -1,0 +1,1008 @@
+/*
+ * mplb_oracle.c -- CPU oracle (FP64) for MPLB's per-step lattice update.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path and the
+ * timed CPU baseline.  Nothing in the product library links or calls it.
+ *
+ * It restates, sweep by sweep and in the reference's floating-point
+ * operation order, the StreamCollision step of inmeso/mplb
+ * (Src/evolution.cpp:137-193):
+ *   macros -> envelope copy -> body force -> BGK collision -> halo transfer
+ *   -> pull stream -> block boundaries in definition order.
+ * Build with -ffp-contract=off so that no multiply-add is fused: the
+ * reference's arithmetic is then reproduced operation for operation.
+ *
+ * Parity pins (checked by tests/test_oracle_*.py): lattice identities,
+ * equilibrium moments, collision invariants, periodic permutation and mass
+ * conservation, the rest state as a fixed point of the wall scheme, the
+ * shear-wave decay rate, body-force Poiseuille flow, the Ghia et al.
+ * cavity profile, and the reference quirks (frozen 3D corner populations,
+ * last-definition-wins at shared edges, the extrapolation rescale that
+ * skips the last population).  The halo slab extents at multi-block
+ * junction corners and FDPeriodic are "parity unpinned" (no fixture and
+ * no closed form reaches them); FDPeriodic is not restated.
+ */
+#include "mplb_oracle.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---- enums, values as in the reference headers ---------------------- */
+/* VertexType  (Src/flowfield_host_device.h:21-37) */
+enum { VT_FLUID = 10, VT_INLET = 11, VT_OUTLET = 12, VT_MDPERIODIC = 13,
+       VT_SYMMETRY = 14, VT_FDPERIODIC = 15, VT_WALL = 1000,
+       VT_IMMERSEDSOLID = -1, VT_IMMERSEDBOUNDARY = -2,
+       VT_VIRTUALBOUNDARY = -3 };
+/* VertexGeometryType faces (Src/flowfield_host_device.h:77-144) */
+enum { VG_FLUID = 0, VG_IMMERSEDSOLID = -1, VG_IP = 10, VG_IM = 11,
+       VG_JP = 20, VG_JM = 21, VG_KP = 30, VG_KM = 31 };
+/* BoundarySurface (Src/boundary_host_device.h:7-29) */
+enum { BS_LEFT = 0, BS_RIGHT = 1, BS_TOP = 2, BS_BOTTOM = 3, BS_FRONT = 4,
+       BS_BACK = 5, BS_LEFTBACK = 50, BS_LEFTFRONT = 40, BS_RIGHTBACK = 51,
+       BS_RIGHTFRONT = 41, BS_TOPBACK = 52, BS_TOPFRONT = 42,
+       BS_BOTTOMBACK = 53, BS_BOTTOMFRONT = 43, BS_LEFTTOP = 20,
+       BS_LEFTBOTTOM = 30, BS_RIGHTTOP = 21, BS_RIGHTBOTTOM = 31 };
+/* BoundaryScheme (Src/boundary.h:71-84) */
+enum { SCH_EXTRAPOLPRESSURE1ST = 16, SCH_MDPERIODIC = 18,
+       SCH_FDPERIODIC = 19, SCH_BOUNCEBACK = 20, SCH_EQMDIFFUSEREFL = 24,
+       SCH_NONE = -1 };
+/* BndryDvType (Src/boundary_host_device.h:34-39) */
+enum { DV_INCOMING = 1, DV_OUTGOING = 2, DV_PARALLEL = 3, DV_INVALID = -1 };
+
+#define MAXQ 19
+#define MAXB 64
+#define MAXBC 256
+#define MAXLINK 64
+
+typedef struct {
+    int id, n[3];      /* interior size; n[2] = 1 in 2D */
+    int h[3];          /* halo depth per axis: 1,1,1 (3D) or 1,1,0 (2D) */
+    int s[3];          /* allocated extent per axis (n + 2h) */
+    size_t nalloc;     /* nodes incl. halo */
+    double *f, *fs;    /* AoS [node][q] */
+    double *m[4];      /* rho, u, v, w */
+    double *mc[4];     /* copies for the residual (MacroVarsCopy) */
+    int *nt, *vg;      /* node type, geometry property */
+    double g[3];       /* uniform acceleration */
+} OBlock;
+
+typedef struct { int from, fsurf, to, tsurf, type; } OLink;
+typedef struct { int block, surface, scheme, vtype, nvars;
+                 double vars[8]; } OBC;
+
+struct orc_case {
+    int dim, Q;
+    double CS;
+    double XI[MAXQ * 3];   /* stored like the reference: XI[l*D + d] */
+    double W[MAXQ];
+    int OPP[MAXQ];
+    int nb;
+    OBlock b[MAXB];
+    int nlink;
+    OLink link[MAXLINK];
+    int nbc;
+    OBC bc[MAXBC];
+    double tau, dt;
+    int force_type, force_macros;
+    int prepared;
+    char err[256];
+};
+
+static int g_threads = 0;
+void orc_set_threads(int n) { g_threads = n; }
+int orc_get_threads(void) {
+#ifdef _OPENMP
+    return g_threads > 0 ? g_threads : omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+#ifdef _OPENMP
+#define OMP_FOR _Pragma("omp parallel for schedule(static) num_threads(orc_get_threads())")
+#else
+#define OMP_FOR
+#endif
+
+static int fail(orc_case* c, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(c->err, sizeof c->err, fmt, ap);
+    va_end(ap);
+    return -1;
+}
+const char* orc_error(orc_case* c) { return c->err; }
+
+/* ---- lattice tables: model.cpp:99-131, FindReverseXi model.cpp:82-97 -- */
+static void setup_lattice(orc_case* c, int lattice) {
+    c->CS = sqrt(3.0); /* model.cpp:67-68 */
+    if (lattice == 0) {
+        const double t00 = 4.0 / 9.0, t01 = 1.0 / 9.0, t11 = 1.0 / 36.0;
+        const double t[9] = {t00, t11, t01, t11, t01, t11, t01, t11, t01};
+        const int cx[9] = {0, -1, -1, -1, 0, 1, 1, 1, 0};
+        const int cy[9] = {0, 1, 0, -1, -1, -1, 0, 1, 1};
+        const int op[9] = {0, 5, 6, 7, 8, 1, 2, 3, 4};
+        c->Q = 9;
+        for (int l = 0; l < 9; l++) {
+            c->XI[l * 2] = cx[l];
+            c->XI[l * 2 + 1] = cy[l];
+            c->W[l] = t[l];
+            c->OPP[l] = op[l];
+        }
+    } else {
+        const double t000 = 1 / ((double)3), t001 = 1 / ((double)18),
+                     t011 = 1 / ((double)36);
+        const double t[19] = {t000, t001, t001, t001, t011, t011, t011,
+                              t011, t011, t011, t001, t001, t001, t011,
+                              t011, t011, t011, t011, t011};
+        const int cx[19] = {0, -1, 0, 0, -1, -1, -1, -1, 0, 0,
+                            1, 0, 0, 1, 1, 1, 1, 0, 0};
+        const int cy[19] = {0, 0, -1, 0, -1, 1, 0, 0, -1, -1,
+                            0, 1, 0, 1, -1, 0, 0, 1, 1};
+        const int cz[19] = {0, 0, 0, -1, 0, 0, -1, 1, -1, 1,
+                            0, 0, 1, 0, 0, 1, -1, 1, -1};
+        c->Q = 19;
+        for (int l = 0; l < 19; l++) {
+            c->XI[l * 3] = cx[l];
+            c->XI[l * 3 + 1] = cy[l];
+            c->XI[l * 3 + 2] = cz[l];
+            c->W[l] = t[l];
+        }
+        /* brute-force opposite search, first match wins */
+        for (int i = 0; i < 19; i++)
+            for (int j = 0; j < 19; j++) {
+                int rev = 1;
+                for (int k = 0; k < 3; k++)
+                    rev = rev && (c->XI[i * 3 + k] + c->XI[j * 3 + k] == 0.0);
+                if (rev) { c->OPP[i] = j; break; }
+            }
+    }
+}
+
+/* ---- index helpers ---------------------------------------------------- */
+static inline size_t nidx(const OBlock* b, int i, int j, int k) {
+    return ((size_t)(k + b->h[2]) * b->s[1] + (size_t)(j + b->h[1])) *
+               b->s[0] + (size_t)(i + b->h[0]);
+}
+static inline int XIi(const orc_case* c, int l, int d) {
+    return d < c->dim ? (int)c->XI[l * c->dim + d] : 0;
+}
+
+/* ---- CalcBGKFeq, T=1, polyOrder=2: model_host_device.h:34-57, 58-83 --- */
+static inline double feq2d(const orc_case* c, int l, double rho, double u,
+                           double v) {
+    const double CS = c->CS;
+    const double* XI = c->XI;
+    const double T = 1;
+    double cu = (CS * XI[l * 2] * u + CS * XI[l * 2 + 1] * v);
+    double c2 = (CS * XI[l * 2] * CS * XI[l * 2] +
+                 CS * XI[l * 2 + 1] * CS * XI[l * 2 + 1]);
+    double cu2 = cu * cu;
+    double u2 = u * u + v * v;
+    double res = 1.0 + cu + 0.5 * (cu2 - u2 + (T - 1.0) * (c2 - 2));
+    return c->W[l] * rho * res;
+}
+static inline double feq3d(const orc_case* c, int l, double rho, double u,
+                           double v, double w) {
+    const double CS = c->CS;
+    const double* XI = c->XI;
+    const double T = 1;
+    double cu = (CS * XI[l * 3] * u + CS * XI[l * 3 + 1] * v +
+                 CS * XI[l * 3 + 2] * w);
+    double c2 = (CS * XI[l * 3] * CS * XI[l * 3] +
+                 CS * XI[l * 3 + 1] * CS * XI[l * 3 + 1] +
+                 CS * XI[l * 3 + 2] * CS * XI[l * 3 + 2]);
+    double cu2 = cu * cu;
+    double u2 = u * u + v * v + w * w;
+    double res = 1.0 + cu + 0.5 * (cu2 - u2 + (T - 1.0) * (c2 - 3));
+    return c->W[l] * rho * res;
+}
+static inline double feq(const orc_case* c, int l, double rho, double u,
+                         double v, double w) {
+    return c->dim == 2 ? feq2d(c, l, rho, u, v) : feq3d(c, l, rho, u, v, w);
+}
+
+/* ---- creation -------------------------------------------------------- */
+orc_case* orc_create(int space_dim, int lattice) {
+    if (!((space_dim == 2 && lattice == 0) || (space_dim == 3 && lattice == 1)))
+        return NULL;
+    orc_case* c = (orc_case*)calloc(1, sizeof(orc_case));
+    c->dim = space_dim;
+    setup_lattice(c, lattice);
+    c->tau = 1.0;
+    c->dt = 1.0; /* DT{1} flowfield.cpp:88 */
+    c->force_type = 0;
+    return c;
+}
+
+void orc_destroy(orc_case* c) {
+    if (!c) return;
+    for (int ib = 0; ib < c->nb; ib++) {
+        OBlock* b = &c->b[ib];
+        free(b->f); free(b->fs); free(b->nt); free(b->vg);
+        for (int v = 0; v < 4; v++) { free(b->m[v]); free(b->mc[v]); }
+    }
+    free(c);
+}
+
+int orc_add_block(orc_case* c, int id, int nx, int ny, int nz) {
+    if (c->nb >= MAXB) return fail(c, "too many blocks");
+    if (id != c->nb) return fail(c, "blocks must be added in id order");
+    if (nx < 1 || ny < 1 || (c->dim == 3 && nz < 1))
+        return fail(c, "bad block size");
+    OBlock* b = &c->b[c->nb++];
+    memset(b, 0, sizeof *b);
+    b->id = id;
+    b->n[0] = nx; b->n[1] = ny; b->n[2] = c->dim == 3 ? nz : 1;
+    b->h[0] = 1; b->h[1] = 1; b->h[2] = c->dim == 3 ? 1 : 0;
+    for (int a = 0; a < 3; a++) b->s[a] = b->n[a] + 2 * b->h[a];
+    b->nalloc = (size_t)b->s[0] * b->s[1] * b->s[2];
+    /* ops_decl_dat without data: zero-filled storage */
+    b->f = (double*)calloc(b->nalloc * c->Q, sizeof(double));
+    b->fs = (double*)calloc(b->nalloc * c->Q, sizeof(double));
+    b->nt = (int*)calloc(b->nalloc, sizeof(int));
+    b->vg = (int*)calloc(b->nalloc, sizeof(int));
+    for (int v = 0; v < 4; v++) {
+        b->m[v] = (double*)calloc(b->nalloc, sizeof(double));
+        b->mc[v] = (double*)calloc(b->nalloc, sizeof(double));
+    }
+    return 0;
+}
+
+int orc_add_connection(orc_case* c, int from, int fsurf, int to, int tsurf,
+                       int type) {
+    if (c->nlink >= MAXLINK) return fail(c, "too many connections");
+    if (from < 0 || from >= c->nb || to < 0 || to >= c->nb)
+        return fail(c, "connection names an undefined block");
+    for (int l = 0; l < c->nlink; l++) /* Block::AddNeighbor block.cpp:141 */
+        if (c->link[l].from == from && c->link[l].fsurf == fsurf)
+            return fail(c, "a neighbour is already defined on this surface");
+    OLink L = {from, fsurf, to, tsurf, type};
+    c->link[c->nlink++] = L;
+    return 0;
+}
+
+int orc_set_collision(orc_case* c, double tau_ref, int force_type,
+                      int force_macros) {
+    if (force_type != 0 && force_type != 1)
+        return fail(c, "only BodyForce_None and BodyForce_1st are restated");
+    c->tau = tau_ref;
+    c->force_type = force_type;
+    c->force_macros = force_macros;
+    return 0;
+}
+
+int orc_add_boundary(orc_case* c, int block, int surface, int scheme,
+                     const double* vars, int nvars, int vtype) {
+    if (c->nbc >= MAXBC) return fail(c, "too many boundaries");
+    if (block < 0 || block >= c->nb) return fail(c, "undefined block");
+    if (scheme == SCH_FDPERIODIC)
+        return fail(c, "FDPeriodic is not restated by the oracle");
+    if (nvars > 8) return fail(c, "too many given variables");
+    OBC* bc = &c->bc[c->nbc++];
+    memset(bc, 0, sizeof *bc);
+    bc->block = block; bc->surface = surface; bc->scheme = scheme;
+    bc->vtype = vtype; bc->nvars = nvars;
+    for (int i = 0; i < nvars; i++) bc->vars[i] = vars[i];
+    return 0;
+}
+
+/* ---- Block::RangeStart/RangeEnd, block.cpp:14-97 ---------------------- */
+static void surface_range(const orc_case* c, const OBlock* b, int s,
+                          int r[6]) {
+    for (int a = 0; a < 3; a++) { r[2 * a] = 0; r[2 * a + 1] = b->n[a]; }
+    if (s == BS_RIGHT || s == BS_RIGHTTOP || s == BS_RIGHTBOTTOM ||
+        (c->dim == 3 && (s == BS_RIGHTFRONT || s == BS_RIGHTBACK)))
+        r[0] = b->n[0] - 1;
+    if (s == BS_LEFT || s == BS_LEFTBOTTOM || s == BS_LEFTTOP ||
+        (c->dim == 3 && (s == BS_LEFTFRONT || s == BS_LEFTBACK)))
+        r[1] = 1;
+    /* block.cpp:35 tests "OP_3D", never defined: TopBack/TopFront keep the
+     * full y range (parity quirk, reproduced). */
+    if (s == BS_TOP || s == BS_LEFTTOP || s == BS_RIGHTTOP)
+        r[2] = b->n[1] - 1;
+    if (s == BS_BOTTOM || s == BS_LEFTBOTTOM || s == BS_RIGHTBOTTOM ||
+        (c->dim == 3 && (s == BS_BOTTOMBACK || s == BS_BOTTOMFRONT)))
+        r[3] = 1;
+    if (c->dim == 3) {
+        if (s == BS_FRONT || s == BS_LEFTFRONT || s == BS_RIGHTFRONT ||
+            s == BS_TOPFRONT || s == BS_BOTTOMFRONT)
+            r[4] = b->n[2] - 1;
+        if (s == BS_BACK || s == BS_LEFTBACK || s == BS_RIGHTBACK ||
+            s == BS_TOPBACK || s == BS_BOTTOMBACK)
+            r[5] = 1;
+    }
+}
+
+static void set_int_range(OBlock* b, int* field, const int r[6], int val) {
+    for (int k = r[4]; k < r[5]; k++)
+        for (int j = r[2]; j < r[3]; j++)
+            for (int i = r[0]; i < r[1]; i++) field[nidx(b, i, j, k)] = val;
+}
+
+/* the six halo slabs around a block (flowfield_wrapper.cpp:303-390, the
+ * same ranges again in SetBulkandHaloNodesType :671-788) */
+static void set_halo_slabs(const orc_case* c, OBlock* b, int* field,
+                           int val) {
+    const int nx = b->n[0], ny = b->n[1], nz = b->n[2];
+    const int zl = c->dim == 3 ? -1 : 0, zh = c->dim == 3 ? nz + 1 : 1;
+    int r[6];
+    int rb[6] = {-1, nx + 1, -1, 0, zl, zh};      set_int_range(b, field, rb, val);
+    int rt[6] = {-1, nx + 1, ny, ny + 1, zl, zh}; set_int_range(b, field, rt, val);
+    int rl[6] = {-1, 0, -1, ny + 1, zl, zh};      set_int_range(b, field, rl, val);
+    int rr[6] = {nx, nx + 1, -1, ny + 1, zl, zh}; set_int_range(b, field, rr, val);
+    if (c->dim == 3) {
+        int rk[6] = {-1, nx + 1, -1, ny + 1, -1, 0};    set_int_range(b, field, rk, val);
+        int rf[6] = {-1, nx + 1, -1, ny + 1, nz, nz + 1}; set_int_range(b, field, rf, val);
+    }
+    (void)r;
+}
+
+/* SetBlockGeometryProperty, flowfield_wrapper.cpp:291-623 */
+static void set_geometry(orc_case* c, OBlock* b) {
+    const int nx = b->n[0], ny = b->n[1], nz = b->n[2];
+    int r[6];
+    int bulk[6] = {1, nx - 1, 1, ny - 1, c->dim == 3 ? 1 : 0,
+                   c->dim == 3 ? nz - 1 : 1};
+    set_int_range(b, b->vg, bulk, VG_FLUID);
+    set_halo_slabs(c, b, b->vg, VG_IMMERSEDSOLID);
+    surface_range(c, b, BS_BOTTOM, r); set_int_range(b, b->vg, r, VG_JP);
+    surface_range(c, b, BS_TOP, r);    set_int_range(b, b->vg, r, VG_JM);
+    surface_range(c, b, BS_LEFT, r);   set_int_range(b, b->vg, r, VG_IP);
+    surface_range(c, b, BS_RIGHT, r);  set_int_range(b, b->vg, r, VG_IM);
+    if (c->dim == 3) {
+        surface_range(c, b, BS_BACK, r);  set_int_range(b, b->vg, r, VG_KP);
+        surface_range(c, b, BS_FRONT, r); set_int_range(b, b->vg, r, VG_KM);
+    }
+    if (c->dim == 2) {
+        int a[6] = {0, 1, 0, 1, 0, 1};           set_int_range(b, b->vg, a, 10200);
+        int d[6] = {0, 1, ny - 1, ny, 0, 1};     set_int_range(b, b->vg, d, 10210);
+        int e[6] = {nx - 1, nx, ny - 1, ny, 0, 1}; set_int_range(b, b->vg, e, 11210);
+        int g[6] = {nx - 1, nx, 0, 1, 0, 1};     set_int_range(b, b->vg, g, 11200);
+        return;
+    }
+    /* 12 edges */
+    const int E[12][7] = {
+        {0, 1, 0, 1, 0, nz, 10200},           {0, 1, ny - 1, ny, 0, nz, 10210},
+        {nx - 1, nx, ny - 1, ny, 0, nz, 11210}, {nx - 1, nx, 0, 1, 0, nz, 11200},
+        {0, 1, 0, ny, 0, 1, 10300},           {0, 1, 0, ny, nz - 1, nz, 10310},
+        {nx - 1, nx, 0, ny, nz - 1, nz, 11310}, {nx - 1, nx, 0, ny, 0, 1, 11300},
+        {0, nx, 0, 1, 0, 1, 20300},           {0, nx, 0, 1, nz - 1, nz, 20310},
+        {0, nx, ny - 1, ny, nz - 1, nz, 21310}, {0, nx, ny - 1, ny, 0, 1, 21300}};
+    for (int e = 0; e < 12; e++) set_int_range(b, b->vg, E[e], E[e][6]);
+    /* 8 corners */
+    const int C8[8][7] = {
+        {0, 1, 0, 1, 0, 1, 1020300},
+        {0, 1, 0, 1, nz - 1, nz, 1020310},
+        {0, 1, ny - 1, ny, 0, 1, 1021300},
+        {0, 1, ny - 1, ny, nz - 1, nz, 1021310},
+        {nx - 1, nx, 0, 1, 0, 1, 1120300},
+        {nx - 1, nx, 0, 1, nz - 1, nz, 1120310},
+        {nx - 1, nx, ny - 1, ny, 0, 1, 1121300},
+        {nx - 1, nx, ny - 1, ny, nz - 1, nz, 1121310}};
+    for (int e = 0; e < 8; e++) set_int_range(b, b->vg, C8[e], C8[e][6]);
+}
+
+/* SetBulkandHaloNodesType, flowfield_wrapper.cpp:671-788 */
+static void set_bulk_halo_types(orc_case* c, OBlock* b) {
+    int bulk[6] = {1, b->n[0] - 1, 1, b->n[1] - 1, c->dim == 3 ? 1 : 0,
+                   c->dim == 3 ? b->n[2] - 1 : 1};
+    set_int_range(b, b->nt, bulk, VT_FLUID);
+    set_halo_slabs(c, b, b->nt, VT_IMMERSEDSOLID);
+}
+
+/* SetBoundaryNodeType, flowfield_wrapper.cpp:625-669 */
+static void set_boundary_types(orc_case* c) {
+    for (int ib = 0; ib < c->nbc; ib++) {
+        OBC* bc = &c->bc[ib];
+        OBlock* b = &c->b[bc->block];
+        int r[6];
+        surface_range(c, b, bc->surface, r);
+        if (bc->vtype == VT_VIRTUALBOUNDARY) {
+            if (bc->surface == BS_LEFT || bc->surface == BS_RIGHT) {
+                r[2]++; r[3]--;
+                if (c->dim == 3) { r[4]++; r[5]--; }
+            }
+            if (bc->surface == BS_TOP || bc->surface == BS_BOTTOM) {
+                r[0]++; r[1]--;
+                if (c->dim == 3) { r[4]++; r[5]--; }
+            }
+            if (c->dim == 3 &&
+                (bc->surface == BS_FRONT || bc->surface == BS_BACK)) {
+                r[0]++; r[1]--; r[2]++; r[3]--;
+            }
+        }
+        set_int_range(b, b->nt, r, bc->vtype);
+    }
+}
+
+static void copy_current_macros(orc_case* c) {
+    for (int ib = 0; ib < c->nb; ib++) {
+        OBlock* b = &c->b[ib];
+        for (int v = 0; v < 4; v++)
+            memcpy(b->mc[v], b->m[v], b->nalloc * sizeof(double));
+    }
+}
+
+int orc_prepare(orc_case* c) {
+    for (int l = 0; l < c->nlink; l++) {
+        const OLink* L = &c->link[l];
+        const OBlock* a = &c->b[L->from];
+        const OBlock* t = &c->b[L->to];
+        int ax = (L->fsurf == BS_LEFT || L->fsurf == BS_RIGHT) ? 0
+               : (L->fsurf == BS_TOP || L->fsurf == BS_BOTTOM) ? 1 : 2;
+        for (int d = 0; d < c->dim; d++)
+            if (d != ax && a->n[d] != t->n[d])
+                return fail(c, "connected blocks differ in a transverse size");
+    }
+    for (int ib = 0; ib < c->nb; ib++) {
+        set_geometry(c, &c->b[ib]);
+        set_bulk_halo_types(c, &c->b[ib]);
+    }
+    set_boundary_types(c);
+    copy_current_macros(c);
+    c->prepared = 1;
+    return 0;
+}
+
+int orc_set_macros(orc_case* c, int block, const double* rho,
+                   const double* u, const double* v, const double* w) {
+    if (block < 0 || block >= c->nb) return fail(c, "undefined block");
+    OBlock* b = &c->b[block];
+    const double* src[4] = {rho, u, v, w};
+    size_t p = 0;
+    for (int k = 0; k < b->n[2]; k++)
+        for (int j = 0; j < b->n[1]; j++)
+            for (int i = 0; i < b->n[0]; i++, p++) {
+                size_t n = nidx(b, i, j, k);
+                for (int var = 0; var < 1 + c->dim; var++)
+                    b->m[var][n] = src[var][p];
+            }
+    return 0;
+}
+
+void orc_set_dt(orc_case* c, double dt) { c->dt = dt; }
+
+int orc_set_accel(orc_case* c, int block, const double* g) {
+    if (block < 0 || block >= c->nb) return fail(c, "undefined block");
+    for (int d = 0; d < 3; d++) c->b[block].g[d] = d < c->dim ? g[d] : 0.0;
+    return 0;
+}
+
+/* KerInitialiseBGK2nd[3D], model_kernel.inc:193-219, 352-380, over the
+ * WholeRange, then CopyCurrentMacroVar (model_wrapper.cpp:387-389) */
+int orc_init_feq(orc_case* c) {
+    if (!c->prepared) return fail(c, "call orc_prepare first");
+    const int Q = c->Q;
+    for (int ib = 0; ib < c->nb; ib++) {
+        OBlock* b = &c->b[ib];
+        for (int k = 0; k < b->n[2]; k++)
+            for (int j = 0; j < b->n[1]; j++)
+                for (int i = 0; i < b->n[0]; i++) {
+                    size_t n = nidx(b, i, j, k);
+                    if (b->nt[n] == VT_IMMERSEDSOLID) continue;
+                    for (int q = 0; q < Q; q++)
+                        b->f[n * Q + q] = feq(c, q, b->m[0][n], b->m[1][n],
+                                              b->m[2][n], b->m[3][n]);
+                }
+    }
+    copy_current_macros(c);
+    return 0;
+}
+
+/* ---- the step ---------------------------------------------------------- */
+
+/* KerCalcDensity[3D], KerCalcU/V/W[3D] and the *Force variants,
+ * model_kernel.inc:57-186, 570-759, driven by model_wrapper.cpp:112-275 */
+void orc_update_macros(orc_case* c) {
+    const int Q = c->Q, D = c->dim;
+    const double CS = c->CS;
+    const double* XI = c->XI;
+    for (int ib = 0; ib < c->nb; ib++) {
+        OBlock* b = &c->b[ib];
+        const int nk = b->n[2];
+        OMP_FOR
+        for (int k = 0; k < nk; k++)
+            for (int j = 0; j < b->n[1]; j++)
+                for (int i = 0; i < b->n[0]; i++) {
+                    size_t n = nidx(b, i, j, k);
+                    int vt = b->nt[n];
+                    if (vt == VT_IMMERSEDSOLID) continue;
+                    const double* f = &b->f[n * Q];
+                    double rho = 0;
+                    for (int q = 0; q < Q; q++) rho += f[q];
+                    b->m[0][n] = rho;
+                    for (int d = 0; d < D; d++) {
+                        double u = 0;
+                        for (int q = 0; q < Q; q++)
+                            u += CS * XI[q * D + d] * f[q];
+                        u /= b->m[0][n];
+                        if (c->force_macros &&
+                            (vt == VT_FLUID || vt == VT_MDPERIODIC))
+                            u += (c->dt * b->g[d] / 2);
+                        b->m[1 + d][n] = u;
+                    }
+                }
+    }
+}
+
+/* CopyBlockEnvelopDistribution, flowfield_wrapper.cpp:113-232 */
+static void envelope_copy(orc_case* c) {
+    const int Q = c->Q;
+    const int faces[6] = {BS_LEFT, BS_RIGHT, BS_BOTTOM, BS_TOP, BS_BACK,
+                          BS_FRONT};
+    for (int ib = 0; ib < c->nb; ib++) {
+        OBlock* b = &c->b[ib];
+        for (int fidx = 0; fidx < (c->dim == 3 ? 6 : 4); fidx++) {
+            int r[6];
+            surface_range(c, b, faces[fidx], r);
+            for (int k = r[4]; k < r[5]; k++)
+                for (int j = r[2]; j < r[3]; j++)
+                    for (int i = r[0]; i < r[1]; i++) {
+                        size_t n = nidx(b, i, j, k);
+                        for (int q = 0; q < Q; q++)
+                            b->fs[n * Q + q] = b->f[n * Q + q];
+                    }
+        }
+    }
+}
+
+/* KerCalcBodyForce1ST[3D] / KerCalcBodyForceNone[3D],
+ * model_kernel.inc:312-348, 504-541; CalcBodyForce model_host_device.h:25-32 */
+static void body_force(orc_case* c) {
+    const int Q = c->Q, D = c->dim;
+    const double CS = c->CS;
+    const double* XI = c->XI;
+    for (int ib = 0; ib < c->nb; ib++) {
+        OBlock* b = &c->b[ib];
+        OMP_FOR
+        for (int k = 0; k < b->n[2]; k++)
+            for (int j = 0; j < b->n[1]; j++)
+                for (int i = 0; i < b->n[0]; i++) {
+                    size_t n = nidx(b, i, j, k);
+                    int vt = b->nt[n];
+                    if (!(vt == VT_FLUID || vt == VT_MDPERIODIC)) continue;
+                    for (int q = 0; q < Q; q++) {
+                        if (c->force_type == 1) {
+                            double cf = 0;
+                            for (int d = 0; d < D; d++)
+                                cf += CS * XI[q * D + d] * b->g[d];
+                            b->fs[n * Q + q] = c->W[q] * b->m[0][n] * cf;
+                        } else {
+                            b->fs[n * Q + q] = 0;
+                        }
+                    }
+                }
+    }
+}
+
+/* KerCollideBGKIsothermal[3D], model_kernel.inc:221-265, 410-455 */
+static void collide(orc_case* c) {
+    const int Q = c->Q;
+    const double tau = c->tau, dt = c->dt;
+    const double dtOvertauPlusdt = dt / (tau + 0.5 * dt);
+    for (int ib = 0; ib < c->nb; ib++) {
+        OBlock* b = &c->b[ib];
+        OMP_FOR
+        for (int k = 0; k < b->n[2]; k++)
+            for (int j = 0; j < b->n[1]; j++)
+                for (int i = 0; i < b->n[0]; i++) {
+                    size_t n = nidx(b, i, j, k);
+                    int vt = b->nt[n];
+                    if (vt == VT_IMMERSEDSOLID) continue;
+                    const double rho = b->m[0][n], u = b->m[1][n],
+                                 v = b->m[2][n], w = b->m[3][n];
+                    double* fs = &b->fs[n * Q];
+                    const double* f = &b->f[n * Q];
+                    for (int q = 0; q < Q; q++) {
+                        const double fe = feq(c, q, rho, u, v, w);
+                        if (vt == VT_FLUID || vt == VT_MDPERIODIC)
+                            fs[q] = fe + (1 - dtOvertauPlusdt) * (f[q] - fe) +
+                                    tau * dtOvertauPlusdt * fs[q];
+                        else
+                            fs[q] = fe + (1 - dtOvertauPlusdt) * (f[q] - fe);
+                    }
+                }
+    }
+}
+
+/* ops_halo_transfer of fStage over the halos declared by Field::CreateHalos,
+ * field.h:215-395: blocks in id order, neighbours in BoundarySurface order,
+ * slabs one node deep that span the transverse halos. */
+static void halo_transfer(orc_case* c) {
+    const int Q = c->Q;
+    for (int ib = 0; ib < c->nb; ib++) {
+        const int order[6] = {BS_LEFT, BS_RIGHT, BS_TOP, BS_BOTTOM, BS_FRONT,
+                              BS_BACK};
+        for (int oi = 0; oi < 6; oi++) {
+            for (int l = 0; l < c->nlink; l++) {
+                const OLink* L = &c->link[l];
+                if (L->from != ib || L->fsurf != order[oi]) continue;
+                OBlock* a = &c->b[L->from];
+                OBlock* t = &c->b[L->to];
+                const int hz = c->dim == 3 ? 1 : 0;
+                int sz[3] = {a->n[0] + 2, a->n[1] + 2, a->n[2] + 2 * hz};
+                int src[3] = {-1, -1, -hz}, dst[3] = {-1, -1, -hz};
+                switch (L->fsurf) {
+                    case BS_RIGHT: sz[0] = 1; src[0] = a->n[0] - 1; dst[0] = -1; break;
+                    case BS_LEFT:  sz[0] = 1; src[0] = 0; dst[0] = t->n[0]; break;
+                    case BS_BOTTOM: sz[1] = 1; src[1] = 0; dst[1] = t->n[1]; break;
+                    case BS_TOP:   sz[1] = 1; src[1] = a->n[1] - 1; dst[1] = -1; break;
+                    case BS_BACK:  sz[2] = 1; src[2] = 0; dst[2] = t->n[2]; break;
+                    case BS_FRONT: sz[2] = 1; src[2] = a->n[2] - 1; dst[2] = -1; break;
+                }
+                for (int z = 0; z < sz[2]; z++)
+                    for (int y = 0; y < sz[1]; y++)
+                        for (int x = 0; x < sz[0]; x++) {
+                            size_t ns = nidx(a, src[0] + x, src[1] + y, src[2] + z);
+                            size_t nd = nidx(t, dst[0] + x, dst[1] + y, dst[2] + z);
+                            memcpy(&t->fs[nd * Q], &a->fs[ns * Q],
+                                   Q * sizeof(double));
+                        }
+            }
+        }
+    }
+}
+
+/* ---- geometry decoding ------------------------------------------------- */
+/* A geometry code concatenates two-digit face codes (10=IP, 11=IM, 20=JP,
+ * 21=JM, 30=KP, 31=KM) and, for edges and corners, a trailing digit 0 (_I)
+ * or 1 (_O); flowfield_host_device.h:77-144. */
+static int vg_decode(int vg, int ax[3], int sg[3], int* inner) {
+    int nf, codes[3];
+    if (vg >= 10 && vg < 100) {
+        nf = 1; codes[0] = vg; *inner = 1;
+    } else if (vg >= 10000 && vg < 100000) {
+        nf = 2; *inner = (vg % 10) == 0; vg /= 10;
+        codes[0] = vg / 100; codes[1] = vg % 100;
+    } else if (vg >= 1000000 && vg < 10000000) {
+        nf = 3; *inner = (vg % 10) == 0; vg /= 10;
+        codes[0] = vg / 10000; codes[1] = (vg / 100) % 100; codes[2] = vg % 100;
+    } else {
+        return 0;
+    }
+    for (int i = 0; i < nf; i++) {
+        ax[i] = codes[i] / 10 - 1;
+        sg[i] = (codes[i] % 10) == 0 ? 1 : -1;
+        if (ax[i] < 0 || ax[i] > 2 || (codes[i] % 10) > 1) return 0;
+    }
+    return nf;
+}
+
+/* pull predicate of KerStream[3D] at boundary nodes,
+ * scheme_kernel.inc:80-217 (2D), 1428-2049 (3D): every normal axis must
+ * satisfy sign*c <= 0; all axes (_I) or any axis (_O). */
+static int stream_pulls(int vg, const int cv[3]) {
+    int ax[3], sg[3], inner;
+    int nf = vg_decode(vg, ax, sg, &inner);
+    if (nf == 0) return 0;
+    int all = 1, any = 0;
+    for (int i = 0; i < nf; i++) {
+        int ok = sg[i] * cv[ax[i]] <= 0;
+        all = all && ok;
+        any = any || ok;
+    }
+    return (nf == 1 || inner) ? all : any;
+}
+
+/* FindBdyDvType[3D], boundary_host_device.h:47-183, 191-706, in components
+ * along the inward normals cn[k] = sign_k * c_{axis_k}. */
+static int dv_type(int vg, const int cv[3]) {
+    int ax[3], sg[3], inner;
+    int nf = vg_decode(vg, ax, sg, &inner);
+    if (nf == 0) return DV_INVALID;
+    int cn[3];
+    for (int i = 0; i < nf; i++) cn[i] = sg[i] * cv[ax[i]];
+    if (nf == 1) {
+        if (cn[0] < 0) return DV_INCOMING;
+        if (cn[0] == 0) return DV_PARALLEL;
+        return DV_OUTGOING;
+    }
+    if (nf == 2) {
+        if ((cn[0] <= 0 && cn[1] < 0) || (cn[0] < 0 && cn[1] == 0))
+            return DV_INCOMING;
+        if ((cn[0] >= 0 && cn[1] > 0) || (cn[0] > 0 && cn[1] == 0))
+            return DV_OUTGOING;
+        return DV_PARALLEL;
+    }
+    /* 3D corners: the reference names the wall-pointing set "Outgoing" */
+    if (cv[0] == 0 && cv[1] == 0 && cv[2] == 0) return DV_PARALLEL;
+    if (cn[0] <= 0 && cn[1] <= 0 && cn[2] <= 0) return DV_OUTGOING;
+    if (cn[0] >= 0 && cn[1] >= 0 && cn[2] >= 0) return DV_INCOMING;
+    return DV_PARALLEL;
+}
+
+/* KerStream / KerStream3D, scheme_kernel.inc:53-221, 1400-2052 */
+static void stream(orc_case* c) {
+    const int Q = c->Q;
+    for (int ib = 0; ib < c->nb; ib++) {
+        OBlock* b = &c->b[ib];
+        OMP_FOR
+        for (int k = 0; k < b->n[2]; k++)
+            for (int j = 0; j < b->n[1]; j++)
+                for (int i = 0; i < b->n[0]; i++) {
+                    size_t n = nidx(b, i, j, k);
+                    const int vt = b->nt[n], vg = b->vg[n];
+                    for (int q = 0; q < Q; q++) {
+                        int cv[3] = {XIi(c, q, 0), XIi(c, q, 1), XIi(c, q, 2)};
+                        size_t src = nidx(b, i - cv[0], j - cv[1], k - cv[2]);
+                        if (vt == VT_FLUID || vt == VT_MDPERIODIC ||
+                            vt == VT_VIRTUALBOUNDARY) {
+                            b->f[n * Q + q] = b->fs[src * Q + q];
+                            continue;
+                        }
+                        if (vt == VT_IMMERSEDSOLID) continue;
+                        if (cv[0] == 0 && cv[1] == 0 && cv[2] == 0) {
+                            b->f[n * Q + q] = b->fs[n * Q + q];
+                            continue;
+                        }
+                        if (stream_pulls(vg, cv))
+                            b->f[n * Q + q] = b->fs[src * Q + q];
+                    }
+                }
+    }
+}
+
+/* KerCutCellEQMDiffuseRefl[3D], boundary_kernel.inc:534-613, 1184-1276 */
+static void bc_eqm(orc_case* c, OBlock* b, const int r[6], const double* gv) {
+    const int Q = c->Q, D = c->dim;
+    const double CS = c->CS;
+    const double* XI = c->XI;
+    const double u = gv[0], v = gv[1], w = D == 3 ? gv[2] : 0.0;
+    for (int k = r[4]; k < r[5]; k++)
+        for (int j = r[2]; j < r[3]; j++)
+            for (int i = r[0]; i < r[1]; i++) {
+                size_t n = nidx(b, i, j, k);
+                double* f = &b->f[n * Q];
+                const int vg = b->vg[n];
+                int outgoing[MAXQ], parallel[MAXQ];
+                int nout = 0, npar = 0;
+                double rhoIncoming = 0, rhoParallel = 0, deltaRho = 0;
+                for (int q = 0; q < Q; q++) {
+                    int cv[3] = {XIi(c, q, 0), XIi(c, q, 1), XIi(c, q, 2)};
+                    const double cx = CS * XI[q * D];
+                    const double cy = CS * XI[q * D + 1];
+                    const double cz = D == 3 ? CS * XI[q * D + 2] : 0.0;
+                    switch (dv_type(vg, cv)) {
+                        case DV_INCOMING:
+                            rhoIncoming += f[q];
+                            break;
+                        case DV_OUTGOING:
+                            outgoing[nout++] = q;
+                            if (D == 2)
+                                deltaRho += (2 * c->W[q]) * (cx * u + cy * v);
+                            else
+                                deltaRho += (2 * c->W[q]) *
+                                            (cx * u + cy * v + cz * w);
+                            break;
+                        case DV_PARALLEL:
+                            parallel[npar++] = q;
+                            rhoParallel += feq(c, q, 1, u, v, w);
+                            break;
+                        default:
+                            break;
+                    }
+                }
+                const double rhoWall =
+                    2 * rhoIncoming / (1 - deltaRho - rhoParallel);
+                for (int p = 0; p < npar; p++)
+                    f[parallel[p]] = feq(c, parallel[p], rhoWall, u, v, w);
+                for (int p = 0; p < nout; p++) {
+                    const int q = outgoing[p];
+                    const double cx = CS * XI[q * D];
+                    const double cy = CS * XI[q * D + 1];
+                    const double cz = D == 3 ? CS * XI[q * D + 2] : 0.0;
+                    if (D == 2)
+                        f[q] = f[c->OPP[q]] +
+                               2 * rhoWall * c->W[q] * (cx * u + cy * v);
+                    else
+                        f[q] = f[c->OPP[q]] + 2 * rhoWall * c->W[q] *
+                                                  (cx * u + cy * v + cz * w);
+                }
+            }
+}
+
+/* inward normal of the six block faces, as used by the extrapolation */
+static int surface_face_code(int s) {
+    switch (s) {
+        case BS_LEFT: return 10;
+        case BS_RIGHT: return 11;
+        case BS_BOTTOM: return 20;
+        case BS_TOP: return 21;
+        case BS_BACK: return 30;
+        case BS_FRONT: return 31;
+    }
+    return 0;
+}
+
+/* KerCutCellExtrapolPressure1ST[3D], boundary_kernel.inc:404-488, 894-1182.
+ * Nodes of one face never read each other, so the order inside the face
+ * range does not matter; the final rescale stops before the last
+ * population (":484", ":1178"). */
+static void bc_extrapol(orc_case* c, OBlock* b, const int r[6], int surface,
+                        double rhoGiven) {
+    const int Q = c->Q;
+    const int scode = surface_face_code(surface);
+    for (int k = r[4]; k < r[5]; k++)
+        for (int j = r[2]; j < r[3]; j++)
+            for (int i = r[0]; i < r[1]; i++) {
+                size_t n = nidx(b, i, j, k);
+                double* f = &b->f[n * Q];
+                const int vg = b->vg[n];
+                int ax[3], sg[3], inner;
+                const int nf = vg_decode(vg, ax, sg, &inner);
+                /* offset: a face copies from its inward neighbour whatever
+                 * the surface; an edge or corner only when the surface is
+                 * one of its faces */
+                int off[3] = {0, 0, 0}, have = 0;
+                if (nf == 1 && inner) {
+                    off[ax[0]] = sg[0]; have = 1;
+                } else if (nf >= 2 && inner) {
+                    for (int a = 0; a < nf; a++)
+                        if (scode == (ax[a] + 1) * 10 + (sg[a] > 0 ? 0 : 1)) {
+                            off[ax[a]] = sg[a]; have = 1;
+                        }
+                }
+                double rho = 0;
+                for (int q = 0; q < Q; q++) {
+                    int cv[3] = {XIi(c, q, 0), XIi(c, q, 1), XIi(c, q, 2)};
+                    int unknown = 0;
+                    if (nf >= 1 && inner) {
+                        int allge = 1, nz = 0;
+                        for (int a = 0; a < nf; a++) {
+                            int cn = sg[a] * cv[ax[a]];
+                            allge = allge && cn >= 0;
+                            nz = nz || cn != 0;
+                        }
+                        unknown = nf == 1 ? (sg[0] * cv[ax[0]] > 0)
+                                          : (allge && nz);
+                    }
+                    if (unknown && have) {
+                        size_t src = nidx(b, i + off[0], j + off[1], k + off[2]);
+                        f[q] = b->f[src * Q + q];
+                    }
+                    rho += f[q];
+                }
+                const double ratio = rhoGiven / rho;
+                for (int q = 0; q < Q - 1; q++) f[q] *= ratio;
+            }
+}
+
+/* ImplementBoundary[3D] -> TreatBlockBoundary[3D], boundary.cpp:379-398,
+ * boundary_wrapper.cpp:12-131 */
+static void boundaries(orc_case* c) {
+    for (int ib = 0; ib < c->nbc; ib++) {
+        OBC* bc = &c->bc[ib];
+        OBlock* b = &c->b[bc->block];
+        int r[6];
+        surface_range(c, b, bc->surface, r);
+        if (bc->scheme == SCH_EQMDIFFUSEREFL)
+            bc_eqm(c, b, r, bc->vars);
+        else if (bc->scheme == SCH_EXTRAPOLPRESSURE1ST)
+            bc_extrapol(c, b, r, bc->surface, bc->vars[0]);
+    }
+}
+
+int orc_step(orc_case* c, int nsteps) {
+    if (!c->prepared) return fail(c, "call orc_prepare first");
+    for (int s = 0; s < nsteps; s++) {
+        orc_update_macros(c);
+        envelope_copy(c);
+        body_force(c);
+        collide(c);
+        halo_transfer(c);
+        stream(c);
+        boundaries(c);
+    }
+    return 0;
+}
+
+/* ---- accessors ---------------------------------------------------------- */
+int orc_get_f(orc_case* c, int block, double* out) {
+    if (block < 0 || block >= c->nb) return fail(c, "undefined block");
+    OBlock* b = &c->b[block];
+    const int Q = c->Q;
+    size_t p = 0;
+    for (int q = 0; q < Q; q++)
+        for (int k = 0; k < b->n[2]; k++)
+            for (int j = 0; j < b->n[1]; j++)
+                for (int i = 0; i < b->n[0]; i++)
+                    out[p++] = b->f[nidx(b, i, j, k) * Q + q];
+    return 0;
+}
+
+int orc_set_f(orc_case* c, int block, const double* in) {
+    if (block < 0 || block >= c->nb) return fail(c, "undefined block");
+    OBlock* b = &c->b[block];
+    const int Q = c->Q;
+    size_t p = 0;
+    for (int q = 0; q < Q; q++)
+        for (int k = 0; k < b->n[2]; k++)
+            for (int j = 0; j < b->n[1]; j++)
+                for (int i = 0; i < b->n[0]; i++)
+                    b->f[nidx(b, i, j, k) * Q + q] = in[p++];
+    return 0;
+}
+
+int orc_get_macro(orc_case* c, int block, int var, double* out) {
+    if (block < 0 || block >= c->nb) return fail(c, "undefined block");
+    if (var < 0 || var > c->dim) return fail(c, "bad macro variable");
+    OBlock* b = &c->b[block];
+    size_t p = 0;
+    for (int k = 0; k < b->n[2]; k++)
+        for (int j = 0; j < b->n[1]; j++)
+            for (int i = 0; i < b->n[0]; i++)
+                out[p++] = b->m[var][nidx(b, i, j, k)];
+    return 0;
+}
+
+int orc_get_int(orc_case* c, int block, int which, int* out) {
+    if (block < 0 || block >= c->nb) return fail(c, "undefined block");
+    OBlock* b = &c->b[block];
+    const int* src = which == 0 ? b->nt : b->vg;
+    size_t p = 0;
+    for (int k = 0; k < b->n[2]; k++)
+        for (int j = 0; j < b->n[1]; j++)
+            for (int i = 0; i < b->n[0]; i++) out[p++] = src[nidx(b, i, j, k)];
+    return 0;
+}
+
+double orc_mass(orc_case* c) {
+    double mass = 0;
+    const int Q = c->Q;
+    for (int ib = 0; ib < c->nb; ib++) {
+        OBlock* b = &c->b[ib];
+        for (int k = 0; k < b->n[2]; k++)
+            for (int j = 0; j < b->n[1]; j++)
+                for (int i = 0; i < b->n[0]; i++) {
+                    const double* f = &b->f[nidx(b, i, j, k) * Q];
+                    for (int q = 0; q < Q; q++) mass += f[q];
+                }
+    }
+    return mass;
+}
+
+/* CalcResidualError (flowfield_wrapper.cpp:31-89): the squared difference
+ * against the copy, the copy refreshed, then the sum of squares; the two
+ * reductions start from zero (the reduction handle is reset by OPS when
+ * its result is fetched, an assumption: OPS is not available). */
+double orc_residual(orc_case* c, int check_period, double* per_var) {
+    const int nv = 1 + c->dim;
+    double diff[4] = {0, 0, 0, 0}, sum[4] = {0, 0, 0, 0};
+    for (int v = 0; v < nv; v++)
+        for (int ib = 0; ib < c->nb; ib++) {
+            OBlock* b = &c->b[ib];
+            for (int k = 0; k < b->n[2]; k++)
+                for (int j = 0; j < b->n[1]; j++)
+                    for (int i = 0; i < b->n[0]; i++) {
+                        size_t n = nidx(b, i, j, k);
+                        diff[v] = diff[v] + (b->m[v][n] - b->mc[v][n]) *
+                                                (b->m[v][n] - b->mc[v][n]);
+                    }
+        }
+    copy_current_macros(c);
+    for (int v = 0; v < nv; v++)
+        for (int ib = 0; ib < c->nb; ib++) {
+            OBlock* b = &c->b[ib];
+            for (int k = 0; k < b->n[2]; k++)
+                for (int j = 0; j < b->n[1]; j++)
+                    for (int i = 0; i < b->n[0]; i++) {
+                        size_t n = nidx(b, i, j, k);
+                        sum[v] = sum[v] + (b->m[v][n]) * (b->m[v][n]);
+                    }
+        }
+    /* GetMaximumResidual, flowfield.cpp:148-158 */
+    double maxRes = 0;
+    for (int v = 0; v < nv; v++) {
+        double rel = (diff[v] / sum[v]) / (check_period * c->dt);
+        if (per_var) per_var[v] = rel;
+        if (maxRes <= rel) maxRes = rel;
+    }
+    return maxRes;
+}
