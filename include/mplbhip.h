@@ -1,0 +1,128 @@
+/*
+ * mplbhip.h -- C ABI of the MI355X lattice-update library (libmplbhip.so).
+ *
+ * This is the drop-in boundary under MPLB's HiLeMMS case API.  Each entry
+ * point replaces one reference interface (cited per function); the C++
+ * HiLeMMS layer (mplb_amd/hilemms) and the Python binding (mplb_amd/hip.py)
+ * both sit on top of it.
+ *
+ * Conventions
+ *  - Every call returns 0 on success or a negative code; the message of the
+ *    last failure is kept per context (mplbhip_last_error).  No exception
+ *    crosses this boundary.  Errors are reported, never silently absorbed:
+ *    there is no CPU fallback.
+ *  - Plain pointers and sizes only.  Host arrays are owned by the caller;
+ *    device memory is owned by the context.
+ *  - Host field layout: interior nodes only, i fastest, [k][j][i]; the
+ *    distribution function adds the population index outermost,
+ *    [q][k][j][i].  2D cases use k = 0 only.
+ *  - One context = one case (one component) on one device, driven by one
+ *    host thread; calls on one context are not thread-safe.
+ *  - Enumerations carry the reference's integer values (VertexType,
+ *    BoundarySurface, BoundaryScheme, BodyForceType).
+ */
+#ifndef MPLBHIP_H
+#define MPLBHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mplbhip_ctx mplbhip_ctx;
+
+/* population storage in HBM */
+enum {
+    MPLBHIP_STORE_F64 = 0,   /* f in double: the bit-exact parity path */
+    MPLBHIP_STORE_F32 = 1,   /* f - w_i in float, FP32 arithmetic */
+    MPLBHIP_STORE_F16 = 2    /* (f - w_i)/s_i in half, FP32 arithmetic */
+};
+
+enum {
+    MPLBHIP_ERR_ARG = -1,     /* bad argument or call order */
+    MPLBHIP_ERR_HIP = -2,     /* a HIP runtime call failed */
+    MPLBHIP_ERR_UNSUPPORTED = -3,
+    MPLBHIP_ERR_NODEVICE = -4 /* compute requested on a plan-only context */
+};
+
+/* Create a context.  device >= 0 selects a GPU; device = -1 builds a
+ * plan-only context (geometry, node typing and boundary programs can be
+ * inspected, nothing is computed).  space_dim 2 selects D2Q9, 3 D3Q19
+ * (DefineCase flowfield.cpp:96-104 + DefineComponents model.cpp:208-310). */
+int mplbhip_create(mplbhip_ctx** ctx, int device, int space_dim, int storage);
+void mplbhip_destroy(mplbhip_ctx* ctx);
+const char* mplbhip_last_error(const mplbhip_ctx* ctx);
+/* library build string (arch, version) */
+const char* mplbhip_version(void);
+
+/* DefineBlocks (flowfield.cpp:160-223): ids 0..n-1 in order; size has
+ * space_dim entries, each at least 3. */
+int mplbhip_define_block(mplbhip_ctx* ctx, int id, const int* size);
+/* DefineBlockConnection (flowfield.cpp:258-283), one pair per call:
+ * fStage halos are copied from from_block's from_surface slab to
+ * to_block (Field::CreateHalos field.h:215-395). */
+int mplbhip_define_connection(mplbhip_ctx* ctx, int from_block,
+                              int from_surface, int to_block, int to_surface,
+                              int vertex_type);
+/* DefineComponents tauRef (model.cpp:236) + DefineCollision
+ * (Collision_BGKIsothermal2nd only, model.cpp:406-445) + DefineBodyForce
+ * (model.cpp:447-489, force_type BodyForce_None=0 / BodyForce_1st=1) +
+ * the macro kind of DefineMacroVars (force_macros=1 selects
+ * Variable_U_Force..., model_kernel.inc:665-759). */
+int mplbhip_define_collision(mplbhip_ctx* ctx, double tau_ref,
+                             int force_type, int force_macros);
+/* DefineBlockBoundary (boundary.cpp:61-130); the call order is the
+ * application order (ImplementBoundary boundary.cpp:379-386).  Schemes:
+ * EQMDiffuseRefl (24, vars = wall velocity u,v[,w]), ExtrapolPressure1ST
+ * (16, vars = rho), None (-1) / MDPeriodic (18) (node typing only). */
+int mplbhip_define_boundary(mplbhip_ctx* ctx, int block, int surface,
+                            int scheme, const double* vars, int nvars,
+                            int vertex_type);
+/* Partition -> PrepareFlowField (flowfield.cpp:108-111, 224-244): node
+ * typing, geometry, per-region boundary programs, HBM allocation. */
+int mplbhip_prepare(mplbhip_ctx* ctx);
+/* node type (which=0) or geometry property (which=1) of one block, as the
+ * reference's NodeType_<compo> / GeometryProperty fields. */
+int mplbhip_node_info(mplbhip_ctx* ctx, int block, int which, int* out);
+
+/* SetInitialMacrosVars upload (app hook, e.g. lbm3d_cavity.cpp:44-68);
+ * w is ignored in 2D and may be NULL there. */
+int mplbhip_set_macros(mplbhip_ctx* ctx, int block, const double* rho,
+                       const double* u, const double* v, const double* w);
+/* PreDefinedInitialCondition[3D] (model_wrapper.cpp:347-391): f = feq of
+ * the uploaded macros at every non-solid node. */
+int mplbhip_init_feq(mplbhip_ctx* ctx);
+/* restart: set f of every block (array per block, [q][k][j][i]) */
+int mplbhip_set_f(mplbhip_ctx* ctx, int block, const double* f);
+/* SetTimeStep (flowfield.cpp:146) */
+int mplbhip_set_timestep(mplbhip_ctx* ctx, double dt);
+/* acceleration of one block (g_MacroBodyforce, uniform per block) */
+int mplbhip_set_accel(mplbhip_ctx* ctx, int block, const double* g);
+
+/* StreamCollision x nsteps (evolution.cpp:137-193), queued on the
+ * context stream and returned once complete. */
+int mplbhip_step(mplbhip_ctx* ctx, int nsteps);
+/* nsteps timed with HIP events on the context stream around each step
+ * kernel: ms_total is the wall span, ms_kernel the summed step-kernel time */
+int mplbhip_step_timed(mplbhip_ctx* ctx, int nsteps, double* ms_total,
+                       double* ms_kernel);
+/* UpdateMacroVars[3D] (model_wrapper.cpp:112-275): macros of the current f */
+int mplbhip_update_macros(mplbhip_ctx* ctx);
+/* current distribution function (post boundary conditions), [q][k][j][i] */
+int mplbhip_get_f(mplbhip_ctx* ctx, int block, double* out);
+/* macro variable 0=rho 1=u 2=v 3=w as last updated */
+int mplbhip_get_macro(mplbhip_ctx* ctx, int block, int var, double* out);
+/* CalcResidualError + GetMaximumResidual (flowfield_wrapper.cpp:31-89,
+ * flowfield.cpp:148-158) on the current f, in FP64 on the device.
+ * per_var receives 1+space_dim values; *max their maximum. */
+int mplbhip_residual(mplbhip_ctx* ctx, int check_period, double* per_var,
+                     double* max);
+/* FP64 total mass of the current f over every interior node */
+int mplbhip_mass(mplbhip_ctx* ctx, double* mass);
+/* steps taken since the initial condition */
+int mplbhip_time_step(mplbhip_ctx* ctx, long long* step);
+int mplbhip_sync(mplbhip_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,660 @@
+// lattice_kernels.hip -- gfx950 kernels of the MPLB lattice update.
+//
+// One fused pull kernel per step: a thread owns one node, gathers the
+// post-collision populations of the previous step from its neighbours
+// (stream, scheme_kernel.inc:53-221 / 1400-2052), applies the boundary
+// program of its region (boundary_kernel.inc), computes the macroscopic
+// variables (model_kernel.inc:57-186 / 570-759) and collides
+// (model_kernel.inc:221-265 / 410-455), writing the post-collision state
+// to the other ping-pong buffer.  The state kept in HBM is therefore fStage
+// (the reference's post-collision field) plus, for the few boundary regions
+// whose unstreamed populations must survive a step, a compact persist
+// buffer holding their post-boundary f.
+//
+// FP64 storage reproduces the reference arithmetic operation for operation
+// (contraction off, same summation order, IEEE division), so the parity
+// path matches the CPU oracle bit for bit.  FP32/FP16 storage keeps
+// g = f - w_i and does the bulk update in FP32 in deviation form.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "../../include/mplbhip.h"
+#include "lattice_kernels.hpp"
+
+namespace mplbhip {
+
+// sqrt(3) rounded to double: equals the host's sqrt(3.0) (model.cpp:67-68)
+constexpr double kCS = 1.7320508075688772935274463415059;
+
+__host__ __device__ constexpr int cvel(int D, int q, int a) {
+    constexpr int t2[9][2] = {{0, 0},  {-1, 1}, {-1, 0}, {-1, -1}, {0, -1},
+                              {1, -1}, {1, 0},  {1, 1},  {0, 1}};
+    constexpr int t3[19][3] = {
+        {0, 0, 0},  {-1, 0, 0},  {0, -1, 0}, {0, 0, -1}, {-1, -1, 0},
+        {-1, 1, 0}, {-1, 0, -1}, {-1, 0, 1}, {0, -1, -1}, {0, -1, 1},
+        {1, 0, 0},  {0, 1, 0},   {0, 0, 1},  {1, 1, 0},  {1, -1, 0},
+        {1, 0, 1},  {1, 0, -1},  {0, 1, 1},  {0, 1, -1}};
+    return D == 2 ? (a < 2 ? t2[q][a] : 0) : t3[q][a];
+}
+__host__ __device__ constexpr double wgt(int D, int q) {
+    constexpr double w2[9] = {4.0 / 9.0,  1.0 / 36.0, 1.0 / 9.0,
+                              1.0 / 36.0, 1.0 / 9.0,  1.0 / 36.0,
+                              1.0 / 9.0,  1.0 / 36.0, 1.0 / 9.0};
+    constexpr double w3[19] = {
+        1.0 / 3.0,  1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0,
+        1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+        1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0,
+        1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+    return D == 2 ? w2[q] : w3[q];
+}
+__host__ __device__ constexpr int opp(int D, int q) {
+    constexpr int o2[9] = {0, 5, 6, 7, 8, 1, 2, 3, 4};
+    constexpr int o3[19] = {0, 10, 11, 12, 13, 14, 15, 16, 17, 18,
+                            1, 2,  3,  4,  5,  6,  7,  8,  9};
+    return D == 2 ? o2[q] : o3[q];
+}
+
+// ---- storage traits -------------------------------------------------------
+template <int ST> struct Store;
+template <> struct Store<MPLBHIP_STORE_F64> { using E = double; using T = double; };
+template <> struct Store<MPLBHIP_STORE_F32> { using E = float; using T = float; };
+template <> struct Store<MPLBHIP_STORE_F16> { using E = __half; using T = float; };
+
+template <int ST>
+__device__ __forceinline__ typename Store<ST>::T ld(const typename Store<ST>::E* p,
+                                                    int64_t i, float sc) {
+    if constexpr (ST == MPLBHIP_STORE_F16) return __half2float(p[i]) * sc;
+    else return p[i];
+}
+template <int ST>
+__device__ __forceinline__ void st(typename Store<ST>::E* p, int64_t i,
+                                   typename Store<ST>::T v, float isc) {
+    if constexpr (ST == MPLBHIP_STORE_F16) p[i] = __float2half_rn(v * isc);
+    else p[i] = v;
+}
+// non-temporal store for the streamed post-collision state
+template <int ST>
+__device__ __forceinline__ void st_nt(typename Store<ST>::E* p, int64_t i,
+                                      typename Store<ST>::T v, float isc) {
+    if constexpr (ST == MPLBHIP_STORE_F16) {
+        p[i] = __float2half_rn(v * isc);
+    } else {
+        __builtin_nontemporal_store(v, p + i);
+    }
+}
+
+// ---- reference arithmetic (any T; exact for double) -------------------------
+// CalcBGKFeq, T = 1, polyOrder = 2 (model_host_device.h:34-83).  The
+// (T-1)(c2-D) term is an exact zero and is left out.
+template <int D, typename T>
+__device__ __forceinline__ T feq_ref(int q, T rho, T u, T v, T w) {
+#pragma clang fp contract(off)
+    const T cs = T(kCS);
+    T cu;
+    T u2;
+    if constexpr (D == 2) {
+        cu = cs * T(cvel(D, q, 0)) * u + cs * T(cvel(D, q, 1)) * v;
+        u2 = u * u + v * v;
+    } else {
+        cu = cs * T(cvel(D, q, 0)) * u + cs * T(cvel(D, q, 1)) * v +
+             cs * T(cvel(D, q, 2)) * w;
+        u2 = u * u + v * v + w * w;
+    }
+    const T cu2 = cu * cu;
+    const T res = T(1.0) + cu + T(0.5) * (cu2 - u2);
+    return T(wgt(D, q)) * rho * res;
+}
+
+template <int D, typename T>
+__device__ __forceinline__ T cu_wall(int q, T u, T v, T w) {
+#pragma clang fp contract(off)
+    const T cs = T(kCS);
+    const T cx = cs * T(cvel(D, q, 0)), cy = cs * T(cvel(D, q, 1));
+    if constexpr (D == 2) {
+        return cx * u + cy * v;
+    } else {
+        const T cz = cs * T(cvel(D, q, 2));
+        return cx * u + cy * v + cz * w;
+    }
+}
+
+// KerCutCellEQMDiffuseRefl[3D] (boundary_kernel.inc:534-613, 1184-1276)
+template <int D, typename T>
+__device__ void apply_eqm(T* f, const BCOp& op) {
+#pragma clang fp contract(off)
+    constexpr int Q = D == 2 ? 9 : 19;
+    const T u = T(op.vars[0]), v = T(op.vars[1]), w = T(op.vars[2]);
+    T rhoIn = T(0), rhoPar = T(0), dRho = T(0);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        if (op.m_in >> q & 1u) {
+            rhoIn += f[q];
+        } else if (op.m_out >> q & 1u) {
+            dRho += (T(2) * T(wgt(D, q))) * cu_wall<D, T>(q, u, v, w);
+        } else if (op.m_par >> q & 1u) {
+            rhoPar += feq_ref<D, T>(q, T(1), u, v, w);
+        }
+    }
+    const T rhoW = T(2) * rhoIn / (T(1) - dRho - rhoPar);
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        if (op.m_par >> q & 1u) f[q] = feq_ref<D, T>(q, rhoW, u, v, w);
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        if (op.m_out >> q & 1u)
+            f[q] = f[opp(D, q)] +
+                   T(2) * rhoW * T(wgt(D, q)) * cu_wall<D, T>(q, u, v, w);
+}
+
+// rescale part of KerCutCellExtrapolPressure1ST[3D]
+// (boundary_kernel.inc:404-488, 894-1182); f already holds the copied
+// populations.  The loop stops before the last population (:484, :1178).
+template <int D, typename T>
+__device__ void extrapol_rescale(T* f, double rho_given) {
+#pragma clang fp contract(off)
+    constexpr int Q = D == 2 ? 9 : 19;
+    T rho = T(0);
+#pragma unroll
+    for (int q = 0; q < Q; q++) rho += f[q];
+    const T ratio = T(rho_given) / rho;
+#pragma unroll
+    for (int q = 0; q < Q - 1; q++) f[q] *= ratio;
+}
+
+// ---- node addressing --------------------------------------------------------
+template <int D>
+__device__ __forceinline__ int32_t pull_delta(const BlockParams& B, int q) {
+    return cvel(D, q, 0) + cvel(D, q, 1) * B.sx + cvel(D, q, 2) * B.sz;
+}
+
+__device__ __forceinline__ int region_code(int x, int n) {
+    return x == 0 ? 1 : (x == n - 1 ? 2 : 0);
+}
+
+template <int D>
+__device__ __forceinline__ int64_t persist_slot(const RegionProg& P, int x,
+                                                int y, int z) {
+    const int l0 = P.ext[0] > 1 ? x - 1 : 0;
+    const int l1 = P.ext[1] > 1 ? y - 1 : 0;
+    const int l2 = (D == 3 && P.ext[2] > 1) ? z - 1 : 0;
+    return (int64_t)P.persist + l0 + (int64_t)P.ext[0] * (l1 + (int64_t)P.ext[1] * l2);
+}
+
+// full-value state of a node at the start of the boundary stage: streamed
+// populations from the neighbours, unstreamed ones from the persist buffer
+template <int D, int ST>
+__device__ void gather_masked(const KParams& kp, const BlockParams& B,
+                              const RegionProg& P, int x, int y, int z,
+                              typename Store<ST>::T* f) {
+    using T = typename Store<ST>::T;
+    using E = typename Store<ST>::E;
+    constexpr int Q = D == 2 ? 9 : 19;
+    const E* prev = static_cast<const E*>(kp.fs_prev) + B.base;
+    const int32_t node = z * B.sz + y * B.sx + x;
+    const T* per = static_cast<const T*>(kp.per_prev);
+    int64_t slot = P.persist >= 0 ? persist_slot<D>(P, x, y, z) : 0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        T v;
+        if (P.pull >> q & 1u) {
+            v = ld<ST>(prev, (int64_t)q * B.sq + node - pull_delta<D>(B, q),
+                       kp.scale[q]);
+            if constexpr (ST != MPLBHIP_STORE_F64) v += T(wgt(D, q));
+        } else if (P.persist >= 0) {
+            v = per[slot * Q + q];
+        } else {
+            v = T(0);
+        }
+        f[q] = v;
+    }
+}
+
+// one population of a neighbour, as the extrapolation reads it: after the
+// neighbour's boundary applications that precede it in definition order
+template <int D, int ST>
+__device__ void neighbour_values(const KParams& kp, const BlockParams& B,
+                                 const BCOp& op, int x, int y, int z,
+                                 typename Store<ST>::T* f) {
+    using T = typename Store<ST>::T;
+    constexpr int Q = D == 2 ? 9 : 19;
+    const RegionProg& N = kp.progs[B.prog_base + op.nb_region];
+    const int nx = x + op.off[0], ny = y + op.off[1], nz = z + op.off[2];
+    T g[Q];
+    gather_masked<D, ST>(kp, B, N, nx, ny, nz, g);
+    for (int o = 0; o < op.nb_nops; o++) {
+        if (N.op[o].type == OP_EQM) apply_eqm<D, T>(g, N.op[o]);
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        if (op.m_ext >> q & 1u) f[q] = g[q];
+}
+
+template <int D, int ST>
+__device__ void boundary_program(const KParams& kp, const BlockParams& B,
+                                 const RegionProg& P, int x, int y, int z,
+                                 typename Store<ST>::T* f) {
+    using T = typename Store<ST>::T;
+    for (int o = 0; o < P.nops; o++) {
+        const BCOp& op = P.op[o];
+        if (op.type == OP_EQM) {
+            apply_eqm<D, T>(f, op);
+        } else if (op.type == OP_EXTRAPOL) {
+            if (op.m_ext) neighbour_values<D, ST>(kp, B, op, x, y, z, f);
+            extrapol_rescale<D, T>(f, op.vars[0]);
+        }
+    }
+}
+
+// ---- workgroup reduction of the residual accumulators ------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ void reduce_store(double* acc, double* out) {
+    __shared__ double red[kTY][kNAcc];
+    const int lane = threadIdx.x, wv = threadIdx.y;
+#pragma unroll
+    for (int a = 0; a < kNAcc; a++) {
+        const double s = wave_sum(acc[a]);
+        if (lane == 0) red[wv][a] = s;
+    }
+    __syncthreads();
+    if (lane == 0 && wv == 0) {
+        for (int a = 0; a < kNAcc; a++) {
+            double s = 0.0;
+            for (int k = 0; k < kTY; k++) s += red[k][a];
+            out[a] = s;
+        }
+    }
+}
+
+// logical workgroup -> XCD-contiguous chunk: hardware deals workgroups to
+// the 8 XCDs round robin; this hands each XCD a contiguous run of tiles so
+// that neighbouring tiles (and their halo re-reads) share an L2.
+__device__ __forceinline__ int logical_wg(int wg, int n) {
+    if ((n & 7) != 0) return wg;
+    return (wg & 7) * (n >> 3) + (wg >> 3);
+}
+
+__device__ __forceinline__ int find_block(const KParams& kp, int t) {
+    int lo = 0, hi = kp.nblocks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (kp.blocks[mid].tile_start <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---- the fused kernel ---------------------------------------------------------
+template <int D, int ST, int MODE>
+__global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
+    using T = typename Store<ST>::T;
+    using E = typename Store<ST>::E;
+    constexpr int Q = D == 2 ? 9 : 19;
+    constexpr bool EXACT = ST == MPLBHIP_STORE_F64;
+    const int wg = logical_wg(blockIdx.x, gridDim.x);
+    const int b = find_block(kp, wg);
+    const BlockParams& B = kp.blocks[b];
+    int t = wg - B.tile_start;
+    const int tx = t % B.tiles_x;
+    t /= B.tiles_x;
+    const int ty = t % B.tiles_y;
+    const int z = t / B.tiles_y;
+    const int x = tx * kTX + threadIdx.x;
+    const int y = ty * kTY + threadIdx.y;
+    const bool active = x < B.n[0] && y < B.n[1] && B.active;
+    double acc[kNAcc];
+#pragma unroll
+    for (int a = 0; a < kNAcc; a++) acc[a] = 0.0;
+
+    if (active) {
+        const int32_t node = z * B.sz + y * B.sx + x;
+        const int64_t mnode = B.mbase + ((int64_t)z * B.n[1] + y) * B.n[0] + x;
+        const int r = region_code(x, B.n[0]) + 3 * region_code(y, B.n[1]) +
+                      (D == 3 ? 9 * region_code(z, B.n[2]) : 0);
+        const RegionProg* P = r != 0 ? &kp.progs[B.prog_base + r] : nullptr;
+        const bool solid = P && P->kind == RK_SKIP;
+        const bool force_node = P ? P->force != 0 : true;
+        T f[Q];  // EXACT: f; otherwise g = f - w (bulk) / f (boundary)
+        bool full = false;  // f[] holds full values (boundary path)
+        const E* prev = static_cast<const E*>(kp.fs_prev) + B.base;
+        E* next = static_cast<E*>(kp.fs_next) + B.base;
+
+        if (solid) {
+            // ImmersedSolid: neither streamed nor collided; fStage unchanged
+            if constexpr (MODE != MODE_MATERIALIZE) {
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const int64_t i = (int64_t)q * B.sq + node;
+                    next[i] = prev[i];
+                }
+            }
+            goto reduce;
+        }
+        if constexpr (MODE == MODE_INIT_FEQ || MODE == MODE_INIT_F) {
+            // f^0 in double, from the macros (KerInitialiseBGK2nd[3D],
+            // model_kernel.inc:193-219, 352-380) or from a host array
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                double fv;
+                if constexpr (MODE == MODE_INIT_FEQ) {
+                    fv = feq_ref<D, double>(q, kp.macro[0][mnode], kp.macro[1][mnode],
+                                            kp.macro[2][mnode],
+                                            D == 3 ? kp.macro[3][mnode] : 0.0);
+                } else {
+                    const int64_t nn = (int64_t)B.n[0] * B.n[1] * B.n[2];
+                    fv = kp.fhost[B.fbase + q * nn +
+                                  ((int64_t)z * B.n[1] + y) * B.n[0] + x];
+                }
+                if constexpr (MODE == MODE_INIT_FEQ) {
+                    if (kp.fhost) {
+                        const int64_t nn = (int64_t)B.n[0] * B.n[1] * B.n[2];
+                        kp.fhost[B.fbase + q * nn +
+                                 ((int64_t)z * B.n[1] + y) * B.n[0] + x] = fv;
+                    }
+                }
+                // the deviation is formed in double before rounding
+                if constexpr (EXACT) f[q] = fv;
+                else f[q] = T(fv - wgt(D, q));
+                if (P && P->persist >= 0) {
+                    T* per = static_cast<T*>(kp.per_next);
+                    per[persist_slot<D>(*P, x, y, z) * Q + q] = T(fv);
+                }
+            }
+        } else if (MODE == MODE_MATERIALIZE && kp.fhost_in) {
+            // the initial f, which no previous fStage can reproduce
+            const int64_t nn = (int64_t)B.n[0] * B.n[1] * B.n[2];
+            const int64_t h = B.fbase + ((int64_t)z * B.n[1] + y) * B.n[0] + x;
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const double fv = kp.fhost[h + q * nn];
+                if constexpr (EXACT) f[q] = fv;
+                else f[q] = T(fv - wgt(D, q));
+            }
+        } else if (!P) {
+            // bulk: every population streams in (KerStream3D Fluid branch)
+#pragma unroll
+            for (int q = 0; q < Q; q++)
+                f[q] = ld<ST>(prev, (int64_t)q * B.sq + node - pull_delta<D>(B, q),
+                              kp.scale[q]);
+        } else {
+            gather_masked<D, ST>(kp, B, *P, x, y, z, f);
+            full = true;
+            boundary_program<D, ST>(kp, B, *P, x, y, z, f);
+            if constexpr (MODE != MODE_MATERIALIZE) {
+                if (P->persist >= 0) {
+                    T* per = static_cast<T*>(kp.per_next);
+                    const int64_t s = persist_slot<D>(*P, x, y, z);
+#pragma unroll
+                    for (int q = 0; q < Q; q++) per[s * Q + q] = f[q];
+                }
+            }
+        }
+
+        if constexpr (EXACT) {
+#pragma clang fp contract(off)
+            // UpdateMacroVars: density then each velocity component
+            double rho = 0;
+#pragma unroll
+            for (int q = 0; q < Q; q++) rho += f[q];
+            double uvw[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                double u = 0;
+#pragma unroll
+                for (int q = 0; q < Q; q++) u += kCS * double(cvel(D, q, d)) * f[q];
+                u /= rho;
+                if (kp.force_macros && force_node) u += (kp.dt * B.g[d] / 2);
+                uvw[d] = u;
+            }
+            if constexpr (MODE == MODE_MATERIALIZE || MODE == MODE_STEP_CHECK) {
+                const double m[4] = {rho, uvw[0], uvw[1], uvw[2]};
+#pragma unroll
+                for (int v = 0; v < D + 1; v++) {
+                    if (MODE == MODE_STEP_CHECK || kp.want_residual) {
+                        const double old = kp.macro_copy[v][mnode];
+                        acc[v] = (m[v] - old) * (m[v] - old);
+                        acc[4 + v] = m[v] * m[v];
+                        kp.macro_copy[v][mnode] = m[v];
+                    }
+                    kp.macro[v][mnode] = m[v];
+                }
+                double mass = 0;
+#pragma unroll
+                for (int q = 0; q < Q; q++) mass += f[q];
+                acc[8] = mass;
+            }
+            if constexpr (MODE == MODE_MATERIALIZE) {
+                if (kp.fhost && !kp.fhost_in) {
+                    const int64_t nn = (int64_t)B.n[0] * B.n[1] * B.n[2];
+                    const int64_t h = B.fbase + ((int64_t)z * B.n[1] + y) * B.n[0] + x;
+#pragma unroll
+                    for (int q = 0; q < Q; q++) kp.fhost[h + q * nn] = f[q];
+                }
+            } else {
+                // BGK collision with the optional first-order force source
+                const double tw = B.tau_omega, om1 = B.one_minus_omega;
+                const bool addf = force_node && kp.force_type == 1;
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const double fe = feq_ref<D, double>(q, rho, uvw[0], uvw[1], uvw[2]);
+                    double fsq;
+                    if (addf) {
+                        double cf = 0;
+#pragma unroll
+                        for (int d = 0; d < D; d++)
+                            cf += kCS * double(cvel(D, q, d)) * B.g[d];
+                        const double F = wgt(D, q) * rho * cf;
+                        fsq = fe + om1 * (f[q] - fe) + tw * F;
+                    } else {
+                        fsq = fe + om1 * (f[q] - fe);
+                    }
+                    next[(int64_t)q * B.sq + node] = fsq;
+                }
+            }
+        } else {
+            // deviation form g = f - w, FP32 arithmetic
+            if (full) {
+#pragma unroll
+                for (int q = 0; q < Q; q++) f[q] -= T(wgt(D, q));
+            }
+            float drho = 0.f, j[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                drho += f[q];
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    if (cvel(D, q, d) > 0) j[d] += f[q];
+                    if (cvel(D, q, d) < 0) j[d] -= f[q];
+                }
+            }
+            const float rho = 1.f + drho;
+            const float inv = 1.f / rho;
+            float ul[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                ul[d] = j[d] * inv;
+                if (kp.force_macros && force_node) ul[d] += B.fhalf_dtg[d];
+            }
+            const float usq = ul[0] * ul[0] + ul[1] * ul[1] + ul[2] * ul[2];
+            if constexpr (MODE == MODE_MATERIALIZE || MODE == MODE_STEP_CHECK) {
+                double dr = 0.0;
+#pragma unroll
+                for (int q = 0; q < Q; q++) dr += double(f[q]);
+                const double m[4] = {1.0 + dr, kCS * double(ul[0]), kCS * double(ul[1]),
+                                     kCS * double(ul[2])};
+#pragma unroll
+                for (int v = 0; v < D + 1; v++) {
+                    if (MODE == MODE_STEP_CHECK || kp.want_residual) {
+                        const double old = kp.macro_copy[v][mnode];
+                        acc[v] = (m[v] - old) * (m[v] - old);
+                        acc[4 + v] = m[v] * m[v];
+                        kp.macro_copy[v][mnode] = m[v];
+                    }
+                    kp.macro[v][mnode] = m[v];
+                }
+                acc[8] = 1.0 + dr;
+            }
+            if constexpr (MODE == MODE_MATERIALIZE) {
+                if (kp.fhost && !kp.fhost_in) {
+                    const int64_t nn = (int64_t)B.n[0] * B.n[1] * B.n[2];
+                    const int64_t h = B.fbase + ((int64_t)z * B.n[1] + y) * B.n[0] + x;
+#pragma unroll
+                    for (int q = 0; q < Q; q++)
+                        kp.fhost[h + q * nn] = double(f[q]) + wgt(D, q);
+                }
+            } else {
+                const float om = B.fomega, om1 = B.fone_minus_omega;
+                const bool addf = force_node && kp.force_type == 1;
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    float cu = 0.f;
+#pragma unroll
+                    for (int d = 0; d < D; d++) {
+                        if (cvel(D, q, d) > 0) cu += ul[d];
+                        if (cvel(D, q, d) < 0) cu -= ul[d];
+                    }
+                    const float wq = float(wgt(D, q));
+                    const float geq =
+                        wq * (drho + rho * (3.f * cu + 4.5f * cu * cu - 1.5f * usq));
+                    float gn = om1 * f[q] + om * geq;
+                    if (addf) {
+                        float cg = 0.f;
+#pragma unroll
+                        for (int d = 0; d < D; d++) cg += float(cvel(D, q, d)) * float(B.g[d]);
+                        gn += B.fforce * wq * rho * cg;
+                    }
+                    st_nt<ST>(next, (int64_t)q * B.sq + node, gn, kp.inv_scale[q]);
+                }
+            }
+        }
+    }
+reduce:
+    if constexpr (MODE == MODE_STEP_CHECK || MODE == MODE_MATERIALIZE)
+        reduce_store(acc, kp.partials + (int64_t)wg * kNAcc);
+}
+
+// fStage halo slab copy of one block connection (Field::TransferHalos ->
+// ops_halo_transfer, field.h:94-99, slabs of field.h:215-395)
+template <typename E>
+__global__ void k_halo(HaloParams hp) {
+    const int64_t ncell = (int64_t)hp.ext[0] * hp.ext[1] * hp.ext[2];
+    const int64_t total = ncell * hp.Q;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int q = (int)(i / ncell);
+        int64_t c = i - (int64_t)q * ncell;
+        const int a = (int)(c % hp.ext[0]);
+        c /= hp.ext[0];
+        const int bb = (int)(c % hp.ext[1]);
+        const int cc = (int)(c / hp.ext[1]);
+        const int64_t cell = i - (int64_t)q * ncell;
+        const E* src = static_cast<const E*>(hp.other_flag && hp.other_flag[cell]
+                                                 ? hp.src_other
+                                                 : hp.src_cur);
+        const int64_t si = hp.src_base + (int64_t)q * hp.src_sq +
+                           (int64_t)(hp.org_src[2] + cc) * hp.src_sz +
+                           (int64_t)(hp.org_src[1] + bb) * hp.src_sx + hp.org_src[0] + a;
+        const int64_t di = hp.dst_base + (int64_t)q * hp.dst_sq +
+                           (int64_t)(hp.org_dst[2] + cc) * hp.dst_sz +
+                           (int64_t)(hp.org_dst[1] + bb) * hp.dst_sx + hp.org_dst[0] + a;
+        static_cast<E*>(hp.dst)[di] = src[si];
+    }
+}
+
+// deterministic sums of per-workgroup partials: out[b*kNAcc + a] per block,
+// out[nblocks*kNAcc + a] over all blocks (block order)
+__global__ void k_reduce_partials(const double* partials, const BlockParams* blocks,
+                                  int nblocks, int ntiles, double* out) {
+    const int a = threadIdx.x;
+    if (a >= kNAcc) return;
+    double total = 0.0;
+    for (int b = 0; b < nblocks; b++) {
+        const int t0 = blocks[b].tile_start;
+        const int t1 = b + 1 < nblocks ? blocks[b + 1].tile_start : ntiles;
+        double s = 0.0;
+        for (int t = t0; t < t1; t++) s += partials[(int64_t)t * kNAcc + a];
+        out[b * kNAcc + a] = s;
+        total += s;
+    }
+    out[nblocks * kNAcc + a] = total;
+}
+
+__global__ void k_fill_bytes(uint32_t* p, int64_t n, uint32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ---- host-side launch helpers ----------------------------------------------------
+template <int D, int ST>
+static hipError_t launch_mode(int mode, const KParams& kp, hipStream_t s) {
+    dim3 blk(kTX, kTY, 1), grd(kp.ntiles, 1, 1);
+    switch (mode) {
+        case MODE_STEP:
+            hipLaunchKernelGGL((k_lattice<D, ST, MODE_STEP>), grd, blk, 0, s, kp);
+            break;
+        case MODE_STEP_CHECK:
+            hipLaunchKernelGGL((k_lattice<D, ST, MODE_STEP_CHECK>), grd, blk, 0, s, kp);
+            break;
+        case MODE_MATERIALIZE:
+            hipLaunchKernelGGL((k_lattice<D, ST, MODE_MATERIALIZE>), grd, blk, 0, s, kp);
+            break;
+        case MODE_INIT_FEQ:
+            hipLaunchKernelGGL((k_lattice<D, ST, MODE_INIT_FEQ>), grd, blk, 0, s, kp);
+            break;
+        case MODE_INIT_F:
+            hipLaunchKernelGGL((k_lattice<D, ST, MODE_INIT_F>), grd, blk, 0, s, kp);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_lattice(int dim, int storage, int mode, const KParams& kp,
+                          hipStream_t s) {
+    if (dim == 2) {
+        if (storage == MPLBHIP_STORE_F64) return launch_mode<2, MPLBHIP_STORE_F64>(mode, kp, s);
+        if (storage == MPLBHIP_STORE_F32) return launch_mode<2, MPLBHIP_STORE_F32>(mode, kp, s);
+        return launch_mode<2, MPLBHIP_STORE_F16>(mode, kp, s);
+    }
+    if (storage == MPLBHIP_STORE_F64) return launch_mode<3, MPLBHIP_STORE_F64>(mode, kp, s);
+    if (storage == MPLBHIP_STORE_F32) return launch_mode<3, MPLBHIP_STORE_F32>(mode, kp, s);
+    return launch_mode<3, MPLBHIP_STORE_F16>(mode, kp, s);
+}
+
+hipError_t launch_halo(const HaloParams& hp, hipStream_t s) {
+    const int64_t total = (int64_t)hp.ext[0] * hp.ext[1] * hp.ext[2] * hp.Q;
+    int grid = (int)((total + 255) / 256);
+    if (grid > 4096) grid = 4096;
+    if (grid < 1) grid = 1;
+    if (hp.elem_bytes == 8)
+        hipLaunchKernelGGL(k_halo<double>, dim3(grid), dim3(256), 0, s, hp);
+    else if (hp.elem_bytes == 4)
+        hipLaunchKernelGGL(k_halo<float>, dim3(grid), dim3(256), 0, s, hp);
+    else
+        hipLaunchKernelGGL(k_halo<__half>, dim3(grid), dim3(256), 0, s, hp);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(const double* partials, const BlockParams* blocks, int nblocks,
+                         int ntiles, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partials, blocks,
+                       nblocks, ntiles, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(void* p, int64_t bytes, hipStream_t s) {
+    const int64_t n = bytes / 4;
+    int grid = (int)((n + 255) / 256);
+    if (grid > 8192) grid = 8192;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_fill_bytes, dim3(grid), dim3(256), 0, s,
+                       static_cast<uint32_t*>(p), n, 0u);
+    return hipGetLastError();
+}
+
+}  // namespace mplbhip

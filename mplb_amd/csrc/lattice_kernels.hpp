@@ -1,0 +1,80 @@
+// lattice_kernels.hpp -- parameter blocks shared by the host runtime and the
+// gfx950 kernels.
+#pragma once
+#include <cstdint>
+
+#include "plan.hpp"
+
+namespace mplbhip {
+
+constexpr int kTX = 64;   // one wavefront along x
+constexpr int kTY = 4;    // four wavefronts per workgroup along y
+constexpr int kNAcc = 9;  // residual: 4 squared differences, 4 squares, mass
+
+// One block as laid out in HBM: every population is a 3-D array with a one
+// node halo on each axis (no z halo in 2D), rows padded so that x = 0 is
+// aligned; populations follow each other at pitch sq.
+struct BlockParams {
+    int32_t n[3];
+    int32_t sx;          // row pitch (elements)
+    int32_t sz;          // plane pitch
+    int32_t sq;          // population pitch
+    int64_t base;        // element offset of node (0,0,0), population 0
+    int32_t tiles_x, tiles_y;
+    int32_t tile_start;  // first logical workgroup of this block
+    int32_t prog_base;   // first region program of this block
+    int64_t mbase;       // offset of this block in the macro arrays
+    int64_t fbase;       // offset of this block in host-layout f arrays
+    int64_t persist_base;
+    double g[3];         // acceleration (code units)
+    double tau, omega, one_minus_omega, tau_omega;
+    float fomega, fone_minus_omega, fforce, fhalf_dtg[3];
+    int32_t active;      // batched runs: 0 freezes a converged block
+    int32_t pad;
+};
+
+enum Mode : int {
+    MODE_STEP = 0,        // gather, boundary, collide -> fs_next
+    MODE_STEP_CHECK = 1,  // as STEP, plus macros and residual partials
+    MODE_MATERIALIZE = 2, // gather, boundary -> f (host layout), macros
+    MODE_INIT_FEQ = 3,    // f = feq(macros) -> collide -> fs_next (and f
+                          // to fhost when given)
+    MODE_INIT_F = 4       // f from host-layout array -> collide -> fs_next
+};
+
+struct KParams {
+    const BlockParams* blocks;
+    int32_t nblocks;
+    int32_t ntiles;
+    const RegionProg* progs;
+    const void* fs_prev;
+    void* fs_next;
+    const void* per_prev;
+    void* per_next;
+    double* macro[4];       // MacroVars (current values, host layout)
+    double* macro_copy[4];  // MacroVarsCopy (residual reference)
+    double* fhost;      // materialize output / init input, host layout
+    double* partials;   // per logical workgroup: kNAcc doubles
+    float scale[kMaxQ];     // FP16 storage: g = h * scale[q]
+    float inv_scale[kMaxQ];
+    int32_t force_type;     // 0 none, 1 first order
+    int32_t force_macros;   // *_Force velocity kernels
+    int32_t fhost_in;       // MATERIALIZE: take f from fhost instead of streaming
+    int32_t want_residual;  // MATERIALIZE: accumulate the residual as CHECK does
+    double dt;
+};
+
+struct HaloParams {
+    const void* src_cur;    // buffer being exchanged (current step)
+    const void* src_other;  // the other ping-pong buffer
+    void* dst;
+    int64_t src_base, dst_base;  // node (0,0,0) population 0 of each block
+    int32_t src_sx, src_sz, src_sq;
+    int32_t dst_sx, dst_sz, dst_sq;
+    int32_t org_src[3], org_dst[3], ext[3];
+    int32_t Q;
+    int32_t elem_bytes;
+    const uint8_t* other_flag;  // per slab cell: 1 = read src_other
+};
+
+}  // namespace mplbhip

@@ -1,0 +1,888 @@
+// mplbhip.cpp -- context, HBM layout, launches and the C ABI of
+// libmplbhip.so (include/mplbhip.h).
+#include "../../include/mplbhip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lattice_kernels.hpp"
+#include "plan.hpp"
+
+namespace mplbhip {
+hipError_t launch_lattice(int dim, int storage, int mode, const KParams& kp,
+                          hipStream_t s);
+hipError_t launch_halo(const HaloParams& hp, hipStream_t s);
+hipError_t launch_reduce(const double* partials, const BlockParams* blocks,
+                         int nblocks, int ntiles, double* out, hipStream_t s);
+hipError_t launch_fill(void* p, int64_t bytes, hipStream_t s);
+}  // namespace mplbhip
+
+using namespace mplbhip;
+
+namespace {
+
+struct HaloLink {
+    HaloParams hp;       // src/dst buffers filled per launch
+    int from, to;
+    uint8_t* d_flag = nullptr;
+};
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct mplbhip_ctx {
+    int device = -1;
+    int dim = 3;
+    int storage = MPLBHIP_STORE_F64;
+    int Q = 19;
+    Plan plan;
+    double tau = 1.0, dt = 1.0;
+    int force_type = 0, force_macros = 0;
+    std::vector<std::array<double, 3>> accel;
+    bool prepared = false;
+    bool initialized = false;
+    bool init_from_f = false;
+    long long tstep = 0;
+    int cur = 0;  // buffer holding the latest fStage
+    std::string err;
+
+    // layout
+    std::vector<BlockParams> bp;
+    int64_t total_elems = 0;
+    int64_t interior_nodes = 0;
+    int ntiles = 0;
+    size_t es = 8;
+
+    // device memory
+    hipStream_t stream = nullptr;
+    DevBuf fs[2], per[2], macro[4], macro_copy[4], progs, blocks, partials,
+        red, fhost;
+    std::vector<HaloLink> halos;
+    std::vector<double> host_init_f;  // f given by mplbhip_set_f, per block
+    std::vector<float> scale, inv_scale;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int graph_steps = 0;
+};
+
+static int fail(mplbhip_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                      \
+    do {                                                                     \
+        hipError_t e_ = (expr);                                              \
+        if (e_ != hipSuccess)                                                \
+            return fail((c), MPLBHIP_ERR_HIP, "%s failed: %s", #expr,        \
+                        hipGetErrorString(e_));                              \
+    } while (0)
+
+static int need_device(mplbhip_ctx* c) {
+    if (c->device < 0)
+        return fail(c, MPLBHIP_ERR_NODEVICE,
+                    "this context was created without a device (plan only)");
+    return 0;
+}
+
+static int dalloc(mplbhip_ctx* c, DevBuf& b, size_t bytes) {
+    if (b.p) {
+        (void)hipFree(b.p);
+        b.p = nullptr;
+    }
+    b.bytes = bytes;
+    if (bytes == 0) return 0;
+    HIPCHK(c, hipMalloc(&b.p, bytes));
+    return 0;
+}
+
+static void dfree(DevBuf& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+static double kCSh() { return std::sqrt(3.0); }
+
+// collision constants per block, computed in the reference's order
+// (model_kernel.inc:428-429: omega = dt/(tau + 0.5*dt))
+static void fill_collision(mplbhip_ctx* c) {
+    const double tau = c->tau, dt = c->dt;
+    const double omega = dt / (tau + 0.5 * dt);
+    for (size_t b = 0; b < c->bp.size(); b++) {
+        BlockParams& B = c->bp[b];
+        B.tau = tau;
+        B.omega = omega;
+        B.one_minus_omega = 1 - omega;
+        B.tau_omega = tau * omega;
+        B.fomega = (float)omega;
+        B.fone_minus_omega = (float)(1 - omega);
+        B.fforce = (float)(tau * omega * kCSh());
+        for (int d = 0; d < 3; d++) {
+            B.g[d] = c->accel[b][d];
+            // half-step shift of the lattice-unit velocity: (dt g/2)/CS
+            B.fhalf_dtg[d] = (float)((dt * c->accel[b][d] / 2) / kCSh());
+        }
+    }
+}
+
+static int upload_blocks(mplbhip_ctx* c) {
+    fill_collision(c);
+    HIPCHK(c, hipMemcpyAsync(c->blocks.p, c->bp.data(),
+                             c->bp.size() * sizeof(BlockParams),
+                             hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+static void drop_graphs(mplbhip_ctx* c) {
+    for (int i = 0; i < 2; i++) {
+        if (c->gexec[i]) (void)hipGraphExecDestroy(c->gexec[i]);
+        if (c->graph[i]) (void)hipGraphDestroy(c->graph[i]);
+        c->gexec[i] = nullptr;
+        c->graph[i] = nullptr;
+    }
+    c->graph_steps = 0;
+}
+
+static KParams base_params(mplbhip_ctx* c) {
+    KParams kp;
+    std::memset(&kp, 0, sizeof kp);
+    kp.blocks = static_cast<const BlockParams*>(c->blocks.p);
+    kp.nblocks = (int)c->bp.size();
+    kp.ntiles = c->ntiles;
+    kp.progs = static_cast<const RegionProg*>(c->progs.p);
+    for (int v = 0; v < 4; v++) {
+        kp.macro[v] = static_cast<double*>(c->macro[v].p);
+        kp.macro_copy[v] = static_cast<double*>(c->macro_copy[v].p);
+    }
+    kp.partials = static_cast<double*>(c->partials.p);
+    for (int q = 0; q < kMaxQ; q++) {
+        kp.scale[q] = q < c->Q ? c->scale[q] : 1.f;
+        kp.inv_scale[q] = q < c->Q ? c->inv_scale[q] : 1.f;
+    }
+    kp.force_type = c->force_type;
+    kp.force_macros = c->force_macros;
+    kp.dt = c->dt;
+    return kp;
+}
+
+// fStage halo exchange into buffer `into` (ops_halo_transfer order)
+static int exchange(mplbhip_ctx* c, int into) {
+    for (HaloLink& h : c->halos) {
+        HaloParams hp = h.hp;
+        hp.src_cur = c->fs[into].p;
+        hp.src_other = c->fs[1 - into].p;
+        hp.dst = c->fs[into].p;
+        hp.other_flag = h.d_flag;
+        HIPCHK(c, launch_halo(hp, c->stream));
+    }
+    return 0;
+}
+
+// one step: fused kernel from buffer cur into 1-cur, then halos
+static int enqueue_step(mplbhip_ctx* c, int from, bool check) {
+    KParams kp = base_params(c);
+    kp.fs_prev = c->fs[from].p;
+    kp.fs_next = c->fs[1 - from].p;
+    kp.per_prev = c->per[from].p;
+    kp.per_next = c->per[1 - from].p;
+    HIPCHK(c, launch_lattice(c->dim, c->storage,
+                             check ? MODE_STEP_CHECK : MODE_STEP, kp, c->stream));
+    if (int rc = exchange(c, 1 - from)) return rc;
+    return 0;
+}
+
+static int build_graphs(mplbhip_ctx* c, int nsteps) {
+    drop_graphs(c);
+    for (int p = 0; p < 2; p++) {
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int from = p, rc = 0;
+        for (int s = 0; s < nsteps && rc == 0; s++) {
+            rc = enqueue_step(c, from, false);
+            from = 1 - from;
+        }
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess)
+            return fail(c, MPLBHIP_ERR_HIP, "graph capture failed: %s",
+                        hipGetErrorString(e));
+        c->graph[p] = g;
+        HIPCHK(c, hipGraphInstantiate(&c->gexec[p], g, nullptr, nullptr, 0));
+    }
+    c->graph_steps = nsteps;
+    return 0;
+}
+
+static int ensure_init(mplbhip_ctx* c) {
+    if (!c->prepared) return fail(c, MPLBHIP_ERR_ARG, "call mplbhip_prepare first");
+    if (!c->initialized)
+        return fail(c, MPLBHIP_ERR_ARG,
+                    "no initial condition: call mplbhip_init_feq or mplbhip_set_f");
+    return 0;
+}
+
+// materialize the current f: macros always, optionally f in host layout
+// (device buffer c->fhost) and the residual against MacroVarsCopy
+static int materialize(mplbhip_ctx* c, bool want_f, bool residual,
+                       std::vector<double>* acc_out) {
+    if (int rc = ensure_init(c)) return rc;
+    KParams kp = base_params(c);
+    const size_t fbytes = (size_t)c->Q * c->interior_nodes * sizeof(double);
+    if ((want_f || c->tstep == 0) && c->fhost.bytes != fbytes) {
+        if (int rc = dalloc(c, c->fhost, fbytes)) return rc;
+    }
+    if (c->tstep == 0) {
+        // f^0 is not reachable from a previous fStage: recompute it
+        if (c->init_from_f) {
+            HIPCHK(c, hipMemcpyAsync(c->fhost.p, c->host_init_f.data(), fbytes,
+                                     hipMemcpyHostToDevice, c->stream));
+        } else {
+            KParams ki = base_params(c);
+            ki.fs_next = c->fs[c->cur].p;
+            ki.per_next = c->per[c->cur].p;
+            ki.fhost = static_cast<double*>(c->fhost.p);
+            HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_INIT_FEQ, ki, c->stream));
+        }
+        kp.fhost = static_cast<double*>(c->fhost.p);
+        kp.fhost_in = 1;
+    } else {
+        kp.fs_prev = c->fs[1 - c->cur].p;
+        kp.per_prev = c->per[1 - c->cur].p;
+        kp.fhost = want_f ? static_cast<double*>(c->fhost.p) : nullptr;
+    }
+    kp.want_residual = residual ? 1 : 0;
+    HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_MATERIALIZE, kp, c->stream));
+    if (acc_out) {
+        HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
+                                static_cast<const BlockParams*>(c->blocks.p),
+                                (int)c->bp.size(), c->ntiles,
+                                static_cast<double*>(c->red.p), c->stream));
+        acc_out->resize((c->bp.size() + 1) * kNAcc);
+        HIPCHK(c, hipMemcpyAsync(acc_out->data(), c->red.p,
+                                 acc_out->size() * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* mplbhip_version(void) {
+    return "mplbhip 0.1 gfx950";
+}
+
+int mplbhip_create(mplbhip_ctx** out, int device, int space_dim, int storage) {
+    if (!out) return MPLBHIP_ERR_ARG;
+    *out = nullptr;
+    if (space_dim != 2 && space_dim != 3) return MPLBHIP_ERR_ARG;
+    if (storage < MPLBHIP_STORE_F64 || storage > MPLBHIP_STORE_F16)
+        return MPLBHIP_ERR_ARG;
+    mplbhip_ctx* c = new mplbhip_ctx();
+    c->device = device;
+    c->dim = space_dim;
+    c->storage = storage;
+    c->Q = space_dim == 2 ? 9 : 19;
+    c->plan.dim = space_dim;
+    c->es = storage == MPLBHIP_STORE_F64 ? 8 : (storage == MPLBHIP_STORE_F32 ? 4 : 2);
+    c->scale.assign(c->Q, 1.f);
+    c->inv_scale.assign(c->Q, 1.f);
+    if (storage == MPLBHIP_STORE_F16) {
+        // per-population scale: g/w_i is O(1e-1) for |u| ~ 0.05, so
+        // s_i = w_i/2^10 keeps h = g/s_i in the normal FP16 range with ~11
+        // significant bits
+        for (int q = 0; q < c->Q; q++) {
+            const double w = space_dim == 2 ? kD2Q9W[q] : kD3Q19W[q];
+            c->scale[q] = (float)(w / 1024.0);
+            c->inv_scale[q] = (float)(1024.0 / w);
+        }
+    }
+    if (device >= 0) {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || device >= n) {
+            delete c;
+            return MPLBHIP_ERR_NODEVICE;
+        }
+        if (hipSetDevice(device) != hipSuccess ||
+            hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreate(&c->ev[0]) != hipSuccess ||
+            hipEventCreate(&c->ev[1]) != hipSuccess) {
+            delete c;
+            return MPLBHIP_ERR_HIP;
+        }
+    }
+    *out = c;
+    return 0;
+}
+
+void mplbhip_destroy(mplbhip_ctx* c) {
+    if (!c) return;
+    if (c->device >= 0) {
+        (void)hipSetDevice(c->device);
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        drop_graphs(c);
+        for (int i = 0; i < 2; i++) {
+            dfree(c->fs[i]);
+            dfree(c->per[i]);
+        }
+        for (int v = 0; v < 4; v++) {
+            dfree(c->macro[v]);
+            dfree(c->macro_copy[v]);
+        }
+        dfree(c->progs);
+        dfree(c->blocks);
+        dfree(c->partials);
+        dfree(c->red);
+        dfree(c->fhost);
+        for (HaloLink& h : c->halos)
+            if (h.d_flag) (void)hipFree(h.d_flag);
+        for (int i = 0; i < 2; i++)
+            if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+}
+
+const char* mplbhip_last_error(const mplbhip_ctx* c) {
+    return c ? c->err.c_str() : "null context";
+}
+
+int mplbhip_define_block(mplbhip_ctx* c, int id, const int* size) {
+    if (!c || !size) return MPLBHIP_ERR_ARG;
+    if (c->prepared) return fail(c, MPLBHIP_ERR_ARG, "case already prepared");
+    if (id != (int)c->plan.blocks.size())
+        return fail(c, MPLBHIP_ERR_ARG, "blocks must be defined in id order");
+    if ((int)c->plan.blocks.size() >= kMaxBlocks)
+        return fail(c, MPLBHIP_ERR_ARG, "too many blocks");
+    BlockDef b;
+    for (int a = 0; a < 3; a++) b.n[a] = a < c->dim ? size[a] : 1;
+    for (int a = 0; a < c->dim; a++)
+        if (b.n[a] < 3)
+            return fail(c, MPLBHIP_ERR_ARG, "block %d axis %d has %d < 3 nodes",
+                        id, a, b.n[a]);
+    c->plan.blocks.push_back(b);
+    c->accel.push_back({0.0, 0.0, 0.0});
+    return 0;
+}
+
+int mplbhip_define_connection(mplbhip_ctx* c, int from, int fsurf, int to,
+                              int tsurf, int vtype) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    if (c->prepared) return fail(c, MPLBHIP_ERR_ARG, "case already prepared");
+    const int nb = (int)c->plan.blocks.size();
+    if (from < 0 || from >= nb || to < 0 || to >= nb)
+        return fail(c, MPLBHIP_ERR_ARG, "connection names an undefined block");
+    for (const LinkDesc& l : c->plan.links)
+        if (l.from == from && l.fsurf == fsurf)
+            return fail(c, MPLBHIP_ERR_ARG,
+                        "a neighbour is already defined for block %d surface %d",
+                        from, fsurf);
+    c->plan.links.push_back(LinkDesc{from, fsurf, to, tsurf, vtype});
+    return 0;
+}
+
+int mplbhip_define_collision(mplbhip_ctx* c, double tau_ref, int force_type,
+                             int force_macros) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    if (force_type != 0 && force_type != 1)
+        return fail(c, MPLBHIP_ERR_UNSUPPORTED,
+                    "body force type %d: only BodyForce_None and BodyForce_1st",
+                    force_type);
+    c->tau = tau_ref;
+    c->force_type = force_type;
+    c->force_macros = force_macros ? 1 : 0;
+    if (c->prepared && c->device >= 0) {
+        drop_graphs(c);
+        return upload_blocks(c);
+    }
+    return 0;
+}
+
+int mplbhip_define_boundary(mplbhip_ctx* c, int block, int surface, int scheme,
+                            const double* vars, int nvars, int vtype) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    if (c->prepared) return fail(c, MPLBHIP_ERR_ARG, "case already prepared");
+    if (block < 0 || block >= (int)c->plan.blocks.size())
+        return fail(c, MPLBHIP_ERR_ARG, "boundary on undefined block %d", block);
+    if (nvars < 0 || nvars > 8 || (nvars > 0 && !vars))
+        return fail(c, MPLBHIP_ERR_ARG, "bad given variables");
+    if (scheme != SCH_EQMDiffuseRefl && scheme != SCH_ExtrapolPressure1ST &&
+        scheme != SCH_None && scheme != SCH_MDPeriodic)
+        return fail(c, MPLBHIP_ERR_UNSUPPORTED,
+                    "boundary scheme %d is not implemented", scheme);
+    if (scheme == SCH_EQMDiffuseRefl && nvars < c->dim)
+        return fail(c, MPLBHIP_ERR_ARG, "EQMDiffuseRefl needs the wall velocity");
+    if (scheme == SCH_ExtrapolPressure1ST && nvars < 1)
+        return fail(c, MPLBHIP_ERR_ARG, "ExtrapolPressure1ST needs the density");
+    BCDesc d;
+    std::memset(&d, 0, sizeof d);
+    d.block = block;
+    d.surface = surface;
+    d.scheme = scheme;
+    d.vtype = vtype;
+    d.nvars = nvars;
+    for (int i = 0; i < nvars; i++) d.vars[i] = vars[i];
+    c->plan.bcs.push_back(d);
+    return 0;
+}
+
+static int setup_halos(mplbhip_ctx* c) {
+    // ops_halo_transfer order: blocks by id, then surfaces Left, Right, Top,
+    // Bottom, Front, Back (std::map<BoundarySurface> order); edge surfaces
+    // declare no halo (field.h:395 default).
+    const int order[6] = {BS_Left, BS_Right, BS_Top, BS_Bottom, BS_Front, BS_Back};
+    std::vector<LinkDesc> seq;
+    for (int b = 0; b < (int)c->plan.blocks.size(); b++)
+        for (int oi = 0; oi < 6; oi++)
+            for (const LinkDesc& l : c->plan.links)
+                if (l.from == b && l.fsurf == order[oi]) seq.push_back(l);
+    const int hz = c->dim == 3 ? 1 : 0;
+    struct Slab { int b; int org[3]; int ext[3]; };
+    std::vector<Slab> src(seq.size()), dst(seq.size());
+    for (size_t i = 0; i < seq.size(); i++) {
+        const LinkDesc& l = seq[i];
+        const BlockDef& A = c->plan.blocks[l.from];
+        const BlockDef& T = c->plan.blocks[l.to];
+        const int ax = (l.fsurf == BS_Left || l.fsurf == BS_Right) ? 0
+                     : (l.fsurf == BS_Top || l.fsurf == BS_Bottom) ? 1 : 2;
+        if (ax == 2 && c->dim == 2)
+            return fail(c, MPLBHIP_ERR_ARG, "Front/Back connection in 2D");
+        for (int d = 0; d < c->dim; d++)
+            if (d != ax && A.n[d] != T.n[d])
+                return fail(c, MPLBHIP_ERR_ARG,
+                            "connected blocks %d and %d differ across the face",
+                            l.from, l.to);
+        Slab s{l.from, {-1, -1, -hz}, {A.n[0] + 2, A.n[1] + 2, A.n[2] + 2 * hz}};
+        Slab d{l.to, {-1, -1, -hz}, {A.n[0] + 2, A.n[1] + 2, A.n[2] + 2 * hz}};
+        switch (l.fsurf) {
+            case BS_Right: s.org[0] = A.n[0] - 1; d.org[0] = -1; break;
+            case BS_Left: s.org[0] = 0; d.org[0] = T.n[0]; break;
+            case BS_Bottom: s.org[1] = 0; d.org[1] = T.n[1]; break;
+            case BS_Top: s.org[1] = A.n[1] - 1; d.org[1] = -1; break;
+            case BS_Back: s.org[2] = 0; d.org[2] = T.n[2]; break;
+            case BS_Front: s.org[2] = A.n[2] - 1; d.org[2] = -1; break;
+            default: break;
+        }
+        s.ext[ax] = 1;
+        d.ext[ax] = 1;
+        src[i] = s;
+        dst[i] = d;
+    }
+    // Source cells outside the source block's interior are halo cells; the
+    // reference reads them as last written.  Written earlier in this exchange
+    // -> current buffer; written only later -> the previous exchange, which
+    // went to the other ping-pong buffer; never written -> zero in both.
+    auto covers = [&](const Slab& s, int b, int x, int y, int z) {
+        return s.b == b && x >= s.org[0] && x < s.org[0] + s.ext[0] &&
+               y >= s.org[1] && y < s.org[1] + s.ext[1] && z >= s.org[2] &&
+               z < s.org[2] + s.ext[2];
+    };
+    c->halos.clear();
+    for (size_t i = 0; i < seq.size(); i++) {
+        const Slab& s = src[i];
+        const BlockDef& A = c->plan.blocks[s.b];
+        const int64_t ncell = (int64_t)s.ext[0] * s.ext[1] * s.ext[2];
+        std::vector<uint8_t> flag(ncell, 0);
+        bool anyother = false;
+        for (int zz = 0; zz < s.ext[2]; zz++)
+            for (int yy = 0; yy < s.ext[1]; yy++)
+                for (int xx = 0; xx < s.ext[0]; xx++) {
+                    const int x = s.org[0] + xx, y = s.org[1] + yy, z = s.org[2] + zz;
+                    const bool interior = x >= 0 && x < A.n[0] && y >= 0 &&
+                                          y < A.n[1] && z >= 0 && z < A.n[2];
+                    if (interior) continue;
+                    bool before = false, after = false;
+                    for (size_t j = 0; j < seq.size(); j++)
+                        if (covers(dst[j], s.b, x, y, z)) {
+                            if (j < i) before = true; else after = true;
+                        }
+                    if (!before && after) {
+                        flag[((int64_t)zz * s.ext[1] + yy) * s.ext[0] + xx] = 1;
+                        anyother = true;
+                    }
+                }
+        HaloLink h;
+        std::memset(&h.hp, 0, sizeof h.hp);
+        h.from = seq[i].from;
+        h.to = seq[i].to;
+        const BlockParams& SB = c->bp[s.b];
+        const BlockParams& DB = c->bp[dst[i].b];
+        h.hp.src_base = SB.base;
+        h.hp.dst_base = DB.base;
+        h.hp.src_sx = SB.sx; h.hp.src_sz = SB.sz; h.hp.src_sq = SB.sq;
+        h.hp.dst_sx = DB.sx; h.hp.dst_sz = DB.sz; h.hp.dst_sq = DB.sq;
+        for (int a = 0; a < 3; a++) {
+            h.hp.org_src[a] = s.org[a];
+            h.hp.org_dst[a] = dst[i].org[a];
+            h.hp.ext[a] = s.ext[a];
+        }
+        h.hp.Q = c->Q;
+        h.hp.elem_bytes = (int)c->es;
+        if (anyother && c->device >= 0) {
+            HIPCHK(c, hipMalloc(&h.d_flag, ncell));
+            HIPCHK(c, hipMemcpy(h.d_flag, flag.data(), ncell, hipMemcpyHostToDevice));
+        }
+        c->halos.push_back(h);
+    }
+    return 0;
+}
+
+int mplbhip_prepare(mplbhip_ctx* c) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    if (c->prepared) return fail(c, MPLBHIP_ERR_ARG, "case already prepared");
+    if (c->plan.blocks.empty()) return fail(c, MPLBHIP_ERR_ARG, "no blocks defined");
+    if (!c->plan.build()) return fail(c, MPLBHIP_ERR_UNSUPPORTED, "%s", c->plan.error.c_str());
+    // HBM layout: one halo node per axis, x = 0 aligned to 64 bytes
+    const int nb = (int)c->plan.blocks.size();
+    const int hz = c->dim == 3 ? 1 : 0;
+    const int align = (int)(64 / c->es);
+    c->bp.assign(nb, BlockParams{});
+    int64_t off = 0, moff = 0;
+    int tiles = 0;
+    for (int b = 0; b < nb; b++) {
+        const int* n = c->plan.blocks[b].n;
+        BlockParams& B = c->bp[b];
+        std::memset(&B, 0, sizeof B);
+        for (int a = 0; a < 3; a++) B.n[a] = n[a];
+        const int ox = align;
+        const int64_t sx = ((int64_t)ox + n[0] + 1 + align - 1) / align * align;
+        const int64_t sz = sx * (n[1] + 2);
+        int64_t sq = sz * (n[2] + 2 * hz);
+        sq = (sq + 63) / 64 * 64;
+        if ((int64_t)c->Q * sq >= (1LL << 31))
+            return fail(c, MPLBHIP_ERR_UNSUPPORTED, "block %d too large", b);
+        B.sx = (int32_t)sx;
+        B.sz = (int32_t)sz;
+        B.sq = (int32_t)sq;
+        B.base = off + hz * sz + sx + ox;
+        off += ((int64_t)c->Q * sq + 255) / 256 * 256;
+        B.tiles_x = (n[0] + kTX - 1) / kTX;
+        B.tiles_y = (n[1] + kTY - 1) / kTY;
+        B.tile_start = tiles;
+        tiles += B.tiles_x * B.tiles_y * n[2];
+        B.prog_base = b * c->plan.nregions;
+        B.mbase = moff;
+        B.fbase = moff * c->Q;
+        moff += (int64_t)n[0] * n[1] * n[2];
+        B.active = 1;
+    }
+    c->total_elems = off;
+    c->interior_nodes = moff;
+    c->ntiles = tiles;
+    if (int rc = setup_halos(c)) return rc;
+    c->prepared = true;
+    if (c->device < 0) return 0;
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t tbytes = (size_t)c->total_elems * c->es;
+    const size_t tsz = c->storage == MPLBHIP_STORE_F64 ? 8 : 4;
+    for (int i = 0; i < 2; i++) {
+        if (int rc = dalloc(c, c->fs[i], tbytes)) return rc;
+        if (int rc = dalloc(c, c->per[i],
+                            std::max<size_t>(64, c->plan.persist_total * c->Q * tsz)))
+            return rc;
+        HIPCHK(c, launch_fill(c->fs[i].p, (int64_t)tbytes, c->stream));
+        HIPCHK(c, launch_fill(c->per[i].p, (int64_t)c->per[i].bytes, c->stream));
+    }
+    for (int v = 0; v < 4; v++) {
+        const size_t mb = (size_t)c->interior_nodes * sizeof(double);
+        if (int rc = dalloc(c, c->macro[v], mb)) return rc;
+        if (int rc = dalloc(c, c->macro_copy[v], mb)) return rc;
+        HIPCHK(c, launch_fill(c->macro[v].p, (int64_t)mb, c->stream));
+        HIPCHK(c, launch_fill(c->macro_copy[v].p, (int64_t)mb, c->stream));
+    }
+    if (int rc = dalloc(c, c->progs, c->plan.progs.size() * sizeof(RegionProg))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->progs.p, c->plan.progs.data(), c->progs.bytes,
+                             hipMemcpyHostToDevice, c->stream));
+    if (int rc = dalloc(c, c->blocks, nb * sizeof(BlockParams))) return rc;
+    if (int rc = dalloc(c, c->partials, (size_t)tiles * kNAcc * sizeof(double))) return rc;
+    if (int rc = dalloc(c, c->red, (size_t)(nb + 1) * kNAcc * sizeof(double))) return rc;
+    if (int rc = upload_blocks(c)) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mplbhip_node_info(mplbhip_ctx* c, int block, int which, int* out) {
+    if (!c || !out) return MPLBHIP_ERR_ARG;
+    if (!c->prepared) return fail(c, MPLBHIP_ERR_ARG, "call mplbhip_prepare first");
+    if (block < 0 || block >= (int)c->plan.blocks.size())
+        return fail(c, MPLBHIP_ERR_ARG, "undefined block");
+    const int* n = c->plan.blocks[block].n;
+    const int R = c->plan.nregions;
+    size_t p = 0;
+    for (int k = 0; k < n[2]; k++)
+        for (int j = 0; j < n[1]; j++)
+            for (int i = 0; i < n[0]; i++) {
+                const int r = c->plan.region_of(block, i, j, k);
+                out[p++] = which == 0 ? c->plan.region_vt[(size_t)block * R + r]
+                                      : c->plan.region_vg[(size_t)block * R + r];
+            }
+    return 0;
+}
+
+int mplbhip_set_macros(mplbhip_ctx* c, int block, const double* rho,
+                       const double* u, const double* v, const double* w) {
+    if (!c || !rho || !u || !v || (c->dim == 3 && !w)) return MPLBHIP_ERR_ARG;
+    if (int rc = need_device(c)) return rc;
+    if (!c->prepared) return fail(c, MPLBHIP_ERR_ARG, "call mplbhip_prepare first");
+    if (block < 0 || block >= (int)c->bp.size())
+        return fail(c, MPLBHIP_ERR_ARG, "undefined block");
+    const BlockParams& B = c->bp[block];
+    const size_t nn = (size_t)B.n[0] * B.n[1] * B.n[2];
+    const double* src[4] = {rho, u, v, w};
+    for (int var = 0; var < 1 + c->dim; var++) {
+        double* dst = static_cast<double*>(c->macro[var].p) + B.mbase;
+        HIPCHK(c, hipMemcpyAsync(dst, src[var], nn * sizeof(double),
+                                 hipMemcpyHostToDevice, c->stream));
+        double* dc = static_cast<double*>(c->macro_copy[var].p) + B.mbase;
+        HIPCHK(c, hipMemcpyAsync(dc, src[var], nn * sizeof(double),
+                                 hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mplbhip_init_feq(mplbhip_ctx* c) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    if (int rc = need_device(c)) return rc;
+    if (!c->prepared) return fail(c, MPLBHIP_ERR_ARG, "call mplbhip_prepare first");
+    drop_graphs(c);
+    c->cur = 0;
+    c->tstep = 0;
+    c->init_from_f = false;
+    c->initialized = true;
+    // fStage^0 = collide(feq) is formed lazily at the first step, once the
+    // time step is known (the Apps call SetTimeStep after this)
+    return 0;
+}
+
+int mplbhip_set_f(mplbhip_ctx* c, int block, const double* f) {
+    if (!c || !f) return MPLBHIP_ERR_ARG;
+    if (int rc = need_device(c)) return rc;
+    if (!c->prepared) return fail(c, MPLBHIP_ERR_ARG, "call mplbhip_prepare first");
+    if (block < 0 || block >= (int)c->bp.size())
+        return fail(c, MPLBHIP_ERR_ARG, "undefined block");
+    const BlockParams& B = c->bp[block];
+    const size_t nn = (size_t)B.n[0] * B.n[1] * B.n[2];
+    if (c->host_init_f.size() != (size_t)c->Q * c->interior_nodes)
+        c->host_init_f.assign((size_t)c->Q * c->interior_nodes, 0.0);
+    std::memcpy(c->host_init_f.data() + B.fbase, f, nn * c->Q * sizeof(double));
+    drop_graphs(c);
+    c->cur = 0;
+    c->tstep = 0;
+    c->init_from_f = true;
+    c->initialized = true;
+    return 0;
+}
+
+int mplbhip_set_timestep(mplbhip_ctx* c, double dt) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    c->dt = dt;
+    if (c->prepared && c->device >= 0) {
+        drop_graphs(c);
+        return upload_blocks(c);
+    }
+    return 0;
+}
+
+int mplbhip_set_accel(mplbhip_ctx* c, int block, const double* g) {
+    if (!c || !g) return MPLBHIP_ERR_ARG;
+    if (block < 0 || block >= (int)c->accel.size())
+        return fail(c, MPLBHIP_ERR_ARG, "undefined block");
+    for (int d = 0; d < 3; d++) c->accel[block][d] = d < c->dim ? g[d] : 0.0;
+    if (c->prepared && c->device >= 0) {
+        drop_graphs(c);
+        return upload_blocks(c);
+    }
+    return 0;
+}
+
+// the collision of f^0 happens here, with the time step then in force
+static int first_collision(mplbhip_ctx* c) {
+    KParams kp = base_params(c);
+    kp.fs_next = c->fs[0].p;
+    kp.per_next = c->per[0].p;
+    int mode = MODE_INIT_FEQ;
+    if (c->init_from_f) {
+        const size_t fbytes = (size_t)c->Q * c->interior_nodes * sizeof(double);
+        if (c->fhost.bytes != fbytes)
+            if (int rc = dalloc(c, c->fhost, fbytes)) return rc;
+        HIPCHK(c, hipMemcpyAsync(c->fhost.p, c->host_init_f.data(), fbytes,
+                                 hipMemcpyHostToDevice, c->stream));
+        kp.fhost = static_cast<double*>(c->fhost.p);
+        mode = MODE_INIT_F;
+    }
+    HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->stream));
+    if (int rc = exchange(c, 0)) return rc;
+    c->cur = 0;
+    return 0;
+}
+
+static int run_steps(mplbhip_ctx* c, int nsteps, bool timed, double* ms_total,
+                     double* ms_kernel) {
+    if (int rc = need_device(c)) return rc;
+    if (int rc = ensure_init(c)) return rc;
+    if (nsteps < 0) return fail(c, MPLBHIP_ERR_ARG, "negative step count");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->tstep == 0)
+        if (int rc = first_collision(c)) return rc;
+    double ktot = 0.0;
+    if (timed) {
+        // per-step events bracket the fused kernel only
+        std::vector<hipEvent_t> evs(2 * (size_t)nsteps);
+        for (auto& e : evs) HIPCHK(c, hipEventCreate(&e));
+        HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        for (int s = 0; s < nsteps; s++) {
+            KParams kp = base_params(c);
+            kp.fs_prev = c->fs[c->cur].p;
+            kp.fs_next = c->fs[1 - c->cur].p;
+            kp.per_prev = c->per[c->cur].p;
+            kp.per_next = c->per[1 - c->cur].p;
+            HIPCHK(c, hipEventRecord(evs[2 * s], c->stream));
+            HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP, kp, c->stream));
+            HIPCHK(c, hipEventRecord(evs[2 * s + 1], c->stream));
+            if (int rc = exchange(c, 1 - c->cur)) return rc;
+            c->cur ^= 1;
+            c->tstep++;
+        }
+        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev[1]));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        if (ms_total) *ms_total = ms;
+        for (int s = 0; s < nsteps; s++) {
+            float k = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&k, evs[2 * s], evs[2 * s + 1]));
+            ktot += k;
+        }
+        for (auto& e : evs) (void)hipEventDestroy(e);
+        if (ms_kernel) *ms_kernel = ktot;
+        return 0;
+    }
+    // untimed: replay captured graphs of G steps, the remainder one by one
+    const int G = 32;
+    int left = nsteps;
+    if (left >= G) {
+        if (c->graph_steps != G)
+            if (int rc = build_graphs(c, G)) return rc;
+        while (left >= G) {
+            HIPCHK(c, hipGraphLaunch(c->gexec[c->cur], c->stream));
+            // G is even: the parity after G steps is unchanged
+            c->tstep += G;
+            left -= G;
+        }
+    }
+    while (left > 0) {
+        if (int rc = enqueue_step(c, c->cur, false)) return rc;
+        c->cur ^= 1;
+        c->tstep++;
+        left--;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mplbhip_step(mplbhip_ctx* c, int nsteps) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    return run_steps(c, nsteps, false, nullptr, nullptr);
+}
+
+int mplbhip_step_timed(mplbhip_ctx* c, int nsteps, double* ms_total, double* ms_kernel) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    return run_steps(c, nsteps, true, ms_total, ms_kernel);
+}
+
+int mplbhip_update_macros(mplbhip_ctx* c) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    if (int rc = need_device(c)) return rc;
+    return materialize(c, false, false, nullptr);
+}
+
+int mplbhip_get_f(mplbhip_ctx* c, int block, double* out) {
+    if (!c || !out) return MPLBHIP_ERR_ARG;
+    if (int rc = need_device(c)) return rc;
+    if (block < 0 || block >= (int)c->bp.size())
+        return fail(c, MPLBHIP_ERR_ARG, "undefined block");
+    if (int rc = materialize(c, true, false, nullptr)) return rc;
+    const BlockParams& B = c->bp[block];
+    const size_t nn = (size_t)B.n[0] * B.n[1] * B.n[2];
+    HIPCHK(c, hipMemcpy(out, static_cast<double*>(c->fhost.p) + B.fbase,
+                        nn * c->Q * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int mplbhip_get_macro(mplbhip_ctx* c, int block, int var, double* out) {
+    if (!c || !out) return MPLBHIP_ERR_ARG;
+    if (int rc = need_device(c)) return rc;
+    if (block < 0 || block >= (int)c->bp.size() || var < 0 || var > c->dim)
+        return fail(c, MPLBHIP_ERR_ARG, "bad block or variable");
+    const BlockParams& B = c->bp[block];
+    const size_t nn = (size_t)B.n[0] * B.n[1] * B.n[2];
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, static_cast<double*>(c->macro[var].p) + B.mbase,
+                        nn * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int mplbhip_residual(mplbhip_ctx* c, int check_period, double* per_var, double* max) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    if (int rc = need_device(c)) return rc;
+    if (check_period <= 0) return fail(c, MPLBHIP_ERR_ARG, "check period must be > 0");
+    std::vector<double> acc;
+    if (int rc = materialize(c, false, true, &acc)) return rc;
+    const double* tot = acc.data() + c->bp.size() * kNAcc;
+    double mx = 0.0;
+    for (int v = 0; v < 1 + c->dim; v++) {
+        const double rel = (tot[v] / tot[4 + v]) / (check_period * c->dt);
+        if (per_var) per_var[v] = rel;
+        if (mx <= rel) mx = rel;
+    }
+    if (max) *max = mx;
+    return 0;
+}
+
+int mplbhip_mass(mplbhip_ctx* c, double* mass) {
+    if (!c || !mass) return MPLBHIP_ERR_ARG;
+    if (int rc = need_device(c)) return rc;
+    std::vector<double> acc;
+    if (int rc = materialize(c, false, false, &acc)) return rc;
+    *mass = acc[c->bp.size() * kNAcc + 8];
+    return 0;
+}
+
+int mplbhip_time_step(mplbhip_ctx* c, long long* step) {
+    if (!c || !step) return MPLBHIP_ERR_ARG;
+    *step = c->tstep;
+    return 0;
+}
+
+int mplbhip_sync(mplbhip_ctx* c) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    if (c->device < 0) return 0;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -105,7 +105,7 @@ int orc_get_threads(void) {
 #endif
 }
 #ifdef _OPENMP
-#define OMP_FOR _Pragma("omp parallel for schedule(static) num_threads(orc_get_threads())")
+#define OMP_FOR _Pragma("omp parallel for collapse(2) schedule(static) num_threads(orc_get_threads())")
 #else
 #define OMP_FOR
 #endif

@@ -1,0 +1,122 @@
+"""CPU-side checks of the C ABI: the library loads, exports every entry
+point the header declares, and its plan (node typing, geometry, boundary
+programs) reproduces the oracle's per-node tagging.  No compute calls."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from mplb_amd import cases, hip
+from mplb_amd.enums import BoundaryScheme as BSch
+from mplb_amd.enums import BoundarySurface as BS
+from mplb_amd.enums import VertexType as VT
+from oracle.oracle import OracleCase
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(hip.LIB_PATH)
+    names = hip.header_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert b"gfx950" in hip.load().mplbhip_version()
+
+
+class PlanOnly(hip.LatticeCase):
+    """device -1: everything up to Partition; state calls are skipped."""
+
+    def __init__(self, dim):
+        super().__init__(dim, device=-1)
+
+    def set_macros(self, *a, **k):
+        pass
+
+    def init_feq(self):
+        pass
+
+    def set_dt(self, dt):
+        pass
+
+    def set_accel(self, block, g):
+        pass
+
+
+def _compare_tagging(build):
+    dims = {}
+    p = None
+    for Backend in ("plan", "oracle"):
+        if Backend == "plan":
+            dim, fn = build
+            p = PlanOnly(dim)
+            fn(p)
+        else:
+            dim, fn = build
+            o = OracleCase(dim)
+            fn(o)
+    for b in p.sizes:
+        for which in (0, 1):
+            assert np.array_equal(p.node_info(b, which), o.get_int(b, which)), (b, which)
+    return p
+
+
+CASES = {
+    "cavity2d": (2, lambda be: cases.cavity2d(be, 12)),
+    "cavity2d_ragged": (2, lambda be: cases.cavity2d(be, 9, order=("Top", "Bottom", "Left", "Right"))),
+    "cavity3d": (3, lambda be: cases.cavity3d(be, 7, noise=False)),
+    "cavity3d_json_order": (3, lambda be: cases.cavity3d(
+        be, 6, noise=False, order=("Top", "Bottom", "Left", "Right", "Front", "Back"))),
+    "box3d": (3, lambda be: cases.cavity3d(be, 5, noise=False, size=(9, 6, 5))),
+    "lchannel": (3, lambda be: cases.lchannel(be, n=7)),
+    "channel2d": (2, lambda be: cases.channel2d(be, 6, 9, 0.8, 1e-6)),
+    "periodic3d": (3, lambda be: cases.periodic_box(
+        be, (5, 4, 6), 0.8, np.ones((6, 4, 5)), np.zeros((6, 4, 5)),
+        np.zeros((6, 4, 5)), np.zeros((6, 4, 5)))),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_plan_tagging_matches_oracle(name):
+    _compare_tagging(CASES[name])
+
+
+def test_edge_surfaces_and_topback_quirk():
+    """Edge-surface boundaries (RightTop, TopBack ...) and the TopBack y
+    range quirk of block.cpp:35 tag the same nodes in plan and oracle."""
+    def build(be):
+        be.add_block(0, (6, 5, 7))
+        be.set_collision(0.1)
+        wall = (0.0, 0.0, 0.0)
+        for s in ("Left", "Right", "Top", "Bottom", "Front", "Back"):
+            be.add_boundary(0, BS[s], BSch.EQMDiffuseRefl, wall, VT.Wall)
+        be.add_boundary(0, BS.RightTop, BSch.EQMDiffuseRefl, (0.01, 0, 0), VT.Inlet)
+        be.add_boundary(0, BS.TopBack, BSch.None_, (), VT.Symmetry)
+        be.add_boundary(0, BS.LeftFront, BSch.None_, (), VT.OutLet)
+        be.prepare()
+    _compare_tagging((3, build))
+
+
+def test_plan_only_context_refuses_compute():
+    p = PlanOnly(2)
+    cases.cavity2d(p, 8)
+    with pytest.raises(hip.MplbError, match="without a device"):
+        p.step(1)
+
+
+def test_unsupported_scheme_is_an_error():
+    p = PlanOnly(2)
+    p.add_block(0, (8, 8))
+    p.set_collision(0.1)
+    with pytest.raises(hip.MplbError, match="not implemented"):
+        p.add_boundary(0, BS.Left, BSch.FDPeriodic, (), VT.Wall)
+
+
+def test_define_order_errors():
+    p = PlanOnly(3)
+    with pytest.raises(hip.MplbError, match="id order"):
+        p.add_block(1, (8, 8, 8))
+    p.add_block(0, (8, 8, 8))
+    with pytest.raises(hip.MplbError, match="< 3 nodes"):
+        p.add_block(1, (8, 2, 8))
+    p.add_connection(0, BS.Left, 0, BS.Right, VT.MDPeriodic)
+    with pytest.raises(hip.MplbError, match="already defined"):
+        p.add_connection(0, BS.Left, 0, BS.Right, VT.MDPeriodic)
