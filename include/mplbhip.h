@@ -105,6 +105,16 @@ int mplbhip_step(mplbhip_ctx* ctx, int nsteps);
  * kernel: ms_total is the wall span, ms_kernel the summed step-kernel time */
 int mplbhip_step_timed(mplbhip_ctx* ctx, int nsteps, double* ms_total,
                        double* ms_kernel);
+/* Iterate (evolution.cpp:71-135 and the templates evolution.h:78-129).
+ * criteria < 0: run exactly max_steps steps, refreshing the macro variables
+ * at every check_period-th step (the transient form).  criteria >= 0: step
+ * until the residual taken every check_period steps drops below criteria,
+ * or max_steps have been taken.  The macros and residual of a check step
+ * come out of that step's own kernel.  *steps_done and *residual (the last
+ * residual, or -1 if none was taken) may be NULL. */
+int mplbhip_iterate(mplbhip_ctx* ctx, double criteria, int check_period,
+                    long long max_steps, long long* steps_done,
+                    double* residual);
 /* UpdateMacroVars[3D] (model_wrapper.cpp:112-275): macros of the current f */
 int mplbhip_update_macros(mplbhip_ctx* ctx);
 /* current distribution function (post boundary conditions), [q][k][j][i] */
@@ -113,14 +123,28 @@ int mplbhip_get_f(mplbhip_ctx* ctx, int block, double* out);
 int mplbhip_get_macro(mplbhip_ctx* ctx, int block, int var, double* out);
 /* CalcResidualError + GetMaximumResidual (flowfield_wrapper.cpp:31-89,
  * flowfield.cpp:148-158) on the current f, in FP64 on the device.
- * per_var receives 1+space_dim values; *max their maximum. */
+ * per_var receives, for rho, u, v[, w], the ratio
+ * sum (m - m_copy)^2 / sum m^2 that CalcResidualError stores; *max is
+ * GetMaximumResidual(check_period) = max ratio / (check_period * dt).
+ * MacroVarsCopy is refreshed, as in the reference. */
 int mplbhip_residual(mplbhip_ctx* ctx, int check_period, double* per_var,
                      double* max);
+/* the ratios and maximum of the last residual taken, by mplbhip_residual
+ * or by a check step of mplbhip_iterate */
+int mplbhip_last_residual(mplbhip_ctx* ctx, double* per_var, double* max);
 /* FP64 total mass of the current f over every interior node */
 int mplbhip_mass(mplbhip_ctx* ctx, double* mass);
 /* steps taken since the initial condition */
 int mplbhip_time_step(mplbhip_ctx* ctx, long long* step);
 int mplbhip_sync(mplbhip_ctx* ctx);
+
+/* ---- diagnostics (no reference counterpart) ----------------------------
+ * Plain device-to-device copy of nbytes (16-byte vector loads and stores)
+ * repeated iters times on `device`; ms_per_iter is the HIP-event average.
+ * Used to calibrate HBM traffic counters and as the achievable-bandwidth
+ * yardstick next to the lattice kernel. */
+int mplbhip_copy_probe(int device, long long nbytes, int iters,
+                       double* ms_per_iter);
 
 #ifdef __cplusplus
 }

@@ -38,5 +38,28 @@ def build(verbose=False):
     return OUT
 
 
+APPS_DIR = os.path.join(HERE, "hilemms")
+APPS = [("apps/cavity3d.cpp", 3, "cavity3d"), ("apps/case_driver.cpp", 2, "case_driver2d"),
+        ("apps/case_driver.cpp", 3, "case_driver3d")]
+
+
+def app_path(name):
+    return os.path.join(APPS_DIR, "build", name)
+
+
+def build_apps(verbose=False):
+    """The HiLeMMS example drivers, linked against libmplbhip.so."""
+    out = []
+    for src, dim, name in APPS:
+        cmd = ["make", "-s", "-B", "-C", APPS_DIR, "APP=" + os.path.join(APPS_DIR, src),
+               "DIM=%d" % dim, "OUT=" + app_path(name)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+        out.append(app_path(name))
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_apps(verbose="-v" in sys.argv))

@@ -194,3 +194,48 @@ def lchannel(be, n=33, top_lid=-0.001, outlet_rho=1.0, tau_ref=0.01,
     dt = mesh / CS
     be.set_dt(dt)
     return dict(dt=dt)
+
+
+# ---- HiLeMMS JSON case files (keys of the reference's configuration.cpp) ----
+_SCHEME_STRINGS = {"EQMDiffuseREfl": BSch.EQMDiffuseRefl,
+                   "ExtrapolPressure1ST": BSch.ExtrapolPressure1ST,
+                   "MDPeriodic": BSch.MDPeriodic, "FDPeriodic": BSch.FDPeriodic,
+                   "BounceBack": BSch.BounceBack, "None": BSch.None_}
+
+
+def from_config(be, cfg):
+    """Drive a backend through the call sequence of an App's
+    simulate(config) for a parsed HiLeMMS JSON case; returns
+    (steps_to_run, check_period, dt)."""
+    dim = int(cfg["SpaceDim"])
+    sizes = cfg["BlockSize"]
+    ids = cfg["BlockIds"]
+    for k, bid in enumerate(ids):
+        be.add_block(bid, tuple(sizes[dim * k: dim * (k + 1)]))
+    for frm, fs, to, ts, ty in zip(cfg.get("FromBlockIds") or [],
+                                   cfg.get("FromBoundarySurface") or [],
+                                   cfg.get("ToBlockIds") or [],
+                                   cfg.get("ToBoundarySurface") or [],
+                                   cfg.get("BlockConnectionType") or []):
+        be.add_connection(frm, BS[fs], to, BS[ts], VT[ty])
+    force = cfg["BodyForceType"][0]
+    force_macros = any(t.endswith("_Force") for t in cfg["MacroVarTypes"])
+    be.set_collision(cfg["TauRef"][0], force_type=1 if force == "BodyForce_1st" else 0,
+                     force_macros=force_macros)
+    k = 0
+    while ("BoundaryCondition%d" % k) in cfg and cfg["BoundaryCondition%d" % k]:
+        bc = cfg["BoundaryCondition%d" % k]
+        be.add_boundary(bc["BlockIndex"], BS[bc["BoundarySurface"]],
+                        _SCHEME_STRINGS[bc["BoundaryScheme"]], bc["GivenVars"],
+                        VT[bc["BoundaryType"]])
+        k += 1
+    be.prepare()
+    for k, bid in enumerate(ids):
+        shp = tuple(reversed(sizes[dim * k: dim * (k + 1)]))
+        z = np.zeros(shp)
+        be.set_macros(bid, np.ones(shp), z, z, z if dim == 3 else None)
+    be.init_feq()
+    dt = cfg["MeshSize"] / CS
+    be.set_dt(dt)
+    steps = cfg.get("TimeStepsToRun", 0) if cfg.get("Transient", True) else None
+    return steps, cfg["CheckPeriod"], dt

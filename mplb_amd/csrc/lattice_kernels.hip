@@ -413,7 +413,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
                 const double m[4] = {rho, uvw[0], uvw[1], uvw[2]};
 #pragma unroll
                 for (int v = 0; v < D + 1; v++) {
-                    if (MODE == MODE_STEP_CHECK || kp.want_residual) {
+                    if (kp.want_residual) {
                         const double old = kp.macro_copy[v][mnode];
                         acc[v] = (m[v] - old) * (m[v] - old);
                         acc[4 + v] = m[v] * m[v];
@@ -487,7 +487,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
                                      kCS * double(ul[2])};
 #pragma unroll
                 for (int v = 0; v < D + 1; v++) {
-                    if (MODE == MODE_STEP_CHECK || kp.want_residual) {
+                    if (kp.want_residual) {
                         const double old = kp.macro_copy[v][mnode];
                         acc[v] = (m[v] - old) * (m[v] - old);
                         acc[4 + v] = m[v] * m[v];
@@ -644,6 +644,22 @@ hipError_t launch_reduce(const double* partials, const BlockParams* blocks, int 
                          int ntiles, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partials, blocks,
                        nblocks, ntiles, out);
+    return hipGetLastError();
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_copy16(const u32x4* __restrict__ src,
+                                                u32x4* __restrict__ dst, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        __builtin_nontemporal_store(src[i], dst + i);
+}
+
+hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s) {
+    const int64_t n = bytes / 16;
+    hipLaunchKernelGGL(k_copy16, dim3(8192), dim3(256), 0, s,
+                       static_cast<const u32x4*>(src), static_cast<u32x4*>(dst), n);
     return hipGetLastError();
 }
 

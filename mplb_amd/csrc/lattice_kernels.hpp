@@ -35,7 +35,8 @@ struct BlockParams {
 
 enum Mode : int {
     MODE_STEP = 0,        // gather, boundary, collide -> fs_next
-    MODE_STEP_CHECK = 1,  // as STEP, plus macros and residual partials
+    MODE_STEP_CHECK = 1,  // as STEP, plus macros (and residual partials
+                          // when want_residual)
     MODE_MATERIALIZE = 2, // gather, boundary -> f (host layout), macros
     MODE_INIT_FEQ = 3,    // f = feq(macros) -> collide -> fs_next (and f
                           // to fhost when given)
@@ -60,7 +61,7 @@ struct KParams {
     int32_t force_type;     // 0 none, 1 first order
     int32_t force_macros;   // *_Force velocity kernels
     int32_t fhost_in;       // MATERIALIZE: take f from fhost instead of streaming
-    int32_t want_residual;  // MATERIALIZE: accumulate the residual as CHECK does
+    int32_t want_residual;  // accumulate the residual against MacroVarsCopy
     double dt;
 };
 

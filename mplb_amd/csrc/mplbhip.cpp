@@ -23,6 +23,7 @@ hipError_t launch_halo(const HaloParams& hp, hipStream_t s);
 hipError_t launch_reduce(const double* partials, const BlockParams* blocks,
                          int nblocks, int ntiles, double* out, hipStream_t s);
 hipError_t launch_fill(void* p, int64_t bytes, hipStream_t s);
+hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
 }  // namespace mplbhip
 
 using namespace mplbhip;
@@ -76,6 +77,8 @@ struct mplbhip_ctx {
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int graph_steps = 0;
+    double last_ratio[4] = {0, 0, 0, 0};
+    double last_max = -1.0;
 };
 
 static int fail(mplbhip_ctx* c, int code, const char* fmt, ...) {
@@ -206,6 +209,18 @@ static int enqueue_step(mplbhip_ctx* c, int from, bool check) {
     kp.per_next = c->per[1 - from].p;
     HIPCHK(c, launch_lattice(c->dim, c->storage,
                              check ? MODE_STEP_CHECK : MODE_STEP, kp, c->stream));
+    if (int rc = exchange(c, 1 - from)) return rc;
+    return 0;
+}
+
+static int enqueue_check(mplbhip_ctx* c, int from, bool residual) {
+    KParams kp = base_params(c);
+    kp.fs_prev = c->fs[from].p;
+    kp.fs_next = c->fs[1 - from].p;
+    kp.per_prev = c->per[from].p;
+    kp.per_next = c->per[1 - from].p;
+    kp.want_residual = residual ? 1 : 0;
+    HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP_CHECK, kp, c->stream));
     if (int rc = exchange(c, 1 - from)) return rc;
     return 0;
 }
@@ -804,6 +819,61 @@ static int run_steps(mplbhip_ctx* c, int nsteps, bool timed, double* ms_total,
     return 0;
 }
 
+int mplbhip_iterate(mplbhip_ctx* c, double criteria, int check_period,
+                    long long max_steps, long long* steps_done, double* residual) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    if (int rc = need_device(c)) return rc;
+    if (int rc = ensure_init(c)) return rc;
+    if (check_period <= 0 || max_steps < 0)
+        return fail(c, MPLBHIP_ERR_ARG, "bad check period or step limit");
+    const bool converge = criteria >= 0;
+    long long done = 0;
+    double res = -1.0;
+    while (done < max_steps) {
+        // steps until the next check step (iteration counts from tstep)
+        long long to_check = check_period - (c->tstep % check_period) - 1;
+        long long plain = std::min<long long>(to_check, max_steps - done);
+        while (plain > 0) {
+            const int chunk = (int)std::min<long long>(plain, 1 << 20);
+            if (int rc = run_steps(c, chunk, false, nullptr, nullptr)) return rc;
+            plain -= chunk;
+            done += chunk;
+        }
+        if (done >= max_steps) break;
+        if (c->tstep == 0)
+            if (int rc = first_collision(c)) return rc;
+        if (int rc = enqueue_check(c, c->cur, converge)) return rc;
+        c->cur ^= 1;
+        c->tstep++;
+        done++;
+        if (converge) {
+            double acc[kNAcc];
+            HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
+                                    static_cast<const BlockParams*>(c->blocks.p),
+                                    (int)c->bp.size(), c->ntiles,
+                                    static_cast<double*>(c->red.p), c->stream));
+            HIPCHK(c, hipMemcpyAsync(acc, static_cast<double*>(c->red.p) +
+                                              c->bp.size() * kNAcc,
+                                     sizeof acc, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            // CalcResidualError ratios, GetMaximumResidual (flowfield.cpp:148-158)
+            double mx = 0.0;
+            for (int v = 0; v < 1 + c->dim; v++) {
+                c->last_ratio[v] = acc[v] / acc[4 + v];
+                const double rel = c->last_ratio[v] / (check_period * c->dt);
+                if (mx <= rel) mx = rel;
+            }
+            res = mx;
+            c->last_max = mx;
+            if (!(res >= criteria)) break;
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (steps_done) *steps_done = done;
+    if (residual) *residual = res;
+    return 0;
+}
+
 int mplbhip_step(mplbhip_ctx* c, int nsteps) {
     if (!c) return MPLBHIP_ERR_ARG;
     return run_steps(c, nsteps, false, nullptr, nullptr);
@@ -855,11 +925,21 @@ int mplbhip_residual(mplbhip_ctx* c, int check_period, double* per_var, double* 
     const double* tot = acc.data() + c->bp.size() * kNAcc;
     double mx = 0.0;
     for (int v = 0; v < 1 + c->dim; v++) {
-        const double rel = (tot[v] / tot[4 + v]) / (check_period * c->dt);
-        if (per_var) per_var[v] = rel;
+        c->last_ratio[v] = tot[v] / tot[4 + v];
+        const double rel = c->last_ratio[v] / (check_period * c->dt);
+        if (per_var) per_var[v] = c->last_ratio[v];
         if (mx <= rel) mx = rel;
     }
+    c->last_max = mx;
     if (max) *max = mx;
+    return 0;
+}
+
+int mplbhip_last_residual(mplbhip_ctx* c, double* per_var, double* max) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    if (per_var)
+        for (int v = 0; v < 1 + c->dim; v++) per_var[v] = c->last_ratio[v];
+    if (max) *max = c->last_max;
     return 0;
 }
 
@@ -883,6 +963,41 @@ int mplbhip_sync(mplbhip_ctx* c) {
     if (c->device < 0) return 0;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
+}
+
+int mplbhip_copy_probe(int device, long long nbytes, int iters,
+                       double* ms_per_iter) {
+    if (nbytes < 16 || iters < 1 || !ms_per_iter) return MPLBHIP_ERR_ARG;
+    nbytes = nbytes / 16 * 16;
+    if (hipSetDevice(device) != hipSuccess) return MPLBHIP_ERR_NODEVICE;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipMalloc(&a, nbytes) != hipSuccess || hipMalloc(&b, nbytes) != hipSuccess ||
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = MPLBHIP_ERR_HIP;
+    } else {
+        (void)launch_fill(a, nbytes, s);
+        (void)launch_copy(a, b, nbytes, s);  // warm up
+        (void)hipEventRecord(e0, s);
+        for (int i = 0; i < iters; i++) (void)launch_copy(a, b, nbytes, s);
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            rc = MPLBHIP_ERR_HIP;
+        } else {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            *ms_per_iter = ms / iters;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
 }
 
 }  // extern "C"

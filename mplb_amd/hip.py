@@ -72,6 +72,9 @@ def load():
         "mplbhip_mass": (I, [P, DP]),
         "mplbhip_time_step": (I, [P, LLP]),
         "mplbhip_sync": (I, [P]),
+        "mplbhip_copy_probe": (I, [I, ctypes.c_longlong, I, DP]),
+        "mplbhip_iterate": (I, [P, D, I, ctypes.c_longlong, LLP, DP]),
+        "mplbhip_last_residual": (I, [P, DP, DP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -83,6 +86,15 @@ def load():
 
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def copy_probe(nbytes, iters=20, device=0):
+    """HIP-event average of a plain nbytes device copy (ms)."""
+    ms = ctypes.c_double()
+    rc = load().mplbhip_copy_probe(device, int(nbytes), int(iters), ctypes.byref(ms))
+    if rc != 0:
+        raise MplbError("copy probe failed (%d)" % rc)
+    return ms.value
 
 
 class LatticeCase:
@@ -187,6 +199,17 @@ class LatticeCase:
                                              ctypes.byref(ker)))
         return tot.value, ker.value
 
+    def iterate(self, criteria, check_period, max_steps):
+        """Iterate(convergenceCriteria, checkPeriod) when criteria >= 0,
+        Iterate(steps, checkPeriod) when criteria < 0 (max_steps steps).
+        Returns (steps taken, last residual or -1)."""
+        done = ctypes.c_longlong()
+        res = ctypes.c_double()
+        self._ck(self.lib.mplbhip_iterate(self.h, float(criteria), int(check_period),
+                                          int(max_steps), ctypes.byref(done),
+                                          ctypes.byref(res)))
+        return done.value, res.value
+
     def update_macros(self):
         self._ck(self.lib.mplbhip_update_macros(self.h))
 
@@ -201,6 +224,8 @@ class LatticeCase:
         return out
 
     def residual(self, check_period):
+        """CalcResidualError + GetMaximumResidual: (max residual, per-variable
+        ratios sum (m - m_copy)^2 / sum m^2)."""
         per = np.zeros(4)
         mx = ctypes.c_double()
         self._ck(self.lib.mplbhip_residual(self.h, int(check_period), _dp(per),

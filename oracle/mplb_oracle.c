@@ -1000,8 +1000,9 @@ double orc_residual(orc_case* c, int check_period, double* per_var) {
     /* GetMaximumResidual, flowfield.cpp:148-158 */
     double maxRes = 0;
     for (int v = 0; v < nv; v++) {
-        double rel = (diff[v] / sum[v]) / (check_period * c->dt);
-        if (per_var) per_var[v] = rel;
+        const double err = diff[v] / sum[v];
+        double rel = err / (check_period * c->dt);
+        if (per_var) per_var[v] = err;
         if (maxRes <= rel) maxRes = rel;
     }
     return maxRes;

@@ -65,8 +65,9 @@ int orc_get_int(orc_case* c, int block, int which, int* out);
 /* total mass: sum over blocks, interior nodes, populations of f */
 double orc_mass(orc_case* c);
 /* CalcResidualError + GetMaximumResidual (flowfield_wrapper.cpp:31-89,
- * flowfield.cpp:148-158).  per_var receives err_v/(check_period*dt) for
- * rho, u, v[, w]; the return value is their maximum. */
+ * flowfield.cpp:148-158).  per_var receives the stored ratios
+ * err_v = sum (m - m_copy)^2 / sum m^2 for rho, u, v[, w]; the return value
+ * is max_v err_v/(check_period*dt). */
 double orc_residual(orc_case* c, int check_period, double* per_var);
 /* OpenMP threads used by the sweeps (0 = library default) */
 void orc_set_threads(int nthreads);

@@ -187,3 +187,36 @@ def test_c2_cavity3d_full_size_few_steps():
     g.step(steps)
     o.step(steps)
     assert compare(g, o, "C2") == 0.0
+
+
+def test_f64_iterate_to_convergence_matches_oracle():
+    """Iterate(criteria, checkPeriod) (evolution.cpp:102-135): the fused
+    check step produces the same residual sequence and stopping step as the
+    oracle's UpdateMacroVars + CalcResidualError loop."""
+    log("iterate to residual 1e-4, 2D 24^2")
+    period, crit = 200, 1e-4
+    g, o, _ = pair(2, lambda be: cases.cavity2d(be, 24, re=20.0, ulid=0.05))
+    steps, rg = g.iterate(crit, period, 200000)
+    it, ro = 0, 1.0
+    while ro >= crit:
+        o.step(period)
+        it += period
+        o.update_macros()
+        ro, _ = o.residual(period)
+    assert steps == it, (steps, it)
+    assert math.isclose(rg, ro, rel_tol=1e-9), (rg, ro)
+    assert compare(g, o, "iterate") == 0.0
+
+
+def test_f64_iterate_transient_refreshes_macros():
+    """Iterate(steps, checkPeriod): macros are those of the last check step."""
+    log("iterate transient")
+    g, o, _ = pair(3, lambda be: cases.cavity3d(be, 9, re=50.0, noise=True))
+    steps, res = g.iterate(-1.0, 10, 35)
+    assert steps == 35 and res == -1.0
+    o.step(30)
+    o.update_macros()
+    for v in range(4):
+        assert np.array_equal(g.get_macro(0, v), o.get_macro(0, v))
+    o.step(5)
+    assert np.array_equal(g.get_f(0), o.get_f(0))
