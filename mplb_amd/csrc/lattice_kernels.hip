@@ -167,6 +167,19 @@ __device__ __forceinline__ int32_t pull_delta(const BlockParams& B, int q) {
     return cvel(D, q, 0) + cvel(D, q, 1) * B.sx + cvel(D, q, 2) * B.sz;
 }
 
+// element q*sq + node - delta of a block array: the uniform part forms a
+// scalar base, the per-thread part a 32-bit byte offset (saddr addressing)
+template <typename E>
+__device__ __forceinline__ const E* at_u(const E* base, int64_t uniform, uint32_t node) {
+    return reinterpret_cast<const E*>(reinterpret_cast<const char*>(base + uniform) +
+                                      node * (uint32_t)sizeof(E));
+}
+template <typename E>
+__device__ __forceinline__ E* at_u(E* base, int64_t uniform, uint32_t node) {
+    return reinterpret_cast<E*>(reinterpret_cast<char*>(base + uniform) +
+                                node * (uint32_t)sizeof(E));
+}
+
 __device__ __forceinline__ int region_code(int x, int n) {
     return x == 0 ? 1 : (x == n - 1 ? 2 : 0);
 }
@@ -376,9 +389,12 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
         } else if (!P) {
             // bulk: every population streams in (KerStream3D Fluid branch)
 #pragma unroll
-            for (int q = 0; q < Q; q++)
-                f[q] = ld<ST>(prev, (int64_t)q * B.sq + node - pull_delta<D>(B, q),
-                              kp.scale[q]);
+            for (int q = 0; q < Q; q++) {
+                const E* p = at_u(prev, (int64_t)q * B.sq - pull_delta<D>(B, q),
+                                  (uint32_t)node);
+                if constexpr (ST == MPLBHIP_STORE_F16) f[q] = __half2float(*p) * kp.scale[q];
+                else f[q] = *p;
+            }
         } else {
             gather_masked<D, ST>(kp, B, *P, x, y, z, f);
             full = true;
@@ -451,7 +467,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
                     } else {
                         fsq = fe + om1 * (f[q] - fe);
                     }
-                    next[(int64_t)q * B.sq + node] = fsq;
+                    *at_u(next, (int64_t)q * B.sq, (uint32_t)node) = fsq;
                 }
             }
         } else {
@@ -526,7 +542,9 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
                         for (int d = 0; d < D; d++) cg += float(cvel(D, q, d)) * float(B.g[d]);
                         gn += B.fforce * wq * rho * cg;
                     }
-                    st_nt<ST>(next, (int64_t)q * B.sq + node, gn, kp.inv_scale[q]);
+                    E* p = at_u(next, (int64_t)q * B.sq, (uint32_t)node);
+                    if constexpr (ST == MPLBHIP_STORE_F16) *p = __float2half_rn(gn * kp.inv_scale[q]);
+                    else __builtin_nontemporal_store(gn, p);
                 }
             }
         }
