@@ -63,6 +63,16 @@ int mplbhip_define_block(mplbhip_ctx* ctx, int id, const int* size);
 int mplbhip_define_connection(mplbhip_ctx* ctx, int from_block,
                               int from_surface, int to_block, int to_surface,
                               int vertex_type);
+/* No reference counterpart (the reference joins same-axis faces only,
+ * field.h:207-214): a connection between two perpendicular faces, for
+ * periodic inlet/outlet pairs of bent channels.  The from-face slab lands in
+ * the halo beyond to_surface turned by the quarter rotation about the third
+ * axis that takes the from-face's outward normal to the to-face's inward
+ * normal; populations are permuted with it.  Transverse sizes must match
+ * under the rotation. */
+int mplbhip_define_rotated_connection(mplbhip_ctx* ctx, int from_block,
+                                      int from_surface, int to_block,
+                                      int to_surface, int vertex_type);
 /* DefineComponents tauRef (model.cpp:236) + DefineCollision
  * (Collision_BGKIsothermal2nd only, model.cpp:406-445) + DefineBodyForce
  * (model.cpp:447-489, force_type BodyForce_None=0 / BodyForce_1st=1) +
@@ -115,6 +125,25 @@ int mplbhip_step_timed(mplbhip_ctx* ctx, int nsteps, double* ms_total,
 int mplbhip_iterate(mplbhip_ctx* ctx, double criteria, int check_period,
                     long long max_steps, long long* steps_done,
                     double* residual);
+/* ---- independent instances (batched cases) ------------------------------
+ * A context may hold many unconnected blocks, each its own case (e.g. a
+ * batch of cavities of different sizes and Reynolds numbers): they advance
+ * together in one launch per step.  Per-block relaxation time and time step
+ * override the context-wide ones (no reference counterpart: the reference
+ * runs one case per process). */
+int mplbhip_define_block_collision(mplbhip_ctx* ctx, int block, double tau_ref,
+                                   double dt);
+/* Step every block until its own velocity residual (the CalcResidualError
+ * ratio of u, v[, w], maximum over components, divided by
+ * check_period * dt_block) falls below criteria, freezing each block as it
+ * converges (a frozen block costs one early-exit per workgroup per step).
+ * steps_per_block[b] receives the step at which block b converged (-1 if
+ * it did not within max_steps); residual_per_block[b] its last residual;
+ * *steps_done the steps taken.  Requires blocks without connections. */
+int mplbhip_iterate_batch(mplbhip_ctx* ctx, double criteria, int check_period,
+                          long long max_steps, long long* steps_per_block,
+                          double* residual_per_block, long long* steps_done);
+
 /* UpdateMacroVars[3D] (model_wrapper.cpp:112-275): macros of the current f */
 int mplbhip_update_macros(mplbhip_ctx* ctx);
 /* current distribution function (post boundary conditions), [q][k][j][i] */

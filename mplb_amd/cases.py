@@ -196,6 +196,65 @@ def lchannel(be, n=33, top_lid=-0.001, outlet_rho=1.0, tau_ref=0.01,
     return dict(dt=dt)
 
 
+def lchannel_periodic(be, a=128, length=256, tau_l=0.8, g=1e-6):
+    """The L-channel of Apps/3DLChannel/lbm3d_L.cpp:72-190 (blocks Top,
+    Middle, Right joined by VirtualBoundary faces, EQMDiffuseRefl walls) run
+    as a closed loop: the outlet face (Right of block 2) and the inlet face
+    (Top of block 0) are joined by a rotated periodic connection, and
+    BodyForce_1st drives the flow with |g| along each arm's axis (-y in the
+    Top arm, +x in the Right arm, the bisector in the Middle block).
+
+    Blocks: 0 = a x length x a, 1 = a x a x a, 2 = length x a x a.  Lattice
+    units: dt = 1, TauRef = tau_l - 1/2.  The periodic faces are defined
+    before each block's walls so that the walls own the edges."""
+    be.add_block(0, (a, length, a))
+    be.add_block(1, (a, a, a))
+    be.add_block(2, (length, a, a))
+    for frm, fs, to, ts in ((0, BS.Bottom, 1, BS.Top), (1, BS.Top, 0, BS.Bottom),
+                            (1, BS.Right, 2, BS.Left), (2, BS.Left, 1, BS.Right)):
+        be.add_connection(frm, fs, to, ts, VT.VirtualBoundary)
+    be.add_rotated_connection(2, BS.Right, 0, BS.Top, VT.MDPeriodic)
+    be.add_rotated_connection(0, BS.Top, 2, BS.Right, VT.MDPeriodic)
+    dt = 1.0
+    be.set_collision(tau_ref_for(tau_l, dt), force_type=1, force_macros=True)
+    wall = (0.0, 0.0, 0.0)
+    E = BSch.EQMDiffuseRefl
+    VB = (BSch.None_, (), VT.VirtualBoundary)
+    MP = (BSch.None_, (), VT.MDPeriodic)
+    be.add_boundary(0, BS.Top, *MP)
+    be.add_boundary(0, BS.Left, E, wall, VT.Wall)
+    be.add_boundary(0, BS.Right, E, wall, VT.Wall)
+    be.add_boundary(0, BS.Bottom, *VB)
+    be.add_boundary(0, BS.Front, E, wall, VT.Wall)
+    be.add_boundary(0, BS.Back, E, wall, VT.Wall)
+    be.add_boundary(1, BS.Left, E, wall, VT.Wall)
+    be.add_boundary(1, BS.Right, *VB)
+    be.add_boundary(1, BS.Top, *VB)
+    be.add_boundary(1, BS.RightTop, E, wall, VT.Wall)
+    be.add_boundary(1, BS.Bottom, E, wall, VT.Wall)
+    be.add_boundary(1, BS.Front, E, wall, VT.Wall)
+    be.add_boundary(1, BS.Back, E, wall, VT.Wall)
+    be.add_boundary(2, BS.Right, *MP)
+    be.add_boundary(2, BS.Left, *VB)
+    be.add_boundary(2, BS.Top, E, wall, VT.Wall)
+    be.add_boundary(2, BS.Bottom, E, wall, VT.Wall)
+    be.add_boundary(2, BS.Front, E, wall, VT.Wall)
+    be.add_boundary(2, BS.Back, E, wall, VT.Wall)
+    be.prepare()
+    s2 = g / math.sqrt(2.0)
+    be.set_accel(0, (0.0, -g, 0.0))
+    be.set_accel(1, (s2, -s2, 0.0))
+    be.set_accel(2, (g, 0.0, 0.0))
+    sizes = ((a, length, a), (a, a, a), (length, a, a))
+    for b, size in enumerate(sizes):
+        shp = tuple(reversed(size))
+        z = np.zeros(shp)
+        be.set_macros(b, np.ones(shp), z, z, z)
+    be.init_feq()
+    be.set_dt(dt)
+    return dict(dt=dt, nodes=sum(int(np.prod(x)) for x in sizes), sizes=sizes)
+
+
 # ---- HiLeMMS JSON case files (keys of the reference's configuration.cpp) ----
 _SCHEME_STRINGS = {"EQMDiffuseREfl": BSch.EQMDiffuseRefl,
                    "ExtrapolPressure1ST": BSch.ExtrapolPressure1ST,
@@ -239,3 +298,45 @@ def from_config(be, cfg):
     be.set_dt(dt)
     steps = cfg.get("TimeStepsToRun", 0) if cfg.get("Transient", True) else None
     return steps, cfg["CheckPeriod"], dt
+
+
+# ---- batched independent cavities -------------------------------------------
+BATCH_SIZES = (64, 96, 128, 192, 256)
+
+
+def batch_spec(count=256, seed=NOISE_SEED, sizes=BATCH_SIZES, re_range=(100.0, 1000.0)):
+    """Sizes and Reynolds numbers of a batch, from the counter-based
+    splitmix64 stream: instance i draws N from `sizes` with
+    splitmix64(seed ^ 2i) and Re uniform in re_range with
+    splitmix64(seed ^ (2i+1))."""
+    i = np.arange(count, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        h1 = splitmix64((2 * i) ^ np.uint64(seed))
+        h2 = splitmix64((2 * i + np.uint64(1)) ^ np.uint64(seed))
+    n = np.array(sizes)[(h1 % np.uint64(len(sizes))).astype(np.int64)]
+    u = (h2 >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    re = re_range[0] + (re_range[1] - re_range[0]) * u
+    return [int(x) for x in n], [float(x) for x in re]
+
+
+def batched_cavities(be, sizes, res, ulid=0.05):
+    """One D2Q9 cavity per block (the 2D cavity App's boundary order), each
+    with its own relaxation time and time step; blocks are unconnected."""
+    for b, n in enumerate(sizes):
+        be.add_block(b, (n, n))
+    be.set_collision(1.0, force_type=0)
+    lid = CS * ulid
+    params = []
+    for b, (n, re) in enumerate(zip(sizes, res)):
+        mesh, dt, tau_l = cavity_params(n, re, ulid)
+        be.set_block_collision(b, tau_ref_for(tau_l, dt), dt)
+        for s in ("Left", "Right", "Top", "Bottom"):
+            vel = (lid, 0.0) if s == "Top" else (0.0, 0.0)
+            be.add_boundary(b, BS[s], BSch.EQMDiffuseRefl, vel, VT.Wall)
+        params.append(dict(n=n, re=re, dt=dt, tau_l=tau_l))
+    be.prepare()
+    for b, n in enumerate(sizes):
+        shp = (n, n)
+        be.set_macros(b, np.ones(shp), np.zeros(shp), np.zeros(shp))
+    be.init_feq()
+    return params

@@ -198,13 +198,14 @@ __device__ __forceinline__ int64_t persist_slot(const RegionProg& P, int x,
 template <int D, int ST>
 __device__ void gather_masked(const KParams& kp, const BlockParams& B,
                               const RegionProg& P, int x, int y, int z,
-                              typename Store<ST>::T* f) {
+                              typename Store<ST>::T* f, const void* fs_prev,
+                              const void* per_prev) {
     using T = typename Store<ST>::T;
     using E = typename Store<ST>::E;
     constexpr int Q = D == 2 ? 9 : 19;
-    const E* prev = static_cast<const E*>(kp.fs_prev) + B.base;
+    const E* prev = static_cast<const E*>(fs_prev) + B.base;
     const int32_t node = z * B.sz + y * B.sx + x;
-    const T* per = static_cast<const T*>(kp.per_prev);
+    const T* per = static_cast<const T*>(per_prev);
     int64_t slot = P.persist >= 0 ? persist_slot<D>(P, x, y, z) : 0;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
@@ -227,13 +228,14 @@ __device__ void gather_masked(const KParams& kp, const BlockParams& B,
 template <int D, int ST>
 __device__ void neighbour_values(const KParams& kp, const BlockParams& B,
                                  const BCOp& op, int x, int y, int z,
-                                 typename Store<ST>::T* f) {
+                                 typename Store<ST>::T* f, const void* fs_prev,
+                                 const void* per_prev) {
     using T = typename Store<ST>::T;
     constexpr int Q = D == 2 ? 9 : 19;
     const RegionProg& N = kp.progs[B.prog_base + op.nb_region];
     const int nx = x + op.off[0], ny = y + op.off[1], nz = z + op.off[2];
     T g[Q];
-    gather_masked<D, ST>(kp, B, N, nx, ny, nz, g);
+    gather_masked<D, ST>(kp, B, N, nx, ny, nz, g, fs_prev, per_prev);
     for (int o = 0; o < op.nb_nops; o++) {
         if (N.op[o].type == OP_EQM) apply_eqm<D, T>(g, N.op[o]);
     }
@@ -245,14 +247,15 @@ __device__ void neighbour_values(const KParams& kp, const BlockParams& B,
 template <int D, int ST>
 __device__ void boundary_program(const KParams& kp, const BlockParams& B,
                                  const RegionProg& P, int x, int y, int z,
-                                 typename Store<ST>::T* f) {
+                                 typename Store<ST>::T* f, const void* fs_prev,
+                                 const void* per_prev) {
     using T = typename Store<ST>::T;
     for (int o = 0; o < P.nops; o++) {
         const BCOp& op = P.op[o];
         if (op.type == OP_EQM) {
             apply_eqm<D, T>(f, op);
         } else if (op.type == OP_EXTRAPOL) {
-            if (op.m_ext) neighbour_values<D, ST>(kp, B, op, x, y, z, f);
+            if (op.m_ext) neighbour_values<D, ST>(kp, B, op, x, y, z, f, fs_prev, per_prev);
             extrapol_rescale<D, T>(f, op.vars[0]);
         }
     }
@@ -318,6 +321,18 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
     const int x = tx * kTX + threadIdx.x;
     const int y = ty * kTY + threadIdx.y;
     const bool active = x < B.n[0] && y < B.n[1] && B.active;
+    // buffers of this block: steps read the latest fStage and write the
+    // other; MATERIALIZE re-streams from the one before the latest
+    const int cur = kp.cur_in[b];
+    const int src = MODE == MODE_MATERIALIZE ? 1 - cur : cur;
+    const void* fs_prev = kp.fs_buf[src];
+    const void* per_prev = kp.per_buf[src];
+    void* fs_next = kp.fs_buf[MODE >= MODE_INIT_FEQ ? cur : 1 - cur];
+    void* per_next = kp.per_buf[MODE >= MODE_INIT_FEQ ? cur : 1 - cur];
+    if constexpr (MODE == MODE_STEP || MODE == MODE_STEP_CHECK) {
+        if (t == 0 && threadIdx.x == 0 && threadIdx.y == 0)
+            kp.cur_out[b] = B.active ? 1 - cur : cur;
+    }
     double acc[kNAcc];
 #pragma unroll
     for (int a = 0; a < kNAcc; a++) acc[a] = 0.0;
@@ -332,8 +347,8 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
         const bool force_node = P ? P->force != 0 : true;
         T f[Q];  // EXACT: f; otherwise g = f - w (bulk) / f (boundary)
         bool full = false;  // f[] holds full values (boundary path)
-        const E* prev = static_cast<const E*>(kp.fs_prev) + B.base;
-        E* next = static_cast<E*>(kp.fs_next) + B.base;
+        const E* prev = static_cast<const E*>(fs_prev) + B.base;
+        E* next = static_cast<E*>(fs_next) + B.base;
 
         if (solid) {
             // ImmersedSolid: neither streamed nor collided; fStage unchanged
@@ -372,7 +387,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
                 if constexpr (EXACT) f[q] = fv;
                 else f[q] = T(fv - wgt(D, q));
                 if (P && P->persist >= 0) {
-                    T* per = static_cast<T*>(kp.per_next);
+                    T* per = static_cast<T*>(per_next);
                     per[persist_slot<D>(*P, x, y, z) * Q + q] = T(fv);
                 }
             }
@@ -396,12 +411,12 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
                 else f[q] = *p;
             }
         } else {
-            gather_masked<D, ST>(kp, B, *P, x, y, z, f);
+            gather_masked<D, ST>(kp, B, *P, x, y, z, f, fs_prev, per_prev);
             full = true;
-            boundary_program<D, ST>(kp, B, *P, x, y, z, f);
+            boundary_program<D, ST>(kp, B, *P, x, y, z, f, fs_prev, per_prev);
             if constexpr (MODE != MODE_MATERIALIZE) {
                 if (P->persist >= 0) {
-                    T* per = static_cast<T*>(kp.per_next);
+                    T* per = static_cast<T*>(per_next);
                     const int64_t s = persist_slot<D>(*P, x, y, z);
 #pragma unroll
                     for (int q = 0; q < Q; q++) per[s * Q + q] = f[q];
@@ -422,7 +437,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
 #pragma unroll
                 for (int q = 0; q < Q; q++) u += kCS * double(cvel(D, q, d)) * f[q];
                 u /= rho;
-                if (kp.force_macros && force_node) u += (kp.dt * B.g[d] / 2);
+                if (kp.force_macros && force_node) u += (B.dt * B.g[d] / 2);
                 uvw[d] = u;
             }
             if constexpr (MODE == MODE_MATERIALIZE || MODE == MODE_STEP_CHECK) {
@@ -575,9 +590,8 @@ __global__ void k_halo(HaloParams hp) {
         const int64_t si = hp.src_base + (int64_t)q * hp.src_sq +
                            (int64_t)(hp.org_src[2] + cc) * hp.src_sz +
                            (int64_t)(hp.org_src[1] + bb) * hp.src_sx + hp.org_src[0] + a;
-        const int64_t di = hp.dst_base + (int64_t)q * hp.dst_sq +
-                           (int64_t)(hp.org_dst[2] + cc) * hp.dst_sz +
-                           (int64_t)(hp.org_dst[1] + bb) * hp.dst_sx + hp.org_dst[0] + a;
+        const int64_t di = hp.dst_base + (int64_t)hp.qmap[q] * hp.dst_sq + hp.dst_org +
+                           a * hp.dstep[0] + bb * hp.dstep[1] + cc * hp.dstep[2];
         static_cast<E*>(hp.dst)[di] = src[si];
     }
 }

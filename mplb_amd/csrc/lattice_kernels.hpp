@@ -27,11 +27,17 @@ struct BlockParams {
     int64_t fbase;       // offset of this block in host-layout f arrays
     int64_t persist_base;
     double g[3];         // acceleration (code units)
+    double dt;           // time step of this block
     double tau, omega, one_minus_omega, tau_omega;
     float fomega, fone_minus_omega, fforce, fhalf_dtg[3];
-    int32_t active;      // batched runs: 0 freezes a converged block
+    int32_t active;      // 0 freezes the block (batched instances)
     int32_t pad;
 };
+
+// Ping-pong parity is kept per block in device memory: a step reads
+// cur_in[b] (the buffer holding block b's latest fStage) and the block's
+// first workgroup writes cur_out[b] (flipped unless the block is frozen),
+// so independent instances can stop inside one captured launch sequence.
 
 enum Mode : int {
     MODE_STEP = 0,        // gather, boundary, collide -> fs_next
@@ -48,10 +54,10 @@ struct KParams {
     int32_t nblocks;
     int32_t ntiles;
     const RegionProg* progs;
-    const void* fs_prev;
-    void* fs_next;
-    const void* per_prev;
-    void* per_next;
+    void* fs_buf[2];        // the two fStage buffers
+    void* per_buf[2];       // the two persist buffers
+    const int32_t* cur_in;  // per-block buffer index of the latest fStage
+    int32_t* cur_out;       // per-block index after this step (step modes)
     double* macro[4];       // MacroVars (current values, host layout)
     double* macro_copy[4];  // MacroVarsCopy (residual reference)
     double* fhost;      // materialize output / init input, host layout
@@ -62,7 +68,6 @@ struct KParams {
     int32_t force_macros;   // *_Force velocity kernels
     int32_t fhost_in;       // MATERIALIZE: take f from fhost instead of streaming
     int32_t want_residual;  // accumulate the residual against MacroVarsCopy
-    double dt;
 };
 
 struct HaloParams {
@@ -72,9 +77,16 @@ struct HaloParams {
     int64_t src_base, dst_base;  // node (0,0,0) population 0 of each block
     int32_t src_sx, src_sz, src_sq;
     int32_t dst_sx, dst_sz, dst_sq;
-    int32_t org_src[3], org_dst[3], ext[3];
+    int32_t org_src[3], ext[3];
     int32_t Q;
     int32_t elem_bytes;
+    // target element of slab cell (a, b, c), population q:
+    // dst_base + qmap[q] * dst_sq + dst_org + a*dstep[0] + b*dstep[1] + c*dstep[2]
+    // (a plain link has dstep = (1, dst_sx, dst_sz) and qmap[q] = q; a
+    // rotated one turns both)
+    int64_t dst_org;
+    int64_t dstep[3];
+    int32_t qmap[kMaxQ];
     const uint8_t* other_flag;  // per slab cell: 1 = read src_other
 };
 

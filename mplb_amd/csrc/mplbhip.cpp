@@ -56,7 +56,9 @@ struct mplbhip_ctx {
     bool initialized = false;
     bool init_from_f = false;
     long long tstep = 0;
-    int cur = 0;  // buffer holding the latest fStage
+    int cur = 0;  // parity of the latest fStage of every active block; also
+                  // selects the per-block parity array read by the next step
+    std::vector<double> btau, bdt;  // per-block collision overrides (NaN: none)
     std::string err;
 
     // layout
@@ -69,7 +71,7 @@ struct mplbhip_ctx {
     // device memory
     hipStream_t stream = nullptr;
     DevBuf fs[2], per[2], macro[4], macro_copy[4], progs, blocks, partials,
-        red, fhost;
+        red, fhost, parity[2];
     std::vector<HaloLink> halos;
     std::vector<double> host_init_f;  // f given by mplbhip_set_f, per block
     std::vector<float> scale, inv_scale;
@@ -128,10 +130,12 @@ static double kCSh() { return std::sqrt(3.0); }
 // collision constants per block, computed in the reference's order
 // (model_kernel.inc:428-429: omega = dt/(tau + 0.5*dt))
 static void fill_collision(mplbhip_ctx* c) {
-    const double tau = c->tau, dt = c->dt;
-    const double omega = dt / (tau + 0.5 * dt);
     for (size_t b = 0; b < c->bp.size(); b++) {
+        const double tau = std::isnan(c->btau[b]) ? c->tau : c->btau[b];
+        const double dt = std::isnan(c->bdt[b]) ? c->dt : c->bdt[b];
+        const double omega = dt / (tau + 0.5 * dt);
         BlockParams& B = c->bp[b];
+        B.dt = dt;
         B.tau = tau;
         B.omega = omega;
         B.one_minus_omega = 1 - omega;
@@ -183,7 +187,12 @@ static KParams base_params(mplbhip_ctx* c) {
     }
     kp.force_type = c->force_type;
     kp.force_macros = c->force_macros;
-    kp.dt = c->dt;
+    for (int i = 0; i < 2; i++) {
+        kp.fs_buf[i] = c->fs[i].p;
+        kp.per_buf[i] = c->per[i].p;
+    }
+    kp.cur_in = static_cast<const int32_t*>(c->parity[c->cur].p);
+    kp.cur_out = static_cast<int32_t*>(c->parity[1 - c->cur].p);
     return kp;
 }
 
@@ -201,39 +210,27 @@ static int exchange(mplbhip_ctx* c, int into) {
 }
 
 // one step: fused kernel from buffer cur into 1-cur, then halos
-static int enqueue_step(mplbhip_ctx* c, int from, bool check) {
+// one step from parity c->cur; leaves c->cur flipped
+static int enqueue_step(mplbhip_ctx* c, bool check, bool residual) {
     KParams kp = base_params(c);
-    kp.fs_prev = c->fs[from].p;
-    kp.fs_next = c->fs[1 - from].p;
-    kp.per_prev = c->per[from].p;
-    kp.per_next = c->per[1 - from].p;
+    kp.want_residual = residual ? 1 : 0;
     HIPCHK(c, launch_lattice(c->dim, c->storage,
                              check ? MODE_STEP_CHECK : MODE_STEP, kp, c->stream));
-    if (int rc = exchange(c, 1 - from)) return rc;
+    if (int rc = exchange(c, 1 - c->cur)) return rc;
+    c->cur ^= 1;
     return 0;
 }
 
-static int enqueue_check(mplbhip_ctx* c, int from, bool residual) {
-    KParams kp = base_params(c);
-    kp.fs_prev = c->fs[from].p;
-    kp.fs_next = c->fs[1 - from].p;
-    kp.per_prev = c->per[from].p;
-    kp.per_next = c->per[1 - from].p;
-    kp.want_residual = residual ? 1 : 0;
-    HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP_CHECK, kp, c->stream));
-    if (int rc = exchange(c, 1 - from)) return rc;
-    return 0;
-}
 
 static int build_graphs(mplbhip_ctx* c, int nsteps) {
     drop_graphs(c);
     for (int p = 0; p < 2; p++) {
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int from = p, rc = 0;
-        for (int s = 0; s < nsteps && rc == 0; s++) {
-            rc = enqueue_step(c, from, false);
-            from = 1 - from;
-        }
+        const int saved = c->cur;
+        c->cur = p;
+        int rc = 0;
+        for (int s = 0; s < nsteps && rc == 0; s++) rc = enqueue_step(c, false, false);
+        c->cur = saved;
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rc) {
@@ -275,16 +272,12 @@ static int materialize(mplbhip_ctx* c, bool want_f, bool residual,
                                      hipMemcpyHostToDevice, c->stream));
         } else {
             KParams ki = base_params(c);
-            ki.fs_next = c->fs[c->cur].p;
-            ki.per_next = c->per[c->cur].p;
             ki.fhost = static_cast<double*>(c->fhost.p);
             HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_INIT_FEQ, ki, c->stream));
         }
         kp.fhost = static_cast<double*>(c->fhost.p);
         kp.fhost_in = 1;
     } else {
-        kp.fs_prev = c->fs[1 - c->cur].p;
-        kp.per_prev = c->per[1 - c->cur].p;
         kp.fhost = want_f ? static_cast<double*>(c->fhost.p) : nullptr;
     }
     kp.want_residual = residual ? 1 : 0;
@@ -372,6 +365,8 @@ void mplbhip_destroy(mplbhip_ctx* c) {
         dfree(c->partials);
         dfree(c->red);
         dfree(c->fhost);
+        dfree(c->parity[0]);
+        dfree(c->parity[1]);
         for (HaloLink& h : c->halos)
             if (h.d_flag) (void)hipFree(h.d_flag);
         for (int i = 0; i < 2; i++)
@@ -400,6 +395,8 @@ int mplbhip_define_block(mplbhip_ctx* c, int id, const int* size) {
                         id, a, b.n[a]);
     c->plan.blocks.push_back(b);
     c->accel.push_back({0.0, 0.0, 0.0});
+    c->btau.push_back(std::nan(""));
+    c->bdt.push_back(std::nan(""));
     return 0;
 }
 
@@ -415,7 +412,19 @@ int mplbhip_define_connection(mplbhip_ctx* c, int from, int fsurf, int to,
             return fail(c, MPLBHIP_ERR_ARG,
                         "a neighbour is already defined for block %d surface %d",
                         from, fsurf);
-    c->plan.links.push_back(LinkDesc{from, fsurf, to, tsurf, vtype});
+    c->plan.links.push_back(LinkDesc{from, fsurf, to, tsurf, vtype, 0});
+    return 0;
+}
+
+int mplbhip_define_rotated_connection(mplbhip_ctx* c, int from, int fsurf, int to,
+                                      int tsurf, int vtype) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    int sf, st;
+    const int af = face_axis(fsurf, &sf), at = face_axis(tsurf, &st);
+    if (af < 0 || at < 0 || af == at || af >= c->dim || at >= c->dim)
+        return fail(c, MPLBHIP_ERR_ARG, "a rotated connection joins two perpendicular faces");
+    if (int rc = mplbhip_define_connection(c, from, fsurf, to, tsurf, vtype)) return rc;
+    c->plan.links.back().rot = 1;
     return 0;
 }
 
@@ -477,32 +486,43 @@ static int setup_halos(mplbhip_ctx* c) {
     const int hz = c->dim == 3 ? 1 : 0;
     struct Slab { int b; int org[3]; int ext[3]; };
     std::vector<Slab> src(seq.size()), dst(seq.size());
+    std::vector<LinkMap> maps(seq.size());
     for (size_t i = 0; i < seq.size(); i++) {
         const LinkDesc& l = seq[i];
         const BlockDef& A = c->plan.blocks[l.from];
         const BlockDef& T = c->plan.blocks[l.to];
-        const int ax = (l.fsurf == BS_Left || l.fsurf == BS_Right) ? 0
-                     : (l.fsurf == BS_Top || l.fsurf == BS_Bottom) ? 1 : 2;
-        if (ax == 2 && c->dim == 2)
-            return fail(c, MPLBHIP_ERR_ARG, "Front/Back connection in 2D");
+        LinkMap& m = maps[i];
+        if (!link_map(l, m) || m.af >= c->dim)
+            return fail(c, MPLBHIP_ERR_ARG, "connection of block %d names no face of a %dD block",
+                        l.from, c->dim);
         for (int d = 0; d < c->dim; d++)
-            if (d != ax && A.n[d] != T.n[d])
-                return fail(c, MPLBHIP_ERR_ARG,
-                            "connected blocks %d and %d differ across the face",
-                            l.from, l.to);
+            for (int j = 0; j < c->dim; j++)
+                if (d != m.af && m.R[j][d] != 0 && A.n[d] != T.n[j])
+                    return fail(c, MPLBHIP_ERR_ARG,
+                                "connected blocks %d and %d differ across the face",
+                                l.from, l.to);
+        // source: the from-face plane with the transverse halo rows
         Slab s{l.from, {-1, -1, -hz}, {A.n[0] + 2, A.n[1] + 2, A.n[2] + 2 * hz}};
-        Slab d{l.to, {-1, -1, -hz}, {A.n[0] + 2, A.n[1] + 2, A.n[2] + 2 * hz}};
-        switch (l.fsurf) {
-            case BS_Right: s.org[0] = A.n[0] - 1; d.org[0] = -1; break;
-            case BS_Left: s.org[0] = 0; d.org[0] = T.n[0]; break;
-            case BS_Bottom: s.org[1] = 0; d.org[1] = T.n[1]; break;
-            case BS_Top: s.org[1] = A.n[1] - 1; d.org[1] = -1; break;
-            case BS_Back: s.org[2] = 0; d.org[2] = T.n[2]; break;
-            case BS_Front: s.org[2] = A.n[2] - 1; d.org[2] = -1; break;
-            default: break;
+        s.org[m.af] = m.sf > 0 ? A.n[m.af] - 1 : 0;
+        s.ext[m.af] = 1;
+        // target: bounding box of the images of the slab's corners
+        Slab d{l.to, {0, 0, 0}, {1, 1, 1}};
+        int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+        for (int corner = 0; corner < 8; corner++) {
+            int p[3], q[3];
+            for (int a = 0; a < 3; a++)
+                p[a] = s.org[a] + (((corner >> a) & 1) ? s.ext[a] - 1 : 0);
+            link_target(m, T.n, p, q);
+            if (c->dim == 2) q[2] = 0;
+            for (int a = 0; a < 3; a++) {
+                lo[a] = std::min(lo[a], q[a]);
+                hi[a] = std::max(hi[a], q[a]);
+            }
         }
-        s.ext[ax] = 1;
-        d.ext[ax] = 1;
+        for (int a = 0; a < 3; a++) {
+            d.org[a] = lo[a];
+            d.ext[a] = hi[a] - lo[a] + 1;
+        }
         src[i] = s;
         dst[i] = d;
     }
@@ -549,10 +569,31 @@ static int setup_halos(mplbhip_ctx* c) {
         h.hp.dst_base = DB.base;
         h.hp.src_sx = SB.sx; h.hp.src_sz = SB.sz; h.hp.src_sq = SB.sq;
         h.hp.dst_sx = DB.sx; h.hp.dst_sz = DB.sz; h.hp.dst_sq = DB.sq;
+        const LinkMap& m = maps[i];
+        const BlockDef& T = c->plan.blocks[dst[i].b];
+        const int64_t stride[3] = {1, DB.sx, DB.sz};
+        int q0[3];
+        link_target(m, T.n, s.org, q0);
+        if (c->dim == 2) q0[2] = 0;
+        h.hp.dst_org = 0;
         for (int a = 0; a < 3; a++) {
             h.hp.org_src[a] = s.org[a];
-            h.hp.org_dst[a] = dst[i].org[a];
             h.hp.ext[a] = s.ext[a];
+            h.hp.dst_org += q0[a] * stride[a];
+            h.hp.dstep[a] = 0;
+            for (int j = 0; j < 3; j++)
+                if (a != m.af && m.R[j][a] != 0) h.hp.dstep[a] = m.R[j][a] * stride[j];
+        }
+        const int (*cv)[3] = c->dim == 3 ? kD3Q19 : kD2Q9;
+        for (int q = 0; q < c->Q; q++) {
+            int v[3] = {0, 0, 0};
+            for (int j = 0; j < 3; j++)
+                for (int k = 0; k < 3; k++) v[j] += m.R[j][k] * cv[q][k];
+            h.hp.qmap[q] = -1;
+            for (int r = 0; r < c->Q; r++)
+                if (cv[r][0] == v[0] && cv[r][1] == v[1] && cv[r][2] == v[2]) h.hp.qmap[q] = r;
+            if (h.hp.qmap[q] < 0)
+                return fail(c, MPLBHIP_ERR_ARG, "rotation leaves the velocity set");
         }
         h.hp.Q = c->Q;
         h.hp.elem_bytes = (int)c->es;
@@ -634,6 +675,11 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     if (int rc = dalloc(c, c->blocks, nb * sizeof(BlockParams))) return rc;
     if (int rc = dalloc(c, c->partials, (size_t)tiles * kNAcc * sizeof(double))) return rc;
     if (int rc = dalloc(c, c->red, (size_t)(nb + 1) * kNAcc * sizeof(double))) return rc;
+    for (int i = 0; i < 2; i++) {
+        if (int rc = dalloc(c, c->parity[i], (size_t)std::max(nb, 16) * sizeof(int32_t)))
+            return rc;
+        HIPCHK(c, launch_fill(c->parity[i].p, (int64_t)c->parity[i].bytes, c->stream));
+    }
     if (int rc = upload_blocks(c)) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
@@ -736,9 +782,11 @@ int mplbhip_set_accel(mplbhip_ctx* c, int block, const double* g) {
 
 // the collision of f^0 happens here, with the time step then in force
 static int first_collision(mplbhip_ctx* c) {
+    // every block starts from buffer 0
+    for (int i = 0; i < 2; i++)
+        HIPCHK(c, launch_fill(c->parity[i].p, (int64_t)c->parity[i].bytes, c->stream));
+    c->cur = 0;
     KParams kp = base_params(c);
-    kp.fs_next = c->fs[0].p;
-    kp.per_next = c->per[0].p;
     int mode = MODE_INIT_FEQ;
     if (c->init_from_f) {
         const size_t fbytes = (size_t)c->Q * c->interior_nodes * sizeof(double);
@@ -751,7 +799,6 @@ static int first_collision(mplbhip_ctx* c) {
     }
     HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->stream));
     if (int rc = exchange(c, 0)) return rc;
-    c->cur = 0;
     return 0;
 }
 
@@ -771,10 +818,6 @@ static int run_steps(mplbhip_ctx* c, int nsteps, bool timed, double* ms_total,
         HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
         for (int s = 0; s < nsteps; s++) {
             KParams kp = base_params(c);
-            kp.fs_prev = c->fs[c->cur].p;
-            kp.fs_next = c->fs[1 - c->cur].p;
-            kp.per_prev = c->per[c->cur].p;
-            kp.per_next = c->per[1 - c->cur].p;
             HIPCHK(c, hipEventRecord(evs[2 * s], c->stream));
             HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP, kp, c->stream));
             HIPCHK(c, hipEventRecord(evs[2 * s + 1], c->stream));
@@ -810,8 +853,7 @@ static int run_steps(mplbhip_ctx* c, int nsteps, bool timed, double* ms_total,
         }
     }
     while (left > 0) {
-        if (int rc = enqueue_step(c, c->cur, false)) return rc;
-        c->cur ^= 1;
+        if (int rc = enqueue_step(c, false, false)) return rc;
         c->tstep++;
         left--;
     }
@@ -842,8 +884,7 @@ int mplbhip_iterate(mplbhip_ctx* c, double criteria, int check_period,
         if (done >= max_steps) break;
         if (c->tstep == 0)
             if (int rc = first_collision(c)) return rc;
-        if (int rc = enqueue_check(c, c->cur, converge)) return rc;
-        c->cur ^= 1;
+        if (int rc = enqueue_step(c, true, converge)) return rc;
         c->tstep++;
         done++;
         if (converge) {
@@ -871,6 +912,101 @@ int mplbhip_iterate(mplbhip_ctx* c, double criteria, int check_period,
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (steps_done) *steps_done = done;
     if (residual) *residual = res;
+    return 0;
+}
+
+int mplbhip_define_block_collision(mplbhip_ctx* c, int block, double tau_ref, double dt) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    if (block < 0 || block >= (int)c->btau.size())
+        return fail(c, MPLBHIP_ERR_ARG, "undefined block %d", block);
+    if (!(dt > 0) || !(tau_ref > 0))
+        return fail(c, MPLBHIP_ERR_ARG, "tau and dt must be positive");
+    c->btau[block] = tau_ref;
+    c->bdt[block] = dt;
+    if (c->prepared && c->device >= 0) {
+        drop_graphs(c);
+        return upload_blocks(c);
+    }
+    return 0;
+}
+
+int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
+                          long long max_steps, long long* steps_b, double* res_b,
+                          long long* steps_done) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    if (int rc = need_device(c)) return rc;
+    if (int rc = ensure_init(c)) return rc;
+    if (check_period <= 0 || max_steps < 0 || criteria < 0)
+        return fail(c, MPLBHIP_ERR_ARG, "bad criteria, check period or step limit");
+    if (!c->plan.links.empty())
+        return fail(c, MPLBHIP_ERR_UNSUPPORTED,
+                    "instances with block connections cannot be frozen one by one");
+    const int nb = (int)c->bp.size();
+    std::vector<long long> conv(nb, -1);
+    std::vector<double> res(nb, -1.0);
+    for (int b = 0; b < nb; b++) c->bp[b].active = 1;
+    if (int rc = upload_blocks(c)) return rc;
+    long long done = 0;
+    int remaining = nb;
+    std::vector<double> acc((size_t)(nb + 1) * kNAcc);
+    while (done < max_steps && remaining > 0) {
+        long long to_check = check_period - (c->tstep % check_period) - 1;
+        long long plain = std::min<long long>(to_check, max_steps - done);
+        while (plain > 0) {
+            const int chunk = (int)std::min<long long>(plain, 1 << 20);
+            if (int rc = run_steps(c, chunk, false, nullptr, nullptr)) return rc;
+            plain -= chunk;
+            done += chunk;
+        }
+        if (done >= max_steps) break;
+        if (c->tstep == 0)
+            if (int rc = first_collision(c)) return rc;
+        if (int rc = enqueue_step(c, true, true)) return rc;
+        c->tstep++;
+        done++;
+        HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
+                                static_cast<const BlockParams*>(c->blocks.p), nb,
+                                c->ntiles, static_cast<double*>(c->red.p), c->stream));
+        HIPCHK(c, hipMemcpyAsync(acc.data(), c->red.p, acc.size() * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        bool changed = false;
+        for (int b = 0; b < nb; b++) {
+            if (!c->bp[b].active) continue;
+            const double* a = acc.data() + (size_t)b * kNAcc;
+            double mx = 0.0;
+            for (int v = 1; v < 1 + c->dim; v++) {
+                const double rel = (a[v] / a[4 + v]) / (check_period * c->bp[b].dt);
+                if (mx <= rel) mx = rel;
+            }
+            res[b] = mx;
+            if (mx < criteria) {
+                c->bp[b].active = 0;
+                conv[b] = c->tstep;
+                remaining--;
+                changed = true;
+            }
+        }
+        if (changed)
+            HIPCHK(c, hipMemcpyAsync(c->blocks.p, c->bp.data(), nb * sizeof(BlockParams),
+                                     hipMemcpyHostToDevice, c->stream));
+    }
+    // freezing lasts for this call only: later steps, macros and f reads
+    // see every instance again
+    bool frozen = false;
+    for (int b = 0; b < nb; b++) {
+        frozen = frozen || !c->bp[b].active;
+        c->bp[b].active = 1;
+    }
+    if (frozen)
+        HIPCHK(c, hipMemcpyAsync(c->blocks.p, c->bp.data(), nb * sizeof(BlockParams),
+                                 hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (steps_b)
+        for (int b = 0; b < nb; b++) steps_b[b] = conv[b];
+    if (res_b)
+        for (int b = 0; b < nb; b++) res_b[b] = res[b];
+    if (steps_done) *steps_done = done;
     return 0;
 }
 

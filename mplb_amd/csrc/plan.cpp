@@ -447,4 +447,52 @@ bool Plan::build() {
     return true;
 }
 
+// ---- halo links ------------------------------------------------------------
+int face_axis(int s, int* side) {
+    *side = 0;
+    switch (s) {
+        case BS_Left: *side = -1; return 0;
+        case BS_Right: *side = 1; return 0;
+        case BS_Bottom: *side = -1; return 1;
+        case BS_Top: *side = 1; return 1;
+        case BS_Back: *side = -1; return 2;
+        case BS_Front: *side = 1; return 2;
+        default: return -1;
+    }
+}
+
+bool link_map(const LinkDesc& l, LinkMap& m) {
+    for (int j = 0; j < 3; j++)
+        for (int k = 0; k < 3; k++) m.R[j][k] = 0;
+    m.af = face_axis(l.fsurf, &m.sf);
+    if (m.af < 0) return false;
+    if (!l.rot) {
+        for (int k = 0; k < 3; k++) m.R[k][k] = 1;
+        m.at = m.af;
+        m.st = -m.sf;
+        return true;
+    }
+    m.at = face_axis(l.tsurf, &m.st);
+    if (m.at < 0 || m.at == m.af) return false;
+    const int a3 = 3 - m.af - m.at;
+    const int sigma = -m.sf * m.st;  // R (sf e_af) = -st e_at
+    m.R[m.at][m.af] = sigma;
+    m.R[m.af][m.at] = -sigma;        // det R = +1
+    m.R[a3][a3] = 1;
+    return true;
+}
+
+void link_target(const LinkMap& m, const int nt[3], const int p[3], int out[3]) {
+    for (int k = 0; k < 3; k++) {
+        if (k == m.af) {
+            out[m.at] = m.st > 0 ? nt[m.at] : -1;
+            continue;
+        }
+        for (int j = 0; j < 3; j++) {
+            if (m.R[j][k] == 1) out[j] = p[k];
+            if (m.R[j][k] == -1) out[j] = nt[j] - 1 - p[k];
+        }
+    }
+}
+
 }  // namespace mplbhip

@@ -67,7 +67,24 @@ struct RegionProg {
 // ---- host-side case description -----------------------------------------
 struct LinkDesc {
     int from, fsurf, to, tsurf, type;
+    int rot;  // 0: reference link (same axis); 1: perpendicular faces, turned
 };
+
+// Geometry of a halo link: R (new = R old, a signed permutation) and the
+// face axes/sides.  A reference link (rot = 0) is the identity into the
+// halo opposite fsurf, whatever tsurf says (Field::CreateHalos uses fsurf
+// only); a rotated link turns by the quarter rotation about the third axis
+// that takes the from-face's outward normal to the to-face's inward one.
+struct LinkMap {
+    int R[3][3];
+    int af, sf;  // from-face axis and outward side (+1 high, -1 low)
+    int at, st;  // target halo axis and side of the target block
+};
+int face_axis(int surface, int* side);  // -1 for non-face surfaces
+bool link_map(const LinkDesc& l, LinkMap& m);
+// target-block node receiving from-block node p of the from-face plane
+// (transverse halo rows included); nt = target block size
+void link_target(const LinkMap& m, const int nt[3], const int p[3], int out[3]);
 struct BCDesc {
     int block, surface, scheme, vtype, nvars;
     double vars[8];

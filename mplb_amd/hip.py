@@ -54,6 +54,7 @@ def load():
         "mplbhip_version": (ctypes.c_char_p, []),
         "mplbhip_define_block": (I, [P, I, IP]),
         "mplbhip_define_connection": (I, [P, I, I, I, I, I]),
+        "mplbhip_define_rotated_connection": (I, [P, I, I, I, I, I]),
         "mplbhip_define_collision": (I, [P, D, I, I]),
         "mplbhip_define_boundary": (I, [P, I, I, I, DP, I, I]),
         "mplbhip_prepare": (I, [P]),
@@ -75,6 +76,8 @@ def load():
         "mplbhip_copy_probe": (I, [I, ctypes.c_longlong, I, DP]),
         "mplbhip_iterate": (I, [P, D, I, ctypes.c_longlong, LLP, DP]),
         "mplbhip_last_residual": (I, [P, DP, DP]),
+        "mplbhip_define_block_collision": (I, [P, I, D, D]),
+        "mplbhip_iterate_batch": (I, [P, D, I, ctypes.c_longlong, LLP, DP, LLP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -139,6 +142,10 @@ class LatticeCase:
     def add_connection(self, frm, fsurf, to, tsurf, vtype):
         self._ck(self.lib.mplbhip_define_connection(self.h, frm, int(fsurf), to,
                                                     int(tsurf), int(vtype)))
+
+    def add_rotated_connection(self, frm, fsurf, to, tsurf, vtype):
+        self._ck(self.lib.mplbhip_define_rotated_connection(self.h, frm, int(fsurf), to,
+                                                            int(tsurf), int(vtype)))
 
     def set_collision(self, tau_ref, force_type=0, force_macros=False):
         self._ck(self.lib.mplbhip_define_collision(self.h, tau_ref, int(force_type),
@@ -209,6 +216,24 @@ class LatticeCase:
                                           int(max_steps), ctypes.byref(done),
                                           ctypes.byref(res)))
         return done.value, res.value
+
+    def set_block_collision(self, block, tau_ref, dt):
+        self._ck(self.lib.mplbhip_define_block_collision(self.h, block, float(tau_ref),
+                                                         float(dt)))
+
+    def iterate_batch(self, criteria, check_period, max_steps):
+        """Advance all instances, freezing each at its own convergence.
+        Returns (steps per block (-1: not converged), residual per block,
+        steps taken)."""
+        nb = len(self.sizes)
+        steps = np.zeros(nb, dtype=np.int64)
+        res = np.zeros(nb)
+        done = ctypes.c_longlong()
+        self._ck(self.lib.mplbhip_iterate_batch(
+            self.h, float(criteria), int(check_period), int(max_steps),
+            steps.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _dp(res),
+            ctypes.byref(done)))
+        return steps, res, done.value
 
     def update_macros(self):
         self._ck(self.lib.mplbhip_update_macros(self.h))

@@ -73,7 +73,7 @@ typedef struct {
     double g[3];       /* uniform acceleration */
 } OBlock;
 
-typedef struct { int from, fsurf, to, tsurf, type; } OLink;
+typedef struct { int from, fsurf, to, tsurf, type, rot; } OLink;
 typedef struct { int block, surface, scheme, vtype, nvars;
                  double vars[8]; } OBC;
 
@@ -263,9 +263,95 @@ int orc_add_connection(orc_case* c, int from, int fsurf, int to, int tsurf,
     for (int l = 0; l < c->nlink; l++) /* Block::AddNeighbor block.cpp:141 */
         if (c->link[l].from == from && c->link[l].fsurf == fsurf)
             return fail(c, "a neighbour is already defined on this surface");
-    OLink L = {from, fsurf, to, tsurf, type};
+    OLink L = {from, fsurf, to, tsurf, type, 0};
     c->link[c->nlink++] = L;
     return 0;
+}
+
+/* face surface -> axis and outward side; -1 for edge surfaces */
+static int face_axis(int s, int* side) {
+    *side = 0;
+    switch (s) {
+        case BS_LEFT: *side = -1; return 0;
+        case BS_RIGHT: *side = 1; return 0;
+        case BS_BOTTOM: *side = -1; return 1;
+        case BS_TOP: *side = 1; return 1;
+        case BS_BACK: *side = -1; return 2;
+        case BS_FRONT: *side = 1; return 2;
+        default: return -1;
+    }
+}
+
+/* Extension with no reference counterpart (the reference links same-axis
+ * faces only, field.h:207-214): fStage of the from-face plane goes into the
+ * halo beyond the to-face, turned by the quarter rotation R about the third
+ * axis that takes the from-face's outward normal n_f to the to-face's inward
+ * normal -n_t.  Populations move with it: f'_{q'} = f_q where c_q' = R c_q. */
+int orc_add_rotated_connection(orc_case* c, int from, int fsurf, int to,
+                               int tsurf, int type) {
+    int sf, st;
+    const int af = face_axis(fsurf, &sf), at = face_axis(tsurf, &st);
+    if (af < 0 || at < 0 || af == at || af >= c->dim || at >= c->dim)
+        return fail(c, "a rotated connection joins two perpendicular faces");
+    if (orc_add_connection(c, from, fsurf, to, tsurf, type)) return -1;
+    c->link[c->nlink - 1].rot = 1;
+    return 0;
+}
+
+/* R (new = R old) of a link: identity for reference links */
+static void link_rotation(const OLink* L, int R[3][3]) {
+    memset(R, 0, 9 * sizeof(int));
+    if (!L->rot) {
+        for (int k = 0; k < 3; k++) R[k][k] = 1;
+        return;
+    }
+    int sf, st;
+    const int af = face_axis(L->fsurf, &sf), at = face_axis(L->tsurf, &st);
+    const int a3 = 3 - af - at;
+    const int sigma = -sf * st;
+    R[at][af] = sigma;  /* n_f = sf e_af  ->  -st e_at */
+    R[af][at] = -sigma; /* det R = +1 */
+    R[a3][a3] = 1;
+}
+
+/* target node of from-face-plane node p (transverse halo rows included) */
+static void link_target(const orc_case* c, const OLink* L, const int p[3],
+                        int out[3]) {
+    const OBlock* t = &c->b[L->to];
+    int R[3][3], sf, st;
+    link_rotation(L, R);
+    const int af = face_axis(L->fsurf, &sf);
+    int at = face_axis(L->tsurf, &st);
+    if (!L->rot) { at = af; st = -sf; } /* the reference ignores tsurf */
+    for (int k = 0; k < 3; k++) {
+        if (k == af) {
+            out[at] = st > 0 ? t->n[at] : -1;
+            continue;
+        }
+        for (int j = 0; j < 3; j++) {
+            if (R[j][k] == 1) out[j] = p[k];
+            if (R[j][k] == -1) out[j] = t->n[j] - 1 - p[k];
+        }
+    }
+    if (c->dim == 2) out[2] = 0;
+}
+
+/* population map of a link: qmap[q] = q' with c_q' = R c_q */
+static void link_qmap(const orc_case* c, const OLink* L, int* qmap) {
+    int R[3][3];
+    link_rotation(L, R);
+    const int D = c->dim;
+    for (int q = 0; q < c->Q; q++) {
+        int v[3] = {0, 0, 0};
+        for (int j = 0; j < D; j++)
+            for (int k = 0; k < D; k++) v[j] += R[j][k] * (int)c->XI[q * D + k];
+        qmap[q] = -1;
+        for (int r = 0; r < c->Q; r++) {
+            int same = 1;
+            for (int j = 0; j < D; j++) same = same && (int)c->XI[r * D + j] == v[j];
+            if (same) { qmap[q] = r; break; }
+        }
+    }
 }
 
 int orc_set_collision(orc_case* c, double tau_ref, int force_type,
@@ -435,11 +521,16 @@ int orc_prepare(orc_case* c) {
         const OLink* L = &c->link[l];
         const OBlock* a = &c->b[L->from];
         const OBlock* t = &c->b[L->to];
-        int ax = (L->fsurf == BS_LEFT || L->fsurf == BS_RIGHT) ? 0
-               : (L->fsurf == BS_TOP || L->fsurf == BS_BOTTOM) ? 1 : 2;
+        int side;
+        const int ax = face_axis(L->fsurf, &side);
+        if (ax < 0 || ax >= c->dim)
+            return fail(c, "a connection joins face surfaces");
+        int R[3][3];
+        link_rotation(L, R);
         for (int d = 0; d < c->dim; d++)
-            if (d != ax && a->n[d] != t->n[d])
-                return fail(c, "connected blocks differ in a transverse size");
+            for (int j = 0; j < c->dim; j++)
+                if (d != ax && R[j][d] != 0 && a->n[d] != t->n[j])
+                    return fail(c, "connected blocks differ in a transverse size");
     }
     for (int ib = 0; ib < c->nb; ib++) {
         set_geometry(c, &c->b[ib]);
@@ -628,22 +719,24 @@ static void halo_transfer(orc_case* c) {
                 OBlock* t = &c->b[L->to];
                 const int hz = c->dim == 3 ? 1 : 0;
                 int sz[3] = {a->n[0] + 2, a->n[1] + 2, a->n[2] + 2 * hz};
-                int src[3] = {-1, -1, -hz}, dst[3] = {-1, -1, -hz};
-                switch (L->fsurf) {
-                    case BS_RIGHT: sz[0] = 1; src[0] = a->n[0] - 1; dst[0] = -1; break;
-                    case BS_LEFT:  sz[0] = 1; src[0] = 0; dst[0] = t->n[0]; break;
-                    case BS_BOTTOM: sz[1] = 1; src[1] = 0; dst[1] = t->n[1]; break;
-                    case BS_TOP:   sz[1] = 1; src[1] = a->n[1] - 1; dst[1] = -1; break;
-                    case BS_BACK:  sz[2] = 1; src[2] = 0; dst[2] = t->n[2]; break;
-                    case BS_FRONT: sz[2] = 1; src[2] = a->n[2] - 1; dst[2] = -1; break;
-                }
+                int src[3] = {-1, -1, -hz};
+                int side;
+                const int ax = face_axis(L->fsurf, &side);
+                if (ax < 0) continue; /* rejected by orc_prepare */
+                sz[ax] = 1;
+                src[ax] = side > 0 ? a->n[ax] - 1 : 0;
+                int qmap[19];
+                link_qmap(c, L, qmap);
                 for (int z = 0; z < sz[2]; z++)
                     for (int y = 0; y < sz[1]; y++)
                         for (int x = 0; x < sz[0]; x++) {
-                            size_t ns = nidx(a, src[0] + x, src[1] + y, src[2] + z);
-                            size_t nd = nidx(t, dst[0] + x, dst[1] + y, dst[2] + z);
-                            memcpy(&t->fs[nd * Q], &a->fs[ns * Q],
-                                   Q * sizeof(double));
+                            const int p[3] = {src[0] + x, src[1] + y, src[2] + z};
+                            int d[3];
+                            link_target(c, L, p, d);
+                            const size_t ns = nidx(a, p[0], p[1], p[2]);
+                            const size_t nd = nidx(t, d[0], d[1], d[2]);
+                            for (int q = 0; q < Q; q++)
+                                t->fs[nd * Q + qmap[q]] = a->fs[ns * Q + q];
                         }
             }
         }

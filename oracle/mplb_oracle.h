@@ -31,6 +31,11 @@ int orc_add_block(orc_case* c, int id, int nx, int ny, int nz);
 /* DefineBlockConnection (flowfield.cpp:258-283) */
 int orc_add_connection(orc_case* c, int from_block, int from_surface,
                        int to_block, int to_surface, int vertex_type);
+/* Extension with no reference counterpart: a connection between two
+ * perpendicular faces, turned by the quarter rotation about the third axis
+ * that takes the from-face's outward normal to the to-face's inward one. */
+int orc_add_rotated_connection(orc_case* c, int from_block, int from_surface,
+                               int to_block, int to_surface, int vertex_type);
 /* DefineComponents tauRef + DefineBodyForce (model.cpp:208-310, 447-489).
  * force_type: 0 = BodyForce_None, 1 = BodyForce_1st.
  * force_macros: 1 selects the Variable_*_Force velocity kernels

@@ -37,6 +37,7 @@ def load():
         "orc_error": (ctypes.c_char_p, [P]),
         "orc_add_block": (I, [P, I, I, I, I]),
         "orc_add_connection": (I, [P, I, I, I, I, I]),
+        "orc_add_rotated_connection": (I, [P, I, I, I, I, I]),
         "orc_set_collision": (I, [P, D, I, I]),
         "orc_add_boundary": (I, [P, I, I, I, DP, I, I]),
         "orc_prepare": (I, [P]),
@@ -99,6 +100,10 @@ class OracleCase:
     def add_connection(self, frm, fsurf, to, tsurf, vtype):
         self._ck(self.lib.orc_add_connection(self.h, frm, int(fsurf), to,
                                              int(tsurf), int(vtype)))
+
+    def add_rotated_connection(self, frm, fsurf, to, tsurf, vtype):
+        self._ck(self.lib.orc_add_rotated_connection(self.h, frm, int(fsurf), to,
+                                                     int(tsurf), int(vtype)))
 
     def set_collision(self, tau_ref, force_type=0, force_macros=False):
         self._ck(self.lib.orc_set_collision(self.h, tau_ref, int(force_type),

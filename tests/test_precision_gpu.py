@@ -31,7 +31,7 @@ def _run(storage, build, steps):
 
 
 @pytest.mark.parametrize("storage,tol_u,tol_rho", [
-    (hip.STORE_F32, 2e-5, 2e-6),
+    (hip.STORE_F32, 1e-4, 1e-5),
     (hip.STORE_F16, 5e-3, 5e-4),
 ])
 def test_reduced_storage_cavity2d_vs_oracle(storage, tol_u, tol_rho):
@@ -64,7 +64,7 @@ def test_fp32_cavity3d_vs_oracle():
              for v in (1, 2, 3)) / p["lid"]
     df = float(np.max(np.abs(g.get_f(0) - o.get_f(0))))
     log("FP32 3D: max|du|/lid = %.3e, max|df| = %.3e" % (du, df))
-    assert du <= 2e-5 and df <= 1e-6
+    assert du <= 2e-5 and df <= 5e-6
 
 
 def test_fp32_mass_drift_periodic():
