@@ -3,9 +3,11 @@
 D3Q19, 256^3 nodes, Re = 400, lid speed 0.05 (lattice units), rest state plus
 seeded density noise +-1e-4, populations stored as f - w_i in FP32.
 
-One step = one fused pull stream-collide sweep over the whole lattice (the
-boundary programs run inside the same kernel).  Inputs are resident in HBM
-before the timed region.  Prints one JSON line.
+One step = the bulk kernel (fused pull stream + macros + BGK collision over
+the interior nodes) and the boundary kernel (the region programs of the
+block's faces, edges and corners).  Inputs are resident in HBM before the
+timed region.  The roofline figure is that of the bulk kernel, which moves
+(n-2)^3 nodes x 19 populations x (read + write).  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps 2000] [--warmup 50]
 """
@@ -47,20 +49,145 @@ def traffic_from_profiles(name):
         return None
 
 
-def cpu_baseline(n, steps):
+def cpu_baseline(n, seconds):
     """The FP64 CPU oracle (a port of the reference step) on the host cores,
-    on the full 256^3 domain for a few steps."""
+    on the full n^3 domain, for as many steps as fill about `seconds`."""
     from oracle.oracle import OracleCase, get_threads
     o = OracleCase(3)
     cases.cavity3d(o, n, re=400.0, ulid=0.05, noise=True)
-    o.step(1)  # first touch of every array
+    t0 = time.perf_counter()
+    o.step(1)  # first touch of every array, and the step-time estimate
+    one = time.perf_counter() - t0
+    steps = max(1, min(500, int(seconds / max(one, 1e-3))))
     t0 = time.perf_counter()
     o.step(steps)
     dt = time.perf_counter() - t0
-    return {"value": n ** 3 * steps / dt / 1e6, "unit": "MLUPS",
+    return {"value": round(n ** 3 * steps / dt / 1e6, 2), "unit": "MLUPS",
             "cores": get_threads(), "kind": "port",
-            "sample": "FP64 oracle, D3Q19 cavity %d^3, %d steps (%.1f s)"
-                      % (n, steps, dt)}
+            "sample": "FP64 oracle, D3Q19 cavity %d^3 (the benchmark workload), "
+                      "%d steps in %.1f s" % (n, steps, dt)}
+
+
+def bench_lchannel(args):
+    """The closed three-block L-channel (blocks 128x256x128, 128^3,
+    256x128x128 for a=128): rotated periodic inlet/outlet, body force 1e-6,
+    tau 0.8, FP16 storage; FP64 mass every 100 steps; max-abs error against
+    an FP64 run of the same setup."""
+    a, length, K = args.lc_a, 2 * args.lc_a, args.steps
+    st = args.storage
+    log("bench: L-channel a=%d storage=%s steps=%d" % (a, st, K))
+    g = hip.LatticeCase(3, storage=STORE[st], device=0)
+    p = cases.lchannel_periodic(g, a=a, length=length, tau_l=0.8, g=1e-6)
+    nodes = p["nodes"]
+    m0 = g.mass()
+    masses = [m0]
+    t_kernel = t_bulk = t_bnd = 0.0
+    t0 = time.perf_counter()
+    done = 0
+    while done < K:
+        k = min(100, K - done)
+        _, ker = g.step_timed(k)
+        b, d = g.kernel_times()
+        t_kernel += ker
+        t_bulk += b
+        t_bnd += d
+        done += k
+        masses.append(g.mass())
+    g.sync()
+    wall = time.perf_counter() - t0
+    drift = max(abs(m - m0) for m in masses) / m0
+    interior = sum((x - 2) * (y - 2) * (z - 2) for x, y, z in p["sizes"])
+    bulk_bytes = interior * 2 * 19 * ELEM[st]
+    achieved = bulk_bytes / (t_bulk / K / 1e3) / 1e9
+    mlups = nodes * K / (t_kernel / 1e3) / 1e6
+    log("bench: L-channel %.0f MLUPS (step kernels), bulk %.0f GB/s, mass drift %.3e"
+        % (mlups, achieved, drift))
+    err = None
+    if not args.no_error and st != "f64":
+        ref = hip.LatticeCase(3, storage=hip.STORE_F64, device=0)
+        cases.lchannel_periodic(ref, a=a, length=length, tau_l=0.8, g=1e-6)
+        ref.step(K)
+        g.update_macros()
+        ref.update_macros()
+        umax = max(float(np.max(np.abs(ref.get_macro(b, v)))) for b in range(3)
+                   for v in (1, 2, 3))
+        err = {"u_max_abs": max(float(np.max(np.abs(g.get_macro(b, v) - ref.get_macro(b, v))))
+                                for b in range(3) for v in (1, 2, 3)),
+               "rho_max_abs": max(float(np.max(np.abs(g.get_macro(b, 0) - ref.get_macro(b, 0))))
+                                  for b in range(3)),
+               "u_max_fp64": umax,
+               "mass_rel": abs(g.mass() - ref.mass()) / ref.mass()}
+        ref.close()
+        log("bench: L-channel error vs FP64: %s" % err)
+    return {
+        "metric": "MLUPS, three-block L-channel",
+        "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": K,
+        "warmup": 0, "ms_per_step": round(t_kernel / K, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": st,
+        "data": "synthetic: rest state, body-force driven",
+        "config": {"workload": "closed L-channel, blocks %dx%dx%d, %d^3, %dx%dx%d, D3Q19, "
+                               "tau 0.8, g 1e-6, rotated periodic inlet/outlet, %s storage"
+                               % (a, length, a, a, length, a, a, st),
+                   "nodes": nodes},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "k_bulk (D3Q19, %s, step)" % st,
+                     "boundary_and_bulk_ms_per_step": round(t_kernel / K, 5),
+                     "bulk_ms_per_step": round(t_bulk / K, 5)},
+        "mass_drift_max_rel": drift,
+        "mass_every_100": masses,
+        "wall_s": round(wall, 3),
+        "error_vs_fp64": err,
+    }
+
+
+def bench_batch(args):
+    """256 independent D2Q9 cavities (N in {64,96,128,192,256}, Re in
+    [100, 1000], seeded), FP32 storage, one launch per step for all of them:
+    aggregate MLUPS over a timed run, then iterate_batch to the FP64
+    velocity residual 1e-8 with per-instance freezing."""
+    st = args.storage
+    sizes, res = cases.batch_spec(args.batch)
+    nodes = sum(n * n for n in sizes)
+    log("bench: %d cavities, %.2fM nodes, storage %s" % (len(sizes), nodes / 1e6, st))
+    g = hip.LatticeCase(2, storage=STORE[st], device=0)
+    cases.batched_cavities(g, sizes, res)
+    g.step(args.warmup)
+    g.sync()
+    t0 = time.perf_counter()
+    _, ker = g.step_timed(args.steps)
+    wall = time.perf_counter() - t0
+    mlups = nodes * args.steps / wall / 1e6
+    b, d = g.kernel_times()
+    log("bench: batch %.0f MLUPS (wall), kernels %.4f ms/step (bulk %.4f, boundary %.4f)"
+        % (mlups, ker / args.steps, b / args.steps, d / args.steps))
+    conv = None
+    if args.converge:
+        t0 = time.perf_counter()
+        steps, resid, done = g.iterate_batch(1e-8, args.check_period, args.max_steps)
+        tconv = time.perf_counter() - t0
+        ok = steps > 0
+        conv = {"criteria": 1e-8, "check_period": args.check_period,
+                "converged": int(ok.sum()), "of": len(sizes),
+                "steps_min": int(steps[ok].min()) if ok.any() else None,
+                "steps_median": float(np.median(steps[ok])) if ok.any() else None,
+                "steps_max": int(steps[ok].max()) if ok.any() else None,
+                "steps_run": int(done), "wall_s": round(tconv, 2),
+                "per_instance_steps": [int(x) for x in steps]}
+        log("bench: batch convergence %s" % {k: v for k, v in conv.items()
+                                              if k != "per_instance_steps"})
+    return {
+        "metric": "aggregate MLUPS, batched cavities",
+        "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": st,
+        "data": "synthetic: %d seeded cavities at rest" % len(sizes),
+        "config": {"workload": "%d D2Q9 lid-driven cavities, N in {64..256}, Re in "
+                               "[100,1000], lid 0.05, %s storage" % (len(sizes), st),
+                   "nodes": nodes},
+        "kernel_ms_per_step": round(ker / args.steps, 5),
+        "convergence": conv,
+    }
 
 
 def main():
@@ -70,16 +197,31 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--storage", choices=sorted(STORE), default="f32")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true",
                     help="skip the FP64 comparison run")
+    ap.add_argument("--workload", choices=("cavity3d", "lchannel", "batch"),
+                    default="cavity3d",
+                    help="cavity3d: the headline line; lchannel and batch: the "
+                         "multi-block and batched workloads")
+    ap.add_argument("--lc-a", type=int, default=128, help="L-channel cross-section")
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--converge", action="store_true")
+    ap.add_argument("--check-period", type=int, default=1000)
+    ap.add_argument("--max-steps", type=int, default=400000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the multi-GPU path "
               "(face-slab halos over xGMI) is a design, not a run. Use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "lchannel":
+        print(json.dumps(bench_lchannel(args)), flush=True)
+        return
+    if args.workload == "batch":
+        print(json.dumps(bench_batch(args)), flush=True)
+        return
 
     n, K, W = args.n, args.steps, args.warmup
     st = args.storage
@@ -93,14 +235,16 @@ def main():
     ms_gpu_span, ms_kernel = g.step_timed(K)
     g.sync()
     wall = time.perf_counter() - t0
+    ms_bulk, ms_bnd = g.kernel_times()
     nodes = n ** 3
     mlups = nodes * K / wall / 1e6
-    kavg_s = ms_kernel / K / 1e3
     q = 19
-    bytes_per_launch = nodes * 2 * q * ELEM[st]
-    achieved = bytes_per_launch / kavg_s / 1e9
-    log("bench: wall %.3f s, gpu span %.3f ms, kernel avg %.4f ms, %.0f MLUPS, "
-        "%.0f GB/s" % (wall, ms_gpu_span, kavg_s * 1e3, mlups, achieved))
+    bulk_s = ms_bulk / K / 1e3
+    bulk_bytes = (n - 2) ** 3 * 2 * q * ELEM[st]  # algorithmic, per launch
+    achieved = bulk_bytes / bulk_s / 1e9
+    log("bench: wall %.3f s, gpu span %.3f ms, bulk kernel avg %.4f ms (%.0f GB/s), "
+        "boundary kernel avg %.4f ms, %.0f MLUPS"
+        % (wall, ms_gpu_span, bulk_s * 1e3, achieved, ms_bnd / K, mlups))
 
     copy_bytes = 2 << 30
     copy_ms = hip.copy_probe(copy_bytes, iters=20)
@@ -128,8 +272,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log("bench: CPU oracle baseline (%d steps at %d^3)" % (args.cpu_steps, n))
-        cpu = cpu_baseline(n, args.cpu_steps)
+        log("bench: CPU oracle baseline (about %.0f s at %d^3)" % (args.cpu_seconds, n))
+        cpu = cpu_baseline(n, args.cpu_seconds)
 
     out = {
         "metric": METRIC,
@@ -152,8 +296,12 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles(
-                         "k_lattice_d3q19_%d_%s_traffic.json" % (n, st)),
-                     "kernel_avg_ms": round(kavg_s * 1e3, 5),
+                         "k_bulk_d3q19_%d_%s_traffic.json" % (n, st)),
+                     "kernel": "k_bulk (D3Q19, %s, step)" % st,
+                     "algorithmic_bytes": bulk_bytes,
+                     "kernel_avg_ms": round(bulk_s * 1e3, 5),
+                     "boundary_kernel_avg_ms": round(ms_bnd / K, 5),
+                     "step_kernels_avg_ms": round(ms_kernel / K, 5),
                      "copy_probe_GBs": round(copy_gbs, 1)},
         "cpu_baseline": cpu,
     }

@@ -73,6 +73,15 @@ int mplbhip_define_connection(mplbhip_ctx* ctx, int from_block,
 int mplbhip_define_rotated_connection(mplbhip_ctx* ctx, int from_block,
                                       int from_surface, int to_block,
                                       int to_surface, int vertex_type);
+/* No reference counterpart (the reference only skips ImmersedSolid nodes,
+ * model_kernel.inc:420, scheme_kernel.inc:1418; its bounce-back is
+ * commented out, boundary_kernel.inc:490-532): in-domain obstacles of one
+ * block, interior nodes [k][j][i], nonzero = obstacle.  Obstacle nodes are
+ * ImmersedSolid; a node pulling from one takes the half-way bounce-back
+ * f_q(x) = fs_opp(q)(x) of a resting wall.  Obstacles may sit on Fluid or
+ * MDPeriodic nodes whose pulling neighbours pull every population; prepare
+ * refuses others.  Before mplbhip_prepare. */
+int mplbhip_define_obstacles(mplbhip_ctx* ctx, int block, const unsigned char* solid);
 /* DefineComponents tauRef (model.cpp:236) + DefineCollision
  * (Collision_BGKIsothermal2nd only, model.cpp:406-445) + DefineBodyForce
  * (model.cpp:447-489, force_type BodyForce_None=0 / BodyForce_1st=1) +
@@ -112,9 +121,13 @@ int mplbhip_set_accel(mplbhip_ctx* ctx, int block, const double* g);
  * context stream and returned once complete. */
 int mplbhip_step(mplbhip_ctx* ctx, int nsteps);
 /* nsteps timed with HIP events on the context stream around each step
- * kernel: ms_total is the wall span, ms_kernel the summed step-kernel time */
+ * kernel: ms_total is the wall span, ms_kernel the summed time of the step
+ * kernels (bulk and boundary; halo copies excluded) */
 int mplbhip_step_timed(mplbhip_ctx* ctx, int nsteps, double* ms_total,
                        double* ms_kernel);
+/* Diagnostic, no reference counterpart: the bulk-kernel and boundary-kernel
+ * shares of the last mplbhip_step_timed call (ms, summed over its steps). */
+int mplbhip_kernel_times(mplbhip_ctx* ctx, double* ms_bulk, double* ms_boundary);
 /* Iterate (evolution.cpp:71-135 and the templates evolution.h:78-129).
  * criteria < 0: run exactly max_steps steps, refreshing the macro variables
  * at every check_period-th step (the transient form).  criteria >= 0: step

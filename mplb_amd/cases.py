@@ -319,7 +319,54 @@ def batch_spec(count=256, seed=NOISE_SEED, sizes=BATCH_SIZES, re_range=(100.0, 1
     return [int(x) for x in n], [float(x) for x in re]
 
 
-def batched_cavities(be, sizes, res, ulid=0.05):
+def rect_mask(n, rects):
+    """Obstacle mask [y][x] of an n x n cavity from rectangles (x0, y0, w, h)."""
+    m = np.zeros((n, n), np.uint8)
+    for x0, y0, w, h in rects:
+        m[y0:y0 + h, x0:x0 + w] = 1
+    return m
+
+
+def obstacle_rects(count, n=128, seed=NOISE_SEED, max_rects=3, side=(8, 32)):
+    """Per case: 0..max_rects axis-aligned rectangles with sides in
+    [side[0], side[1]] nodes, kept two nodes clear of the cavity walls (the
+    bounce-back neighbours must be bulk Fluid nodes).  Case i draws from
+    numpy's PCG64 seeded with (seed, i)."""
+    out = []
+    for i in range(count):
+        rng = np.random.default_rng([seed, i])
+        k = int(rng.integers(0, max_rects + 1))
+        rects = []
+        for _ in range(k):
+            w, h = (int(v) for v in rng.integers(side[0], side[1] + 1, size=2))
+            x0 = int(rng.integers(2, n - 2 - w + 1))
+            y0 = int(rng.integers(2, n - 2 - h + 1))
+            rects.append((x0, y0, w, h))
+        out.append(rects)
+    return out
+
+
+def cavity2d_obstacles(be, n, rects, re=100.0, ulid=0.05):
+    """The 2D cavity (Apps/2DCavity) with in-domain obstacles (half-way
+    bounce-back, an extension): Left, Right, Top (lid), Bottom walls."""
+    mesh, dt, tau_l = cavity_params(n, re, ulid)
+    be.add_block(0, (n, n))
+    be.set_collision(tau_ref_for(tau_l, dt), force_type=0)
+    lid = CS * ulid
+    for s_ in ("Left", "Right", "Top", "Bottom"):
+        vel = (lid, 0.0) if s_ == "Top" else (0.0, 0.0)
+        be.add_boundary(0, BS[s_], BSch.EQMDiffuseRefl, vel, VT.Wall)
+    solid = rect_mask(n, rects)
+    be.set_obstacles(0, solid)
+    be.prepare()
+    shp = (n, n)
+    be.set_macros(0, np.ones(shp), np.zeros(shp), np.zeros(shp))
+    be.init_feq()
+    be.set_dt(dt)
+    return dict(dt=dt, tau_l=tau_l, mesh=mesh, lid=lid, solid=solid)
+
+
+def batched_cavities(be, sizes, res, ulid=0.05, masks=None):
     """One D2Q9 cavity per block (the 2D cavity App's boundary order), each
     with its own relaxation time and time step; blocks are unconnected."""
     for b, n in enumerate(sizes):
@@ -333,6 +380,8 @@ def batched_cavities(be, sizes, res, ulid=0.05):
         for s in ("Left", "Right", "Top", "Bottom"):
             vel = (lid, 0.0) if s == "Top" else (0.0, 0.0)
             be.add_boundary(b, BS[s], BSch.EQMDiffuseRefl, vel, VT.Wall)
+        if masks is not None and masks[b] is not None and masks[b].any():
+            be.set_obstacles(b, masks[b])
         params.append(dict(n=n, re=re, dt=dt, tau_l=tau_l))
     be.prepare()
     for b, n in enumerate(sizes):

@@ -193,13 +193,38 @@ __device__ __forceinline__ int64_t persist_slot(const RegionProg& P, int x,
     return (int64_t)P.persist + l0 + (int64_t)P.ext[0] * (l1 + (int64_t)P.ext[1] * l2);
 }
 
+// obstacle link mask of interior node mnode (0 when the block has none)
+__device__ __forceinline__ uint32_t obst_mask(const KParams& kp, const BlockParams& B,
+                                              int64_t mnode) {
+    return (kp.obst && B.has_obst) ? kp.obst[mnode] : 0u;
+}
+
+// residual terms of a node whose macros do not change (ImmersedSolid):
+// (m - m_copy)^2 and m^2, with the copy refreshed, as the whole-range sums
+// of CalcResidualError include them (flowfield_wrapper.cpp:31-89)
+template <int D, int MODE>
+__device__ __forceinline__ void solid_residual(const KParams& kp, int64_t mnode, double* acc) {
+    if constexpr (MODE == MODE_STEP_CHECK || MODE == MODE_MATERIALIZE) {
+        if (kp.want_residual) {
+#pragma unroll
+            for (int v = 0; v < D + 1; v++) {
+                const double m = kp.macro[v][mnode];
+                const double old = kp.macro_copy[v][mnode];
+                acc[v] = (m - old) * (m - old);
+                acc[4 + v] = m * m;
+                kp.macro_copy[v][mnode] = m;
+            }
+        }
+    }
+}
+
 // full-value state of a node at the start of the boundary stage: streamed
 // populations from the neighbours, unstreamed ones from the persist buffer
 template <int D, int ST>
 __device__ void gather_masked(const KParams& kp, const BlockParams& B,
                               const RegionProg& P, int x, int y, int z,
                               typename Store<ST>::T* f, const void* fs_prev,
-                              const void* per_prev) {
+                              const void* per_prev, uint32_t om) {
     using T = typename Store<ST>::T;
     using E = typename Store<ST>::E;
     constexpr int Q = D == 2 ? 9 : 19;
@@ -211,8 +236,12 @@ __device__ void gather_masked(const KParams& kp, const BlockParams& B,
     for (int q = 0; q < Q; q++) {
         T v;
         if (P.pull >> q & 1u) {
-            v = ld<ST>(prev, (int64_t)q * B.sq + node - pull_delta<D>(B, q),
-                       kp.scale[q]);
+            // an obstacle source bounces back the node's own opposite value
+            if (om >> q & 1u)
+                v = ld<ST>(prev, (int64_t)opp(D, q) * B.sq + node, kp.scale[opp(D, q)]);
+            else
+                v = ld<ST>(prev, (int64_t)q * B.sq + node - pull_delta<D>(B, q),
+                           kp.scale[q]);
             if constexpr (ST != MPLBHIP_STORE_F64) v += T(wgt(D, q));
         } else if (P.persist >= 0) {
             v = per[slot * Q + q];
@@ -234,8 +263,9 @@ __device__ void neighbour_values(const KParams& kp, const BlockParams& B,
     constexpr int Q = D == 2 ? 9 : 19;
     const RegionProg& N = kp.progs[B.prog_base + op.nb_region];
     const int nx = x + op.off[0], ny = y + op.off[1], nz = z + op.off[2];
+    const int64_t nm = B.mbase + ((int64_t)nz * B.n[1] + ny) * B.n[0] + nx;
     T g[Q];
-    gather_masked<D, ST>(kp, B, N, nx, ny, nz, g, fs_prev, per_prev);
+    gather_masked<D, ST>(kp, B, N, nx, ny, nz, g, fs_prev, per_prev, obst_mask(kp, B, nm));
     for (int o = 0; o < op.nb_nops; o++) {
         if (N.op[o].type == OP_EQM) apply_eqm<D, T>(g, N.op[o]);
     }
@@ -294,33 +324,219 @@ __device__ __forceinline__ int logical_wg(int wg, int n) {
     return (wg & 7) * (n >> 3) + (wg >> 3);
 }
 
+// block of logical workgroup t: the last block whose first workgroup
+// (field `first`) is <= t
+template <int32_t BlockParams::*first>
 __device__ __forceinline__ int find_block(const KParams& kp, int t) {
     int lo = 0, hi = kp.nblocks - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (kp.blocks[mid].tile_start <= t) lo = mid; else hi = mid - 1;
+        if (kp.blocks[mid].*first <= t) lo = mid; else hi = mid - 1;
     }
     return lo;
 }
 
-// ---- the fused kernel ---------------------------------------------------------
+// macros (+ residual partials), then the MATERIALIZE output or the BGK
+// collision and the store of fStage.  f holds full values when `full`
+// (boundary path), else (reduced storage) deviations g = f - w.
 template <int D, int ST, int MODE>
+__device__ __forceinline__ void finish_node(const KParams& kp, const BlockParams& B,
+                                            typename Store<ST>::T* f, bool full,
+                                            bool force_node, int32_t node, int64_t mnode,
+                                            typename Store<ST>::E* next, int x, int y,
+                                            int z, double* acc) {
+    using T = typename Store<ST>::T;
+    using E = typename Store<ST>::E;
+    constexpr int Q = D == 2 ? 9 : 19;
+    constexpr bool EXACT = ST == MPLBHIP_STORE_F64;
+    (void)full;
+    if constexpr (EXACT) {
+#pragma clang fp contract(off)
+        // UpdateMacroVars: density then each velocity component
+        double rho = 0;
+#pragma unroll
+        for (int q = 0; q < Q; q++) rho += f[q];
+        double uvw[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            double u = 0;
+#pragma unroll
+            for (int q = 0; q < Q; q++) u += kCS * double(cvel(D, q, d)) * f[q];
+            u /= rho;
+            if (kp.force_macros && force_node) u += (B.dt * B.g[d] / 2);
+            uvw[d] = u;
+        }
+        if constexpr (MODE == MODE_MATERIALIZE || MODE == MODE_STEP_CHECK) {
+            const double m[4] = {rho, uvw[0], uvw[1], uvw[2]};
+#pragma unroll
+            for (int v = 0; v < D + 1; v++) {
+                if (kp.want_residual) {
+                    const double old = kp.macro_copy[v][mnode];
+                    acc[v] = (m[v] - old) * (m[v] - old);
+                    acc[4 + v] = m[v] * m[v];
+                    kp.macro_copy[v][mnode] = m[v];
+                }
+                kp.macro[v][mnode] = m[v];
+            }
+            double mass = 0;
+#pragma unroll
+            for (int q = 0; q < Q; q++) mass += f[q];
+            acc[8] = mass;
+        }
+        if constexpr (MODE == MODE_MATERIALIZE) {
+            if (kp.fhost && !kp.fhost_in) {
+                const int64_t nn = (int64_t)B.n[0] * B.n[1] * B.n[2];
+                const int64_t h = B.fbase + ((int64_t)z * B.n[1] + y) * B.n[0] + x;
+#pragma unroll
+                for (int q = 0; q < Q; q++) kp.fhost[h + q * nn] = f[q];
+            }
+        } else {
+            // BGK collision with the optional first-order force source
+            const double tw = B.tau_omega, om1 = B.one_minus_omega;
+            const bool addf = force_node && kp.force_type == 1;
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const double fe = feq_ref<D, double>(q, rho, uvw[0], uvw[1], uvw[2]);
+                double fsq;
+                if (addf) {
+                    double cf = 0;
+#pragma unroll
+                    for (int d = 0; d < D; d++)
+                        cf += kCS * double(cvel(D, q, d)) * B.g[d];
+                    const double F = wgt(D, q) * rho * cf;
+                    fsq = fe + om1 * (f[q] - fe) + tw * F;
+                } else {
+                    fsq = fe + om1 * (f[q] - fe);
+                }
+                *at_u(next, (int64_t)q * B.sq, (uint32_t)node) = fsq;
+            }
+        }
+    } else {
+        // deviation form g = f - w, FP32 arithmetic
+        if (full) {
+#pragma unroll
+            for (int q = 0; q < Q; q++) f[q] -= T(wgt(D, q));
+        }
+        float drho = 0.f, j[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            drho += f[q];
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                if (cvel(D, q, d) > 0) j[d] += f[q];
+                if (cvel(D, q, d) < 0) j[d] -= f[q];
+            }
+        }
+        const float rho = 1.f + drho;
+        const float inv = 1.f / rho;
+        float ul[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            ul[d] = j[d] * inv;
+            if (kp.force_macros && force_node) ul[d] += B.fhalf_dtg[d];
+        }
+        const float usq = ul[0] * ul[0] + ul[1] * ul[1] + ul[2] * ul[2];
+        if constexpr (MODE == MODE_MATERIALIZE || MODE == MODE_STEP_CHECK) {
+            double dr = 0.0;
+#pragma unroll
+            for (int q = 0; q < Q; q++) dr += double(f[q]);
+            const double m[4] = {1.0 + dr, kCS * double(ul[0]), kCS * double(ul[1]),
+                                 kCS * double(ul[2])};
+#pragma unroll
+            for (int v = 0; v < D + 1; v++) {
+                if (kp.want_residual) {
+                    const double old = kp.macro_copy[v][mnode];
+                    acc[v] = (m[v] - old) * (m[v] - old);
+                    acc[4 + v] = m[v] * m[v];
+                    kp.macro_copy[v][mnode] = m[v];
+                }
+                kp.macro[v][mnode] = m[v];
+            }
+            acc[8] = 1.0 + dr;
+        }
+        if constexpr (MODE == MODE_MATERIALIZE) {
+            if (kp.fhost && !kp.fhost_in) {
+                const int64_t nn = (int64_t)B.n[0] * B.n[1] * B.n[2];
+                const int64_t h = B.fbase + ((int64_t)z * B.n[1] + y) * B.n[0] + x;
+#pragma unroll
+                for (int q = 0; q < Q; q++)
+                    kp.fhost[h + q * nn] = double(f[q]) + wgt(D, q);
+            }
+        } else {
+            const float om = B.fomega, om1 = B.fone_minus_omega;
+            const bool addf = force_node && kp.force_type == 1;
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                float cu = 0.f;
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    if (cvel(D, q, d) > 0) cu += ul[d];
+                    if (cvel(D, q, d) < 0) cu -= ul[d];
+                }
+                const float wq = float(wgt(D, q));
+                const float geq =
+                    wq * (drho + rho * (3.f * cu + 4.5f * cu * cu - 1.5f * usq));
+                float gn = om1 * f[q] + om * geq;
+                if (addf) {
+                    float cg = 0.f;
+#pragma unroll
+                    for (int d = 0; d < D; d++) cg += float(cvel(D, q, d)) * float(B.g[d]);
+                    gn += B.fforce * wq * rho * cg;
+                }
+                E* p = at_u(next, (int64_t)q * B.sq, (uint32_t)node);
+                if constexpr (ST == MPLBHIP_STORE_F16) *p = __float2half_rn(gn * kp.inv_scale[q]);
+                else __builtin_nontemporal_store(gn, p);
+            }
+        }
+    }
+}
+
+// ---- the boundary / whole-lattice kernel --------------------------------------
+// SCOPE_ALL: every node of every block, tiles of kTX x kTY nodes per z plane
+// (initialisation and MATERIALIZE).  SCOPE_BND: the boundary segments only
+// (the step's second kernel; the bulk kernel covers region 0).
+enum Scope : int { SCOPE_ALL = 0, SCOPE_BND = 1 };
+
+template <int D, int ST, int MODE, int SCOPE>
 __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
     using T = typename Store<ST>::T;
     using E = typename Store<ST>::E;
     constexpr int Q = D == 2 ? 9 : 19;
     constexpr bool EXACT = ST == MPLBHIP_STORE_F64;
-    const int wg = logical_wg(blockIdx.x, gridDim.x);
-    const int b = find_block(kp, wg);
+    int b, x, y, z, wg;
+    bool active;
+    if constexpr (SCOPE == SCOPE_ALL) {
+        wg = logical_wg(blockIdx.x, gridDim.x);
+        b = find_block<&BlockParams::tile_start>(kp, wg);
+        const BlockParams& B = kp.blocks[b];
+        int t = wg - B.tile_start;
+        const int tx = t % B.tiles_x;
+        t /= B.tiles_x;
+        const int ty = t % B.tiles_y;
+        z = t / B.tiles_y;
+        x = tx * kTX + threadIdx.x;
+        y = ty * kTY + threadIdx.y;
+        active = x < B.n[0] && y < B.n[1] && B.active;
+    } else {
+        wg = blockIdx.x;
+        b = find_block<&BlockParams::bnd_wg_start>(kp, wg);
+        const BlockParams& B = kp.blocks[b];
+        const int i = (wg - B.bnd_wg_start) * kBndWG + threadIdx.y * kTX + threadIdx.x;
+        int s = B.seg_start;
+        const int send = B.seg_start + B.seg_count;
+        while (s < send && i >= kp.segs[s].start + kp.segs[s].count) s++;
+        active = s < send && B.active;
+        x = y = z = 0;
+        if (active) {
+            const BndSeg& S = kp.segs[s];
+            int r = i - S.start;
+            x = S.org[0] + r % S.ext[0];
+            r /= S.ext[0];
+            y = S.org[1] + r % S.ext[1];
+            z = S.org[2] + r / S.ext[1];
+        }
+    }
     const BlockParams& B = kp.blocks[b];
-    int t = wg - B.tile_start;
-    const int tx = t % B.tiles_x;
-    t /= B.tiles_x;
-    const int ty = t % B.tiles_y;
-    const int z = t / B.tiles_y;
-    const int x = tx * kTX + threadIdx.x;
-    const int y = ty * kTY + threadIdx.y;
-    const bool active = x < B.n[0] && y < B.n[1] && B.active;
     // buffers of this block: steps read the latest fStage and write the
     // other; MATERIALIZE re-streams from the one before the latest
     const int cur = kp.cur_in[b];
@@ -329,10 +545,6 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
     const void* per_prev = kp.per_buf[src];
     void* fs_next = kp.fs_buf[MODE >= MODE_INIT_FEQ ? cur : 1 - cur];
     void* per_next = kp.per_buf[MODE >= MODE_INIT_FEQ ? cur : 1 - cur];
-    if constexpr (MODE == MODE_STEP || MODE == MODE_STEP_CHECK) {
-        if (t == 0 && threadIdx.x == 0 && threadIdx.y == 0)
-            kp.cur_out[b] = B.active ? 1 - cur : cur;
-    }
     double acc[kNAcc];
 #pragma unroll
     for (int a = 0; a < kNAcc; a++) acc[a] = 0.0;
@@ -343,7 +555,8 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
         const int r = region_code(x, B.n[0]) + 3 * region_code(y, B.n[1]) +
                       (D == 3 ? 9 * region_code(z, B.n[2]) : 0);
         const RegionProg* P = r != 0 ? &kp.progs[B.prog_base + r] : nullptr;
-        const bool solid = P && P->kind == RK_SKIP;
+        const uint32_t om = obst_mask(kp, B, mnode);
+        const bool solid = (P && P->kind == RK_SKIP) || (om & kObstSolid);
         const bool force_node = P ? P->force != 0 : true;
         T f[Q];  // EXACT: f; otherwise g = f - w (bulk) / f (boundary)
         bool full = false;  // f[] holds full values (boundary path)
@@ -351,14 +564,23 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
         E* next = static_cast<E*>(fs_next) + B.base;
 
         if (solid) {
-            // ImmersedSolid: neither streamed nor collided; fStage unchanged
+            // ImmersedSolid: neither streamed nor collided; fStage unchanged,
+            // f never set (zero), macros unchanged
             if constexpr (MODE != MODE_MATERIALIZE) {
 #pragma unroll
                 for (int q = 0; q < Q; q++) {
                     const int64_t i = (int64_t)q * B.sq + node;
                     next[i] = prev[i];
                 }
+            } else {
+                if (kp.fhost && !kp.fhost_in) {
+                    const int64_t nn = (int64_t)B.n[0] * B.n[1] * B.n[2];
+                    const int64_t h = B.fbase + ((int64_t)z * B.n[1] + y) * B.n[0] + x;
+#pragma unroll
+                    for (int q = 0; q < Q; q++) kp.fhost[h + q * nn] = 0.0;
+                }
             }
+            solid_residual<D, MODE>(kp, mnode, acc);
             goto reduce;
         }
         if constexpr (MODE == MODE_INIT_FEQ || MODE == MODE_INIT_F) {
@@ -405,13 +627,15 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
             // bulk: every population streams in (KerStream3D Fluid branch)
 #pragma unroll
             for (int q = 0; q < Q; q++) {
-                const E* p = at_u(prev, (int64_t)q * B.sq - pull_delta<D>(B, q),
-                                  (uint32_t)node);
+                const E* p = (om >> q & 1u)
+                                 ? at_u(prev, (int64_t)opp(D, q) * B.sq, (uint32_t)node)
+                                 : at_u(prev, (int64_t)q * B.sq - pull_delta<D>(B, q),
+                                        (uint32_t)node);
                 if constexpr (ST == MPLBHIP_STORE_F16) f[q] = __half2float(*p) * kp.scale[q];
                 else f[q] = *p;
             }
         } else {
-            gather_masked<D, ST>(kp, B, *P, x, y, z, f, fs_prev, per_prev);
+            gather_masked<D, ST>(kp, B, *P, x, y, z, f, fs_prev, per_prev, om);
             full = true;
             boundary_program<D, ST>(kp, B, *P, x, y, z, f, fs_prev, per_prev);
             if constexpr (MODE != MODE_MATERIALIZE) {
@@ -423,150 +647,96 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
                 }
             }
         }
-
-        if constexpr (EXACT) {
-#pragma clang fp contract(off)
-            // UpdateMacroVars: density then each velocity component
-            double rho = 0;
-#pragma unroll
-            for (int q = 0; q < Q; q++) rho += f[q];
-            double uvw[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-            for (int d = 0; d < D; d++) {
-                double u = 0;
-#pragma unroll
-                for (int q = 0; q < Q; q++) u += kCS * double(cvel(D, q, d)) * f[q];
-                u /= rho;
-                if (kp.force_macros && force_node) u += (B.dt * B.g[d] / 2);
-                uvw[d] = u;
-            }
-            if constexpr (MODE == MODE_MATERIALIZE || MODE == MODE_STEP_CHECK) {
-                const double m[4] = {rho, uvw[0], uvw[1], uvw[2]};
-#pragma unroll
-                for (int v = 0; v < D + 1; v++) {
-                    if (kp.want_residual) {
-                        const double old = kp.macro_copy[v][mnode];
-                        acc[v] = (m[v] - old) * (m[v] - old);
-                        acc[4 + v] = m[v] * m[v];
-                        kp.macro_copy[v][mnode] = m[v];
-                    }
-                    kp.macro[v][mnode] = m[v];
-                }
-                double mass = 0;
-#pragma unroll
-                for (int q = 0; q < Q; q++) mass += f[q];
-                acc[8] = mass;
-            }
-            if constexpr (MODE == MODE_MATERIALIZE) {
-                if (kp.fhost && !kp.fhost_in) {
-                    const int64_t nn = (int64_t)B.n[0] * B.n[1] * B.n[2];
-                    const int64_t h = B.fbase + ((int64_t)z * B.n[1] + y) * B.n[0] + x;
-#pragma unroll
-                    for (int q = 0; q < Q; q++) kp.fhost[h + q * nn] = f[q];
-                }
-            } else {
-                // BGK collision with the optional first-order force source
-                const double tw = B.tau_omega, om1 = B.one_minus_omega;
-                const bool addf = force_node && kp.force_type == 1;
-#pragma unroll
-                for (int q = 0; q < Q; q++) {
-                    const double fe = feq_ref<D, double>(q, rho, uvw[0], uvw[1], uvw[2]);
-                    double fsq;
-                    if (addf) {
-                        double cf = 0;
-#pragma unroll
-                        for (int d = 0; d < D; d++)
-                            cf += kCS * double(cvel(D, q, d)) * B.g[d];
-                        const double F = wgt(D, q) * rho * cf;
-                        fsq = fe + om1 * (f[q] - fe) + tw * F;
-                    } else {
-                        fsq = fe + om1 * (f[q] - fe);
-                    }
-                    *at_u(next, (int64_t)q * B.sq, (uint32_t)node) = fsq;
-                }
-            }
-        } else {
-            // deviation form g = f - w, FP32 arithmetic
-            if (full) {
-#pragma unroll
-                for (int q = 0; q < Q; q++) f[q] -= T(wgt(D, q));
-            }
-            float drho = 0.f, j[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-            for (int q = 0; q < Q; q++) {
-                drho += f[q];
-#pragma unroll
-                for (int d = 0; d < D; d++) {
-                    if (cvel(D, q, d) > 0) j[d] += f[q];
-                    if (cvel(D, q, d) < 0) j[d] -= f[q];
-                }
-            }
-            const float rho = 1.f + drho;
-            const float inv = 1.f / rho;
-            float ul[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-            for (int d = 0; d < D; d++) {
-                ul[d] = j[d] * inv;
-                if (kp.force_macros && force_node) ul[d] += B.fhalf_dtg[d];
-            }
-            const float usq = ul[0] * ul[0] + ul[1] * ul[1] + ul[2] * ul[2];
-            if constexpr (MODE == MODE_MATERIALIZE || MODE == MODE_STEP_CHECK) {
-                double dr = 0.0;
-#pragma unroll
-                for (int q = 0; q < Q; q++) dr += double(f[q]);
-                const double m[4] = {1.0 + dr, kCS * double(ul[0]), kCS * double(ul[1]),
-                                     kCS * double(ul[2])};
-#pragma unroll
-                for (int v = 0; v < D + 1; v++) {
-                    if (kp.want_residual) {
-                        const double old = kp.macro_copy[v][mnode];
-                        acc[v] = (m[v] - old) * (m[v] - old);
-                        acc[4 + v] = m[v] * m[v];
-                        kp.macro_copy[v][mnode] = m[v];
-                    }
-                    kp.macro[v][mnode] = m[v];
-                }
-                acc[8] = 1.0 + dr;
-            }
-            if constexpr (MODE == MODE_MATERIALIZE) {
-                if (kp.fhost && !kp.fhost_in) {
-                    const int64_t nn = (int64_t)B.n[0] * B.n[1] * B.n[2];
-                    const int64_t h = B.fbase + ((int64_t)z * B.n[1] + y) * B.n[0] + x;
-#pragma unroll
-                    for (int q = 0; q < Q; q++)
-                        kp.fhost[h + q * nn] = double(f[q]) + wgt(D, q);
-                }
-            } else {
-                const float om = B.fomega, om1 = B.fone_minus_omega;
-                const bool addf = force_node && kp.force_type == 1;
-#pragma unroll
-                for (int q = 0; q < Q; q++) {
-                    float cu = 0.f;
-#pragma unroll
-                    for (int d = 0; d < D; d++) {
-                        if (cvel(D, q, d) > 0) cu += ul[d];
-                        if (cvel(D, q, d) < 0) cu -= ul[d];
-                    }
-                    const float wq = float(wgt(D, q));
-                    const float geq =
-                        wq * (drho + rho * (3.f * cu + 4.5f * cu * cu - 1.5f * usq));
-                    float gn = om1 * f[q] + om * geq;
-                    if (addf) {
-                        float cg = 0.f;
-#pragma unroll
-                        for (int d = 0; d < D; d++) cg += float(cvel(D, q, d)) * float(B.g[d]);
-                        gn += B.fforce * wq * rho * cg;
-                    }
-                    E* p = at_u(next, (int64_t)q * B.sq, (uint32_t)node);
-                    if constexpr (ST == MPLBHIP_STORE_F16) *p = __float2half_rn(gn * kp.inv_scale[q]);
-                    else __builtin_nontemporal_store(gn, p);
-                }
-            }
-        }
+        finish_node<D, ST, MODE>(kp, B, f, full, force_node, node, mnode, next, x, y, z, acc);
     }
 reduce:
-    if constexpr (MODE == MODE_STEP_CHECK || MODE == MODE_MATERIALIZE)
-        reduce_store(acc, kp.partials + (int64_t)wg * kNAcc);
+    if constexpr (MODE == MODE_STEP_CHECK || MODE == MODE_MATERIALIZE) {
+        const int slot = SCOPE == SCOPE_BND ? kp.nbulk + wg : wg;
+        reduce_store(acc, kp.partials + (int64_t)slot * kNAcc);
+    }
+}
+
+// ---- the bulk kernel ------------------------------------------------------------
+// Region 0 of every block: Fluid nodes whose Q populations all stream in
+// (KerStream3D Fluid branch), no boundary program, so the register budget
+// is the bulk's alone.  Tiles of kTX x kTY nodes, z planes 1 .. n-2 (3D).
+// The block's first bulk workgroup also records the block's parity after
+// the step.
+template <int D, int ST, int MODE, bool OBST>
+__global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
+    using T = typename Store<ST>::T;
+    using E = typename Store<ST>::E;
+    constexpr int Q = D == 2 ? 9 : 19;
+    const int wg = logical_wg(blockIdx.x, gridDim.x);
+    const int b = find_block<&BlockParams::bulk_tile_start>(kp, wg);
+    const BlockParams& B = kp.blocks[b];
+    int t = wg - B.bulk_tile_start;
+    const bool first = t == 0;
+    const int tx = t % B.tiles_x;
+    t /= B.tiles_x;
+    const int ty = t % B.tiles_y;
+    const int z = D == 3 ? 1 + t / B.tiles_y : 0;
+    const int x = tx * kTX + threadIdx.x;
+    const int y = ty * kTY + threadIdx.y;
+    const int cur = kp.cur_in[b];
+    if (first && threadIdx.x == 0 && threadIdx.y == 0) kp.cur_out[b] = B.active ? 1 - cur : cur;
+    double acc[kNAcc];
+#pragma unroll
+    for (int a = 0; a < kNAcc; a++) acc[a] = 0.0;
+    // lanes: interior nodes, and the x-face nodes of rows y = 1 .. n-2 when
+    // their region program is simple (masked pull + EQM): their loads share
+    // the wave's cache lines, which a separate kernel would fetch again
+    const bool yin = y >= 1 && y < B.n[1] - 1;
+    const bool interior = x >= 1 && x < B.n[0] - 1 && yin;
+    const RegionProg* P = nullptr;
+    if (!interior && yin && (x == 0 || x == B.n[0] - 1)) {
+        P = &kp.progs[B.prog_base + (x == 0 ? 1 : 2)];
+        if (!P->simple) P = nullptr;
+    }
+    const bool active = B.active && (interior || P);
+    if (active) {
+        const int32_t node = z * B.sz + y * B.sx + x;
+        const int64_t mnode = B.mbase + ((int64_t)z * B.n[1] + y) * B.n[0] + x;
+        const E* prev = static_cast<const E*>(kp.fs_buf[cur]) + B.base;
+        E* next = static_cast<E*>(kp.fs_buf[1 - cur]) + B.base;
+        uint32_t om = 0;
+        if constexpr (OBST) om = obst_mask(kp, B, mnode);
+        const uint32_t pull = P ? P->pull : ~0u;
+        if (OBST && (om & kObstSolid)) {
+            // obstacle node: nothing streams, collides or is read
+            solid_residual<D, MODE>(kp, mnode, acc);
+        } else {
+            T f[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const bool bounce = OBST && (om >> q & 1u);
+                const E* p = bounce ? at_u(prev, (int64_t)opp(D, q) * B.sq, (uint32_t)node)
+                                    : at_u(prev, (int64_t)q * B.sq - pull_delta<D>(B, q),
+                                           (uint32_t)node);
+                T v = T(0);
+                if (pull >> q & 1u) {
+                    if constexpr (ST == MPLBHIP_STORE_F16)
+                        v = __half2float(*p) * kp.scale[bounce ? opp(D, q) : q];
+                    else
+                        v = *p;
+                }
+                f[q] = v;
+            }
+            if (P) {
+                // full values (unpulled populations start at zero; the EQM
+                // applications set every one of them), then the boundary ops
+                if constexpr (ST != MPLBHIP_STORE_F64) {
+#pragma unroll
+                    for (int q = 0; q < Q; q++)
+                        if (pull >> q & 1u) f[q] += T(wgt(D, q));
+                }
+                for (int o = 0; o < P->nops; o++) apply_eqm<D, T>(f, P->op[o]);
+            }
+            finish_node<D, ST, MODE>(kp, B, f, P != nullptr, P ? P->force != 0 : true, node,
+                                     mnode, next, x, y, z, acc);
+        }
+    }
+    if constexpr (MODE == MODE_STEP_CHECK) reduce_store(acc, kp.partials + (int64_t)wg * kNAcc);
 }
 
 // fStage halo slab copy of one block connection (Field::TransferHalos ->
@@ -597,17 +767,30 @@ __global__ void k_halo(HaloParams hp) {
 }
 
 // deterministic sums of per-workgroup partials: out[b*kNAcc + a] per block,
-// out[nblocks*kNAcc + a] over all blocks (block order)
+// out[nblocks*kNAcc + a] over all blocks (block order).  After a split step
+// a block's bulk workgroups come first, then its boundary workgroups; after
+// a whole-lattice launch, its tiles.
 __global__ void k_reduce_partials(const double* partials, const BlockParams* blocks,
-                                  int nblocks, int ntiles, double* out) {
+                                  int nblocks, int ntiles, int nbulk, int nbnd, int split,
+                                  double* out) {
     const int a = threadIdx.x;
     if (a >= kNAcc) return;
     double total = 0.0;
     for (int b = 0; b < nblocks; b++) {
-        const int t0 = blocks[b].tile_start;
-        const int t1 = b + 1 < nblocks ? blocks[b + 1].tile_start : ntiles;
+        const bool last = b + 1 == nblocks;
         double s = 0.0;
-        for (int t = t0; t < t1; t++) s += partials[(int64_t)t * kNAcc + a];
+        if (split) {
+            const int t0 = blocks[b].bulk_tile_start;
+            const int t1 = last ? nbulk : blocks[b + 1].bulk_tile_start;
+            for (int t = t0; t < t1; t++) s += partials[(int64_t)t * kNAcc + a];
+            const int w0 = blocks[b].bnd_wg_start;
+            const int w1 = last ? nbnd : blocks[b + 1].bnd_wg_start;
+            for (int w = w0; w < w1; w++) s += partials[(int64_t)(nbulk + w) * kNAcc + a];
+        } else {
+            const int t0 = blocks[b].tile_start;
+            const int t1 = last ? ntiles : blocks[b + 1].tile_start;
+            for (int t = t0; t < t1; t++) s += partials[(int64_t)t * kNAcc + a];
+        }
         out[b * kNAcc + a] = s;
         total += s;
     }
@@ -621,24 +804,39 @@ __global__ void k_fill_bytes(uint32_t* p, int64_t n, uint32_t v) {
 }
 
 // ---- host-side launch helpers ----------------------------------------------------
+// part: 0 = the whole step (bulk then boundary), 1 = bulk only, 2 = boundary
+// only (timed runs bracket the two separately); other modes ignore it
+template <int D, int ST, int MODE>
+static void launch_step(const KParams& kp, int part, hipStream_t s) {
+    dim3 blk(kTX, kTY, 1);
+    if (part != 2 && kp.nbulk > 0) {
+        if (kp.obst)
+            hipLaunchKernelGGL((k_bulk<D, ST, MODE, true>), dim3(kp.nbulk), blk, 0, s, kp);
+        else
+            hipLaunchKernelGGL((k_bulk<D, ST, MODE, false>), dim3(kp.nbulk), blk, 0, s, kp);
+    }
+    if (part != 1 && kp.nbnd > 0)
+        hipLaunchKernelGGL((k_lattice<D, ST, MODE, SCOPE_BND>), dim3(kp.nbnd), blk, 0, s, kp);
+}
+
 template <int D, int ST>
-static hipError_t launch_mode(int mode, const KParams& kp, hipStream_t s) {
+static hipError_t launch_mode(int mode, int part, const KParams& kp, hipStream_t s) {
     dim3 blk(kTX, kTY, 1), grd(kp.ntiles, 1, 1);
     switch (mode) {
         case MODE_STEP:
-            hipLaunchKernelGGL((k_lattice<D, ST, MODE_STEP>), grd, blk, 0, s, kp);
+            launch_step<D, ST, MODE_STEP>(kp, part, s);
             break;
         case MODE_STEP_CHECK:
-            hipLaunchKernelGGL((k_lattice<D, ST, MODE_STEP_CHECK>), grd, blk, 0, s, kp);
+            launch_step<D, ST, MODE_STEP_CHECK>(kp, part, s);
             break;
         case MODE_MATERIALIZE:
-            hipLaunchKernelGGL((k_lattice<D, ST, MODE_MATERIALIZE>), grd, blk, 0, s, kp);
+            hipLaunchKernelGGL((k_lattice<D, ST, MODE_MATERIALIZE, SCOPE_ALL>), grd, blk, 0, s, kp);
             break;
         case MODE_INIT_FEQ:
-            hipLaunchKernelGGL((k_lattice<D, ST, MODE_INIT_FEQ>), grd, blk, 0, s, kp);
+            hipLaunchKernelGGL((k_lattice<D, ST, MODE_INIT_FEQ, SCOPE_ALL>), grd, blk, 0, s, kp);
             break;
         case MODE_INIT_F:
-            hipLaunchKernelGGL((k_lattice<D, ST, MODE_INIT_F>), grd, blk, 0, s, kp);
+            hipLaunchKernelGGL((k_lattice<D, ST, MODE_INIT_F, SCOPE_ALL>), grd, blk, 0, s, kp);
             break;
         default:
             return hipErrorInvalidValue;
@@ -646,16 +844,16 @@ static hipError_t launch_mode(int mode, const KParams& kp, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_lattice(int dim, int storage, int mode, const KParams& kp,
-                          hipStream_t s) {
+hipError_t launch_lattice(int dim, int storage, int mode, const KParams& kp, hipStream_t s,
+                          int part) {
     if (dim == 2) {
-        if (storage == MPLBHIP_STORE_F64) return launch_mode<2, MPLBHIP_STORE_F64>(mode, kp, s);
-        if (storage == MPLBHIP_STORE_F32) return launch_mode<2, MPLBHIP_STORE_F32>(mode, kp, s);
-        return launch_mode<2, MPLBHIP_STORE_F16>(mode, kp, s);
+        if (storage == MPLBHIP_STORE_F64) return launch_mode<2, MPLBHIP_STORE_F64>(mode, part, kp, s);
+        if (storage == MPLBHIP_STORE_F32) return launch_mode<2, MPLBHIP_STORE_F32>(mode, part, kp, s);
+        return launch_mode<2, MPLBHIP_STORE_F16>(mode, part, kp, s);
     }
-    if (storage == MPLBHIP_STORE_F64) return launch_mode<3, MPLBHIP_STORE_F64>(mode, kp, s);
-    if (storage == MPLBHIP_STORE_F32) return launch_mode<3, MPLBHIP_STORE_F32>(mode, kp, s);
-    return launch_mode<3, MPLBHIP_STORE_F16>(mode, kp, s);
+    if (storage == MPLBHIP_STORE_F64) return launch_mode<3, MPLBHIP_STORE_F64>(mode, part, kp, s);
+    if (storage == MPLBHIP_STORE_F32) return launch_mode<3, MPLBHIP_STORE_F32>(mode, part, kp, s);
+    return launch_mode<3, MPLBHIP_STORE_F16>(mode, part, kp, s);
 }
 
 hipError_t launch_halo(const HaloParams& hp, hipStream_t s) {
@@ -673,9 +871,10 @@ hipError_t launch_halo(const HaloParams& hp, hipStream_t s) {
 }
 
 hipError_t launch_reduce(const double* partials, const BlockParams* blocks, int nblocks,
-                         int ntiles, double* out, hipStream_t s) {
+                         int ntiles, int nbulk, int nbnd, int split, double* out,
+                         hipStream_t s) {
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partials, blocks,
-                       nblocks, ntiles, out);
+                       nblocks, ntiles, nbulk, nbnd, split, out);
     return hipGetLastError();
 }
 

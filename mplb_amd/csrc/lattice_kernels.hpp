@@ -11,14 +11,15 @@ constexpr int kTX = 64;   // one wavefront along x
 constexpr int kTY = 4;    // four wavefronts per workgroup along y
 constexpr int kNAcc = 9;  // residual: 4 squared differences, 4 squares, mass
 
-// One block as laid out in HBM: every population is a 3-D array with a one
-// node halo on each axis (no z halo in 2D), rows padded so that x = 0 is
-// aligned; populations follow each other at pitch sq.
+// One block as laid out in HBM: a one node halo on each axis (no z halo in
+// 2D), rows padded so that x = 0 is 64-byte aligned; the Q populations of a
+// z plane follow each other (pitch sq = one padded plane), and z planes
+// follow at pitch sz = Q * sq.  Element (q,x,y,z): base + z*sz + q*sq + y*sx + x.
 struct BlockParams {
     int32_t n[3];
     int32_t sx;          // row pitch (elements)
-    int32_t sz;          // plane pitch
-    int32_t sq;          // population pitch
+    int32_t sz;          // z-slab pitch (all populations of one plane)
+    int32_t sq;          // population pitch within a slab
     int64_t base;        // element offset of node (0,0,0), population 0
     int32_t tiles_x, tiles_y;
     int32_t tile_start;  // first logical workgroup of this block
@@ -31,8 +32,30 @@ struct BlockParams {
     double tau, omega, one_minus_omega, tau_omega;
     float fomega, fone_minus_omega, fforce, fhalf_dtg[3];
     int32_t active;      // 0 freezes the block (batched instances)
-    int32_t pad;
+    // split step: the bulk kernel covers the interior region (z planes 1 ..
+    // n-2 in 3D) in tiles_x * tiles_y tiles per plane; the boundary kernel
+    // walks the block's boundary segments, 256 nodes per workgroup
+    int32_t bulk_tile_start;  // first bulk workgroup of this block
+    int32_t bnd_wg_start;     // first boundary workgroup of this block
+    int32_t seg_start, seg_count;  // boundary segments of this block
+    int32_t has_obst;         // the block has obstacle link masks
 };
+
+// Obstacle link mask of an interior node (extension: in-domain obstacles
+// with half-way bounce-back): bit q set = the pull source x - c_q is an
+// obstacle node, so f_q(x) = fs_opp(q)(x); kObstSolid = the node itself is
+// an obstacle node (ImmersedSolid).
+constexpr uint32_t kObstSolid = 1u << 31;
+
+// A box of boundary nodes (one region of one block): nodes start ..
+// start+count-1 of the block's boundary list, x fastest.
+struct BndSeg {
+    int32_t org[3];
+    int32_t ext[3];
+    int32_t start;
+    int32_t count;
+};
+constexpr int kBndWG = kTX * kTY;  // boundary nodes per workgroup
 
 // Ping-pong parity is kept per block in device memory: a step reads
 // cur_in[b] (the buffer holding block b's latest fStage) and the block's
@@ -68,6 +91,10 @@ struct KParams {
     int32_t force_macros;   // *_Force velocity kernels
     int32_t fhost_in;       // MATERIALIZE: take f from fhost instead of streaming
     int32_t want_residual;  // accumulate the residual against MacroVarsCopy
+    const uint32_t* obst;   // obstacle link masks per interior node, or null
+    const BndSeg* segs;     // boundary segments, blocks in order
+    int32_t nbulk;          // bulk workgroups (partials 0 .. nbulk-1)
+    int32_t nbnd;           // boundary workgroups (partials nbulk ..)
 };
 
 struct HaloParams {

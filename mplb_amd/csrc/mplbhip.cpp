@@ -18,10 +18,11 @@
 
 namespace mplbhip {
 hipError_t launch_lattice(int dim, int storage, int mode, const KParams& kp,
-                          hipStream_t s);
+                          hipStream_t s, int part = 0);
 hipError_t launch_halo(const HaloParams& hp, hipStream_t s);
 hipError_t launch_reduce(const double* partials, const BlockParams* blocks,
-                         int nblocks, int ntiles, double* out, hipStream_t s);
+                         int nblocks, int ntiles, int nbulk, int nbnd, int split,
+                         double* out, hipStream_t s);
 hipError_t launch_fill(void* p, int64_t bytes, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
 }  // namespace mplbhip
@@ -65,13 +66,17 @@ struct mplbhip_ctx {
     std::vector<BlockParams> bp;
     int64_t total_elems = 0;
     int64_t interior_nodes = 0;
-    int ntiles = 0;
+    int ntiles = 0;            // whole-lattice tiles
+    int nbulk = 0, nbnd = 0;   // bulk tiles and boundary workgroups of a step
+    std::vector<BndSeg> segs;
+    std::vector<std::vector<uint8_t>> solid;  // obstacle nodes per block (empty: none)
+    std::vector<uint32_t> obst;               // link masks per interior node
     size_t es = 8;
 
     // device memory
     hipStream_t stream = nullptr;
     DevBuf fs[2], per[2], macro[4], macro_copy[4], progs, blocks, partials,
-        red, fhost, parity[2];
+        red, fhost, parity[2], dsegs, dobst;
     std::vector<HaloLink> halos;
     std::vector<double> host_init_f;  // f given by mplbhip_set_f, per block
     std::vector<float> scale, inv_scale;
@@ -80,6 +85,7 @@ struct mplbhip_ctx {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int graph_steps = 0;
     double last_ratio[4] = {0, 0, 0, 0};
+    double last_bulk_ms = 0.0, last_bnd_ms = 0.0;  // last timed run
     double last_max = -1.0;
 };
 
@@ -175,6 +181,10 @@ static KParams base_params(mplbhip_ctx* c) {
     kp.blocks = static_cast<const BlockParams*>(c->blocks.p);
     kp.nblocks = (int)c->bp.size();
     kp.ntiles = c->ntiles;
+    kp.nbulk = c->nbulk;
+    kp.nbnd = c->nbnd;
+    kp.segs = static_cast<const BndSeg*>(c->dsegs.p);
+    kp.obst = c->obst.empty() ? nullptr : static_cast<const uint32_t*>(c->dobst.p);
     kp.progs = static_cast<const RegionProg*>(c->progs.p);
     for (int v = 0; v < 4; v++) {
         kp.macro[v] = static_cast<double*>(c->macro[v].p);
@@ -285,7 +295,7 @@ static int materialize(mplbhip_ctx* c, bool want_f, bool residual,
     if (acc_out) {
         HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
                                 static_cast<const BlockParams*>(c->blocks.p),
-                                (int)c->bp.size(), c->ntiles,
+                                (int)c->bp.size(), c->ntiles, c->nbulk, c->nbnd, 0,
                                 static_cast<double*>(c->red.p), c->stream));
         acc_out->resize((c->bp.size() + 1) * kNAcc);
         HIPCHK(c, hipMemcpyAsync(acc_out->data(), c->red.p,
@@ -363,6 +373,8 @@ void mplbhip_destroy(mplbhip_ctx* c) {
         dfree(c->progs);
         dfree(c->blocks);
         dfree(c->partials);
+        dfree(c->dsegs);
+        dfree(c->dobst);
         dfree(c->red);
         dfree(c->fhost);
         dfree(c->parity[0]);
@@ -442,6 +454,19 @@ int mplbhip_define_collision(mplbhip_ctx* c, double tau_ref, int force_type,
         drop_graphs(c);
         return upload_blocks(c);
     }
+    return 0;
+}
+
+int mplbhip_define_obstacles(mplbhip_ctx* c, int block, const unsigned char* solid) {
+    if (!c || !solid) return MPLBHIP_ERR_ARG;
+    if (c->prepared) return fail(c, MPLBHIP_ERR_ARG, "obstacles are defined before prepare");
+    if (block < 0 || block >= (int)c->plan.blocks.size())
+        return fail(c, MPLBHIP_ERR_ARG, "obstacles on undefined block %d", block);
+    const int* n = c->plan.blocks[block].n;
+    const size_t cnt = (size_t)n[0] * n[1] * n[2];
+    if (c->solid.size() < c->plan.blocks.size()) c->solid.resize(c->plan.blocks.size());
+    c->solid[block].assign(solid, solid + cnt);
+    for (uint8_t& v : c->solid[block]) v = v ? 1 : 0;
     return 0;
 }
 
@@ -606,6 +631,92 @@ static int setup_halos(mplbhip_ctx* c) {
     return 0;
 }
 
+static bool full_pull_type(int vt) {
+    return vt == VT_Fluid || vt == VT_MDPeriodic || vt == VT_VirtualBoundary;
+}
+
+// Obstacle link masks (bit q: the pull source x - c_q is an obstacle node;
+// kObstSolid: x is one).  Sources in a halo are resolved through the block
+// connections, as the halo exchange would copy them.  A node pulling from
+// an obstacle must pull every population; obstacles sit on Fluid or
+// MDPeriodic nodes only.
+static int build_obstacles(mplbhip_ctx* c) {
+    c->obst.clear();
+    const int nb = (int)c->plan.blocks.size();
+    bool any = false;
+    for (int b = 0; b < (int)c->solid.size(); b++) any = any || !c->solid[b].empty();
+    if (!any) return 0;
+    c->solid.resize(nb);
+    const int hz = c->dim == 3 ? 1 : 0;
+    // halo-extended obstacle flags per block
+    std::vector<std::vector<uint8_t>> ext(nb);
+    auto eidx = [&](int b, int i, int j, int k) {
+        const int* n = c->plan.blocks[b].n;
+        return ((size_t)(k + hz) * (n[1] + 2) + (j + 1)) * (n[0] + 2) + (i + 1);
+    };
+    for (int b = 0; b < nb; b++) {
+        const int* n = c->plan.blocks[b].n;
+        ext[b].assign((size_t)(n[0] + 2) * (n[1] + 2) * (n[2] + 2 * hz), 0);
+        if (c->solid[b].empty()) continue;
+        size_t p = 0;
+        for (int k = 0; k < n[2]; k++)
+            for (int j = 0; j < n[1]; j++)
+                for (int i = 0; i < n[0]; i++, p++) ext[b][eidx(b, i, j, k)] = c->solid[b][p];
+    }
+    for (const LinkDesc& l : c->plan.links) {
+        if (c->solid[l.from].empty()) continue;
+        LinkMap m;
+        if (!link_map(l, m)) continue;
+        const int* na = c->plan.blocks[l.from].n;
+        const int* nt = c->plan.blocks[l.to].n;
+        int lo[3] = {0, 0, 0}, hi[3] = {na[0], na[1], na[2]};
+        lo[m.af] = m.sf > 0 ? na[m.af] - 1 : 0;
+        hi[m.af] = lo[m.af] + 1;
+        for (int k = lo[2]; k < hi[2]; k++)
+            for (int j = lo[1]; j < hi[1]; j++)
+                for (int i = lo[0]; i < hi[0]; i++) {
+                    const int p[3] = {i, j, k};
+                    int d[3];
+                    link_target(m, nt, p, d);
+                    if (c->dim == 2) d[2] = 0;
+                    ext[l.to][eidx(l.to, d[0], d[1], d[2])] =
+                        c->solid[l.from][((size_t)k * na[1] + j) * na[0] + i];
+                }
+    }
+    c->obst.assign((size_t)c->interior_nodes, 0u);
+    const int (*cv)[3] = c->dim == 3 ? kD3Q19 : kD2Q9;
+    for (int b = 0; b < nb; b++) {
+        const int* n = c->plan.blocks[b].n;
+        bool has = false;
+        size_t p = 0;
+        for (int k = 0; k < n[2]; k++)
+            for (int j = 0; j < n[1]; j++)
+                for (int i = 0; i < n[0]; i++, p++) {
+                    const int vt = c->plan.node_type(b, i, j, k);
+                    uint32_t mask = 0;
+                    if (ext[b][eidx(b, i, j, k)]) {
+                        if (vt != VT_Fluid && vt != VT_MDPeriodic)
+                            return fail(c, MPLBHIP_ERR_ARG,
+                                        "obstacle on a node that is not Fluid or MDPeriodic "
+                                        "(block %d node %d,%d,%d)", b, i, j, k);
+                        mask = kObstSolid;
+                    } else {
+                        for (int q = 0; q < c->Q; q++)
+                            if (ext[b][eidx(b, i - cv[q][0], j - cv[q][1], k - cv[q][2])])
+                                mask |= 1u << q;
+                        if (mask && !full_pull_type(vt))
+                            return fail(c, MPLBHIP_ERR_ARG,
+                                        "obstacle next to a boundary node (block %d node "
+                                        "%d,%d,%d)", b, i, j, k);
+                    }
+                    has = has || mask != 0;
+                    c->obst[(size_t)c->bp[b].mbase + p] = mask;
+                }
+        c->bp[b].has_obst = has ? 1 : 0;
+    }
+    return 0;
+}
+
 int mplbhip_prepare(mplbhip_ctx* c) {
     if (!c) return MPLBHIP_ERR_ARG;
     if (c->prepared) return fail(c, MPLBHIP_ERR_ARG, "case already prepared");
@@ -617,28 +728,65 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     const int align = (int)(64 / c->es);
     c->bp.assign(nb, BlockParams{});
     int64_t off = 0, moff = 0;
-    int tiles = 0;
+    int tiles = 0, bulk = 0, bnd = 0;
+    c->segs.clear();
     for (int b = 0; b < nb; b++) {
         const int* n = c->plan.blocks[b].n;
         BlockParams& B = c->bp[b];
         std::memset(&B, 0, sizeof B);
         for (int a = 0; a < 3; a++) B.n[a] = n[a];
+        // z-slab interleaved populations: element (q, x, y, z) at
+        // base + z*sz + q*sq + y*sx + x with sq = one padded xy plane and
+        // sz = Q planes, so the Q streams of a z plane share one
+        // contiguous slab (fewer DRAM pages and TLB entries per wave than
+        // Q whole-lattice arrays)
         const int ox = align;
         const int64_t sx = ((int64_t)ox + n[0] + 1 + align - 1) / align * align;
-        const int64_t sz = sx * (n[1] + 2);
-        int64_t sq = sz * (n[2] + 2 * hz);
-        sq = (sq + 63) / 64 * 64;
-        if ((int64_t)c->Q * sq >= (1LL << 31))
-            return fail(c, MPLBHIP_ERR_UNSUPPORTED, "block %d too large", b);
+        const int64_t sq = sx * (n[1] + 2);
+        const int64_t sz = sq * c->Q;
+        const int64_t elems = sz * (n[2] + 2 * hz);
+        if (elems >= (1LL << 31) || elems * (int64_t)c->es >= (1LL << 32))
+            return fail(c, MPLBHIP_ERR_UNSUPPORTED,
+                        "block %d too large for 32-bit node offsets", b);
         B.sx = (int32_t)sx;
         B.sz = (int32_t)sz;
         B.sq = (int32_t)sq;
         B.base = off + hz * sz + sx + ox;
-        off += ((int64_t)c->Q * sq + 255) / 256 * 256;
+        off += (elems + 255) / 256 * 256;
         B.tiles_x = (n[0] + kTX - 1) / kTX;
         B.tiles_y = (n[1] + kTY - 1) / kTY;
         B.tile_start = tiles;
         tiles += B.tiles_x * B.tiles_y * n[2];
+        // the step's split: bulk tiles over z = 1 .. n-2 (3D), and the
+        // boundary regions as segments, 256 nodes per workgroup
+        B.bulk_tile_start = bulk;
+        bulk += B.tiles_x * B.tiles_y * (c->dim == 3 ? n[2] - 2 : 1);
+        B.seg_start = (int)c->segs.size();
+        int count = 0;
+        for (int r = 1; r < c->plan.nregions; r++) {
+            const int code[3] = {r % 3, (r / 3) % 3, r / 9};
+            // x-face interiors with simple programs run inside the bulk
+            // kernel, whose waves already load their rows
+            if (r < 3 && c->plan.progs[(size_t)b * c->plan.nregions + r].simple) continue;
+            BndSeg S;
+            for (int a = 0; a < 3; a++) {
+                if (a == 2 && c->dim == 2) {
+                    S.org[a] = 0;
+                    S.ext[a] = 1;
+                } else {
+                    S.org[a] = code[a] == 0 ? 1 : code[a] == 1 ? 0 : n[a] - 1;
+                    S.ext[a] = code[a] == 0 ? n[a] - 2 : 1;
+                }
+            }
+            S.start = count;
+            S.count = S.ext[0] * S.ext[1] * S.ext[2];
+            if (S.count <= 0) continue;
+            count += S.count;
+            c->segs.push_back(S);
+        }
+        B.seg_count = (int)c->segs.size() - B.seg_start;
+        B.bnd_wg_start = bnd;
+        bnd += (count + kBndWG - 1) / kBndWG;
         B.prog_base = b * c->plan.nregions;
         B.mbase = moff;
         B.fbase = moff * c->Q;
@@ -648,7 +796,10 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     c->total_elems = off;
     c->interior_nodes = moff;
     c->ntiles = tiles;
+    c->nbulk = bulk;
+    c->nbnd = bnd;
     if (int rc = setup_halos(c)) return rc;
+    if (int rc = build_obstacles(c)) return rc;
     c->prepared = true;
     if (c->device < 0) return 0;
     HIPCHK(c, hipSetDevice(c->device));
@@ -673,7 +824,18 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     HIPCHK(c, hipMemcpyAsync(c->progs.p, c->plan.progs.data(), c->progs.bytes,
                              hipMemcpyHostToDevice, c->stream));
     if (int rc = dalloc(c, c->blocks, nb * sizeof(BlockParams))) return rc;
-    if (int rc = dalloc(c, c->partials, (size_t)tiles * kNAcc * sizeof(double))) return rc;
+    if (!c->obst.empty()) {
+        if (int rc = dalloc(c, c->dobst, c->obst.size() * sizeof(uint32_t))) return rc;
+        HIPCHK(c, hipMemcpyAsync(c->dobst.p, c->obst.data(), c->dobst.bytes,
+                                 hipMemcpyHostToDevice, c->stream));
+    }
+    const size_t nslots = (size_t)std::max(tiles, bulk + bnd);
+    if (int rc = dalloc(c, c->partials, nslots * kNAcc * sizeof(double))) return rc;
+    if (int rc = dalloc(c, c->dsegs, std::max<size_t>(1, c->segs.size()) * sizeof(BndSeg)))
+        return rc;
+    if (!c->segs.empty())
+        HIPCHK(c, hipMemcpyAsync(c->dsegs.p, c->segs.data(), c->segs.size() * sizeof(BndSeg),
+                                 hipMemcpyHostToDevice, c->stream));
     if (int rc = dalloc(c, c->red, (size_t)(nb + 1) * kNAcc * sizeof(double))) return rc;
     for (int i = 0; i < 2; i++) {
         if (int rc = dalloc(c, c->parity[i], (size_t)std::max(nb, 16) * sizeof(int32_t)))
@@ -697,8 +859,12 @@ int mplbhip_node_info(mplbhip_ctx* c, int block, int which, int* out) {
         for (int j = 0; j < n[1]; j++)
             for (int i = 0; i < n[0]; i++) {
                 const int r = c->plan.region_of(block, i, j, k);
-                out[p++] = which == 0 ? c->plan.region_vt[(size_t)block * R + r]
-                                      : c->plan.region_vg[(size_t)block * R + r];
+                const bool obst = !c->obst.empty() &&
+                                  (c->obst[(size_t)c->bp[block].mbase + p] & kObstSolid);
+                out[p] = which == 0 ? (obst ? (int)VT_ImmersedSolid
+                                            : c->plan.region_vt[(size_t)block * R + r])
+                                    : c->plan.region_vg[(size_t)block * R + r];
+                p++;
             }
     return 0;
 }
@@ -812,15 +978,17 @@ static int run_steps(mplbhip_ctx* c, int nsteps, bool timed, double* ms_total,
         if (int rc = first_collision(c)) return rc;
     double ktot = 0.0;
     if (timed) {
-        // per-step events bracket the fused kernel only
-        std::vector<hipEvent_t> evs(2 * (size_t)nsteps);
+        // per-step events bracket the bulk kernel and the boundary kernel
+        std::vector<hipEvent_t> evs(3 * (size_t)nsteps);
         for (auto& e : evs) HIPCHK(c, hipEventCreate(&e));
         HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
         for (int s = 0; s < nsteps; s++) {
             KParams kp = base_params(c);
-            HIPCHK(c, hipEventRecord(evs[2 * s], c->stream));
-            HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP, kp, c->stream));
-            HIPCHK(c, hipEventRecord(evs[2 * s + 1], c->stream));
+            HIPCHK(c, hipEventRecord(evs[3 * s], c->stream));
+            HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP, kp, c->stream, 1));
+            HIPCHK(c, hipEventRecord(evs[3 * s + 1], c->stream));
+            HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP, kp, c->stream, 2));
+            HIPCHK(c, hipEventRecord(evs[3 * s + 2], c->stream));
             if (int rc = exchange(c, 1 - c->cur)) return rc;
             c->cur ^= 1;
             c->tstep++;
@@ -830,11 +998,17 @@ static int run_steps(mplbhip_ctx* c, int nsteps, bool timed, double* ms_total,
         float ms = 0.f;
         HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
         if (ms_total) *ms_total = ms;
+        double btot = 0.0;
         for (int s = 0; s < nsteps; s++) {
-            float k = 0.f;
-            HIPCHK(c, hipEventElapsedTime(&k, evs[2 * s], evs[2 * s + 1]));
+            float k = 0.f, kb = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&k, evs[3 * s], evs[3 * s + 1]));
+            HIPCHK(c, hipEventElapsedTime(&kb, evs[3 * s + 1], evs[3 * s + 2]));
             ktot += k;
+            btot += kb;
         }
+        c->last_bulk_ms = ktot;
+        c->last_bnd_ms = btot;
+        ktot += btot;
         for (auto& e : evs) (void)hipEventDestroy(e);
         if (ms_kernel) *ms_kernel = ktot;
         return 0;
@@ -891,7 +1065,7 @@ int mplbhip_iterate(mplbhip_ctx* c, double criteria, int check_period,
             double acc[kNAcc];
             HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
                                     static_cast<const BlockParams*>(c->blocks.p),
-                                    (int)c->bp.size(), c->ntiles,
+                                    (int)c->bp.size(), c->ntiles, c->nbulk, c->nbnd, 1,
                                     static_cast<double*>(c->red.p), c->stream));
             HIPCHK(c, hipMemcpyAsync(acc, static_cast<double*>(c->red.p) +
                                               c->bp.size() * kNAcc,
@@ -966,7 +1140,8 @@ int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
         done++;
         HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
                                 static_cast<const BlockParams*>(c->blocks.p), nb,
-                                c->ntiles, static_cast<double*>(c->red.p), c->stream));
+                                c->ntiles, c->nbulk, c->nbnd, 1,
+                                static_cast<double*>(c->red.p), c->stream));
         HIPCHK(c, hipMemcpyAsync(acc.data(), c->red.p, acc.size() * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1018,6 +1193,13 @@ int mplbhip_step(mplbhip_ctx* c, int nsteps) {
 int mplbhip_step_timed(mplbhip_ctx* c, int nsteps, double* ms_total, double* ms_kernel) {
     if (!c) return MPLBHIP_ERR_ARG;
     return run_steps(c, nsteps, true, ms_total, ms_kernel);
+}
+
+int mplbhip_kernel_times(mplbhip_ctx* c, double* ms_bulk, double* ms_boundary) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    if (ms_bulk) *ms_bulk = c->last_bulk_ms;
+    if (ms_boundary) *ms_boundary = c->last_bnd_ms;
+    return 0;
 }
 
 int mplbhip_update_macros(mplbhip_ctx* c) {

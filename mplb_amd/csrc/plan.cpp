@@ -444,6 +444,11 @@ bool Plan::build() {
         error = "persisted boundary state too large";
         return false;
     }
+    for (RegionProg& P : progs) {
+        bool eqm_only = true;
+        for (int o = 0; o < P.nops; o++) eqm_only = eqm_only && P.op[o].type == OP_EQM;
+        P.simple = (P.kind == RK_FULL || P.kind == RK_MASKED) && P.persist < 0 && eqm_only;
+    }
     return true;
 }
 

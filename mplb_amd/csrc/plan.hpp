@@ -60,6 +60,8 @@ struct RegionProg {
     uint32_t pull;       // populations streamed from the neighbours
     int32_t persist;     // element offset of this region's persist slots, -1 none
     int32_t ext[3];      // region extent (nodes) per axis, for persist indexing
+    int32_t simple;      // pull (full or masked) + EQM ops only, no persisted
+                         // state: the bulk kernel may run it inline
     int32_t nops;
     BCOp op[kMaxOps];
 };

@@ -55,6 +55,7 @@ def load():
         "mplbhip_define_block": (I, [P, I, IP]),
         "mplbhip_define_connection": (I, [P, I, I, I, I, I]),
         "mplbhip_define_rotated_connection": (I, [P, I, I, I, I, I]),
+        "mplbhip_define_obstacles": (I, [P, I, ctypes.c_void_p]),
         "mplbhip_define_collision": (I, [P, D, I, I]),
         "mplbhip_define_boundary": (I, [P, I, I, I, DP, I, I]),
         "mplbhip_prepare": (I, [P]),
@@ -66,6 +67,7 @@ def load():
         "mplbhip_set_accel": (I, [P, I, DP]),
         "mplbhip_step": (I, [P, I]),
         "mplbhip_step_timed": (I, [P, I, DP, DP]),
+        "mplbhip_kernel_times": (I, [P, DP, DP]),
         "mplbhip_update_macros": (I, [P]),
         "mplbhip_get_f": (I, [P, I, DP]),
         "mplbhip_get_macro": (I, [P, I, I, DP]),
@@ -143,6 +145,10 @@ class LatticeCase:
         self._ck(self.lib.mplbhip_define_connection(self.h, frm, int(fsurf), to,
                                                     int(tsurf), int(vtype)))
 
+    def set_obstacles(self, block, solid):
+        m = np.ascontiguousarray(np.asarray(solid, dtype=np.uint8))
+        self._ck(self.lib.mplbhip_define_obstacles(self.h, block, m.ctypes.data))
+
     def add_rotated_connection(self, frm, fsurf, to, tsurf, vtype):
         self._ck(self.lib.mplbhip_define_rotated_connection(self.h, frm, int(fsurf), to,
                                                             int(tsurf), int(vtype)))
@@ -205,6 +211,13 @@ class LatticeCase:
         self._ck(self.lib.mplbhip_step_timed(self.h, int(n), ctypes.byref(tot),
                                              ctypes.byref(ker)))
         return tot.value, ker.value
+
+    def kernel_times(self):
+        """(bulk ms, boundary ms) summed over the last step_timed call."""
+        b = ctypes.c_double()
+        d = ctypes.c_double()
+        self._ck(self.lib.mplbhip_kernel_times(self.h, ctypes.byref(b), ctypes.byref(d)))
+        return b.value, d.value
 
     def iterate(self, criteria, check_period, max_steps):
         """Iterate(convergenceCriteria, checkPeriod) when criteria >= 0,

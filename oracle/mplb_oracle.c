@@ -70,6 +70,7 @@ typedef struct {
     double *m[4];      /* rho, u, v, w */
     double *mc[4];     /* copies for the residual (MacroVarsCopy) */
     int *nt, *vg;      /* node type, geometry property */
+    unsigned char* solid; /* obstacle nodes (extension), NULL when none */
     double g[3];       /* uniform acceleration */
 } OBlock;
 
@@ -225,7 +226,7 @@ void orc_destroy(orc_case* c) {
     if (!c) return;
     for (int ib = 0; ib < c->nb; ib++) {
         OBlock* b = &c->b[ib];
-        free(b->f); free(b->fs); free(b->nt); free(b->vg);
+        free(b->f); free(b->fs); free(b->nt); free(b->vg); free(b->solid);
         for (int v = 0; v < 4; v++) { free(b->m[v]); free(b->mc[v]); }
     }
     free(c);
@@ -516,6 +517,90 @@ static void copy_current_macros(orc_case* c) {
     }
 }
 
+/* Extension with no reference counterpart: in-domain obstacles.  Obstacle
+ * nodes are ImmersedSolid (no macros, collision or streaming, like the
+ * reference's solid nodes, model_kernel.inc:420, scheme_kernel.inc:1418);
+ * a node that pulls every population (Fluid, MDPeriodic, VirtualBoundary)
+ * and whose pull source x - c_q is an obstacle node takes the half-way
+ * bounce-back value f_q(x) = fs_opp(q)(x) of a resting wall.  Sources in a
+ * halo are resolved through the block connections.  orc_prepare refuses
+ * obstacles on nodes that are not Fluid or MDPeriodic, or next to nodes
+ * that do not pull every population. */
+int orc_set_obstacles(orc_case* c, int block, const unsigned char* solid) {
+    if (c->prepared) return fail(c, "obstacles are set before orc_prepare");
+    if (block < 0 || block >= c->nb) return fail(c, "undefined block");
+    OBlock* b = &c->b[block];
+    if (!b->solid) b->solid = (unsigned char*)calloc(b->nalloc, 1);
+    size_t p = 0;
+    for (int k = 0; k < b->n[2]; k++)
+        for (int j = 0; j < b->n[1]; j++)
+            for (int i = 0; i < b->n[0]; i++, p++)
+                b->solid[nidx(b, i, j, k)] = solid[p] ? 1 : 0;
+    return 0;
+}
+
+/* obstacle flags of halo nodes, copied through the connections like fStage */
+static void obstacle_halos(orc_case* c) {
+    for (int l = 0; l < c->nlink; l++) {
+        const OLink* L = &c->link[l];
+        OBlock* a = &c->b[L->from];
+        OBlock* t = &c->b[L->to];
+        if (!a->solid) continue;
+        if (!t->solid) t->solid = (unsigned char*)calloc(t->nalloc, 1);
+        int side;
+        const int ax = face_axis(L->fsurf, &side);
+        int lo[3] = {0, 0, 0}, hi[3] = {a->n[0], a->n[1], a->n[2]};
+        lo[ax] = side > 0 ? a->n[ax] - 1 : 0;
+        hi[ax] = lo[ax] + 1;
+        for (int k = lo[2]; k < hi[2]; k++)
+            for (int j = lo[1]; j < hi[1]; j++)
+                for (int i = lo[0]; i < hi[0]; i++) {
+                    const int p[3] = {i, j, k};
+                    int d[3];
+                    link_target(c, L, p, d);
+                    t->solid[nidx(t, d[0], d[1], d[2])] = a->solid[nidx(a, i, j, k)];
+                }
+    }
+}
+
+static int full_pull(int vt) {
+    return vt == VT_FLUID || vt == VT_MDPERIODIC || vt == VT_VIRTUALBOUNDARY;
+}
+
+static int mark_obstacles(orc_case* c) {
+    obstacle_halos(c);
+    for (int ib = 0; ib < c->nb; ib++) {
+        OBlock* b = &c->b[ib];
+        if (!b->solid) continue;
+        for (int k = 0; k < b->n[2]; k++)
+            for (int j = 0; j < b->n[1]; j++)
+                for (int i = 0; i < b->n[0]; i++) {
+                    const size_t n = nidx(b, i, j, k);
+                    if (b->solid[n]) {
+                        if (b->nt[n] != VT_FLUID && b->nt[n] != VT_MDPERIODIC)
+                            return fail(c, "obstacle on a node that is not Fluid or MDPeriodic");
+                        continue;
+                    }
+                    /* a node that pulls from an obstacle must pull everything */
+                    for (int q = 0; q < c->Q; q++) {
+                        const size_t m = nidx(b, i - XIi(c, q, 0), j - XIi(c, q, 1),
+                                              k - XIi(c, q, 2));
+                        if (b->solid[m] && !full_pull(b->nt[n]))
+                            return fail(c, "obstacle next to a boundary node");
+                    }
+                }
+    }
+    for (int ib = 0; ib < c->nb; ib++) {
+        OBlock* b = &c->b[ib];
+        if (!b->solid) continue;
+        for (int k = 0; k < b->n[2]; k++)
+            for (int j = 0; j < b->n[1]; j++)
+                for (int i = 0; i < b->n[0]; i++)
+                    if (b->solid[nidx(b, i, j, k)]) b->nt[nidx(b, i, j, k)] = VT_IMMERSEDSOLID;
+    }
+    return 0;
+}
+
 int orc_prepare(orc_case* c) {
     for (int l = 0; l < c->nlink; l++) {
         const OLink* L = &c->link[l];
@@ -537,6 +622,7 @@ int orc_prepare(orc_case* c) {
         set_bulk_halo_types(c, &c->b[ib]);
     }
     set_boundary_types(c);
+    if (mark_obstacles(c)) return -1;
     copy_current_macros(c);
     c->prepared = 1;
     return 0;
@@ -827,7 +913,10 @@ static void stream(orc_case* c) {
                         size_t src = nidx(b, i - cv[0], j - cv[1], k - cv[2]);
                         if (vt == VT_FLUID || vt == VT_MDPERIODIC ||
                             vt == VT_VIRTUALBOUNDARY) {
-                            b->f[n * Q + q] = b->fs[src * Q + q];
+                            if (b->solid && b->solid[src])
+                                b->f[n * Q + q] = b->fs[n * Q + c->OPP[q]];
+                            else
+                                b->f[n * Q + q] = b->fs[src * Q + q];
                             continue;
                         }
                         if (vt == VT_IMMERSEDSOLID) continue;

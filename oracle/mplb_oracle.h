@@ -36,6 +36,10 @@ int orc_add_connection(orc_case* c, int from_block, int from_surface,
  * that takes the from-face's outward normal to the to-face's inward one. */
 int orc_add_rotated_connection(orc_case* c, int from_block, int from_surface,
                                int to_block, int to_surface, int vertex_type);
+/* Extension with no reference counterpart: obstacle nodes of a block
+ * (interior [k][j][i], nonzero = solid), at least two nodes inside every
+ * face; Fluid neighbours bounce back half-way.  Before orc_prepare. */
+int orc_set_obstacles(orc_case* c, int block, const unsigned char* solid);
 /* DefineComponents tauRef + DefineBodyForce (model.cpp:208-310, 447-489).
  * force_type: 0 = BodyForce_None, 1 = BodyForce_1st.
  * force_macros: 1 selects the Variable_*_Force velocity kernels

@@ -38,6 +38,7 @@ def load():
         "orc_add_block": (I, [P, I, I, I, I]),
         "orc_add_connection": (I, [P, I, I, I, I, I]),
         "orc_add_rotated_connection": (I, [P, I, I, I, I, I]),
+        "orc_set_obstacles": (I, [P, I, ctypes.c_void_p]),
         "orc_set_collision": (I, [P, D, I, I]),
         "orc_add_boundary": (I, [P, I, I, I, DP, I, I]),
         "orc_prepare": (I, [P]),
@@ -100,6 +101,10 @@ class OracleCase:
     def add_connection(self, frm, fsurf, to, tsurf, vtype):
         self._ck(self.lib.orc_add_connection(self.h, frm, int(fsurf), to,
                                              int(tsurf), int(vtype)))
+
+    def set_obstacles(self, block, solid):
+        m = np.ascontiguousarray(np.asarray(solid, dtype=np.uint8))
+        self._ck(self.lib.orc_set_obstacles(self.h, block, m.ctypes.data))
 
     def add_rotated_connection(self, frm, fsurf, to, tsurf, vtype):
         self._ck(self.lib.orc_add_rotated_connection(self.h, frm, int(fsurf), to,
