@@ -190,6 +190,56 @@ def bench_batch(args):
     }
 
 
+def bench_cavity2d(args):
+    """The 2D cavity: D2Q9 128^2, Re 100, lid 0.05, FP64 (bit-exact with the
+    oracle), 10,000 steps from rest; small enough to live in the caches, so
+    it measures launch and latency costs rather than HBM."""
+    n, K = 128, args.steps
+    g = hip.LatticeCase(2, storage=STORE[args.storage], device=0)
+    cases.cavity2d(g, n, re=100.0, ulid=0.05)
+    g.step(args.warmup)
+    g.sync()
+    t0 = time.perf_counter()
+    g.step(K)
+    g.sync()
+    wall = time.perf_counter() - t0
+    mlups = n * n * K / wall / 1e6
+    log("bench: cavity2d %d^2 %s, %d steps in %.3f s, %.0f MLUPS" % (n, args.storage, K, wall,
+                                                                     mlups))
+    return {
+        "metric": "MLUPS and wall time, 2D cavity", "value": round(mlups, 1), "unit": "MLUPS",
+        "n_gpus": 1, "steps": K, "warmup": args.warmup, "ms_per_step": round(wall / K * 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.storage, "data": "synthetic: rest state",
+        "config": {"workload": "2D lid-driven cavity, D2Q9 128^2, Re=100, lid 0.05, %s"
+                               % args.storage, "nodes": n * n},
+        "wall_s": round(wall, 4),
+    }
+
+
+def bench_warmstart(args):
+    """Learned warm start on 128^2 cavities with 0-3 rectangular obstacles:
+    training data from the batched solver, U-Net training on the matrix
+    cores, then time-to-residual 1e-8 of learned versus rest starts on the
+    held-out cases."""
+    from mplb_amd import warmstart
+    res, _ = warmstart.run(n=args.ws_n, n_train=args.ws_train, n_test=args.ws_test,
+                           epochs=args.ws_epochs, log=log)
+    log("bench: warm start %s" % res)
+    return {
+        "metric": "time-to-residual 1e-8, learned vs rest initialisation",
+        "value": res["speedup"], "unit": "x (rest seconds / learned seconds)", "n_gpus": 1,
+        "steps": None, "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 lattice, bf16 network",
+        "data": "synthetic: seeded cavities with rectangular obstacles",
+        "config": {"workload": "%d held-out %d^2 D2Q9 cavities, Re log-uniform in "
+                               "[100,1000], 0-3 rectangles; U-Net %d parameters trained "
+                               "on %d cases" % (res["test_cases"], res["n"], res["params"],
+                                                res["train_cases"])},
+        "warmstart": res,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,7 +251,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true",
                     help="skip the FP64 comparison run")
-    ap.add_argument("--workload", choices=("cavity3d", "lchannel", "batch"),
+    ap.add_argument("--ws-n", type=int, default=128)
+    ap.add_argument("--ws-train", type=int, default=4096)
+    ap.add_argument("--ws-test", type=int, default=512)
+    ap.add_argument("--ws-epochs", type=int, default=30)
+    ap.add_argument("--workload", choices=("cavity3d", "cavity2d", "lchannel", "batch",
+                                           "warmstart"),
                     default="cavity3d",
                     help="cavity3d: the headline line; lchannel and batch: the "
                          "multi-block and batched workloads")
@@ -221,6 +276,12 @@ def main():
         return
     if args.workload == "batch":
         print(json.dumps(bench_batch(args)), flush=True)
+        return
+    if args.workload == "cavity2d":
+        print(json.dumps(bench_cavity2d(args)), flush=True)
+        return
+    if args.workload == "warmstart":
+        print(json.dumps(bench_warmstart(args)), flush=True)
         return
 
     n, K, W = args.n, args.steps, args.warmup

@@ -110,6 +110,12 @@ int mplbhip_set_macros(mplbhip_ctx* ctx, int block, const double* rho,
 /* PreDefinedInitialCondition[3D] (model_wrapper.cpp:347-391): f = feq of
  * the uploaded macros at every non-solid node. */
 int mplbhip_init_feq(mplbhip_ctx* ctx);
+/* No reference counterpart (warm start from a predicted flow): f = feq of
+ * the uploaded macros plus the first-order Chapman-Enskog non-equilibrium
+ * part of BGK, f1_q = -3 tau_L w_q rho (c_qa c_qb - delta_ab/3) d_a u_b in
+ * lattice units, with central-difference gradients (one-sided at block
+ * faces and obstacles).  Obstacle nodes get f = 0. */
+int mplbhip_init_lift(mplbhip_ctx* ctx);
 /* restart: set f of every block (array per block, [q][k][j][i]) */
 int mplbhip_set_f(mplbhip_ctx* ctx, int block, const double* f);
 /* SetTimeStep (flowfield.cpp:146) */

@@ -1,4 +1,5 @@
-"""Builds libmplbhip.so in-tree for gfx950 with hipcc (no JIT, no cache
+"""Builds libmplbhip.so (the lattice library) and libmplbws.so (the warm
+start network kernels) in-tree for gfx950 with hipcc (no JIT, no cache
 outside the repository): `python -m mplb_amd.build`."""
 import os
 import subprocess
@@ -18,11 +19,21 @@ HIP_FLAGS = [
 SOURCES = ["plan.cpp", "mplbhip.cpp", "lattice_kernels.hip"]
 
 
+WS_OUT = os.path.join(HERE, "libmplbws.so")
+WS_SOURCES = ["warmstart.hip"]
+
+
 def build(verbose=False):
+    out = _build_lib(SOURCES, OUT, verbose)
+    _build_lib(WS_SOURCES, WS_OUT, verbose)
+    return out
+
+
+def _build_lib(sources, target, verbose):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     objs = []
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
-    for src in SOURCES:
+    for src in sources:
         obj = os.path.join(HERE, "_build", os.path.splitext(src)[0] + ".o")
         cmd = [hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".hip"):
@@ -31,11 +42,11 @@ def build(verbose=False):
             print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd)
         objs.append(obj)
-    tmp = OUT + ".tmp"
+    tmp = target + ".tmp"
     cmd = [hipcc, "--offload-arch=" + ARCH, "-shared", "-o", tmp] + objs
     subprocess.check_call(cmd)
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, target)
+    return target
 
 
 APPS_DIR = os.path.join(HERE, "hilemms")

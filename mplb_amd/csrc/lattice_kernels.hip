@@ -326,6 +326,14 @@ __device__ __forceinline__ int logical_wg(int wg, int n) {
 
 // block of logical workgroup t: the last block whose first workgroup
 // (field `first`) is <= t
+// workgroup -> tile such that the tx tiles of one row go to the same XCD
+// (workgroups b, b+8, b+16, ... share an XCD) while consecutive rows are
+// spread over the 8 XCDs; a bijection when the grid is a multiple of 8*tx
+__device__ __forceinline__ int row_group_wg(int b, int tx) {
+    const int j = b >> 3;
+    return ((j / tx) * 8 + (b & 7)) * tx + j % tx;
+}
+
 template <int32_t BlockParams::*first>
 __device__ __forceinline__ int find_block(const KParams& kp, int t) {
     int lo = 0, hi = kp.nblocks - 1;
@@ -662,12 +670,13 @@ reduce:
 // is the bulk's alone.  Tiles of kTX x kTY nodes, z planes 1 .. n-2 (3D).
 // The block's first bulk workgroup also records the block's parity after
 // the step.
-template <int D, int ST, int MODE, bool OBST>
+template <int D, int ST, int MODE, bool OBST, bool FOLD>
 __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
     using T = typename Store<ST>::T;
     using E = typename Store<ST>::E;
     constexpr int Q = D == 2 ? 9 : 19;
-    const int wg = logical_wg(blockIdx.x, gridDim.x);
+    const int wg = kp.xgroup ? row_group_wg(blockIdx.x, kp.xgroup)
+                   : kp.xcd_remap ? logical_wg(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int b = find_block<&BlockParams::bulk_tile_start>(kp, wg);
     const BlockParams& B = kp.blocks[b];
     int t = wg - B.bulk_tile_start;
@@ -689,7 +698,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
     const bool yin = y >= 1 && y < B.n[1] - 1;
     const bool interior = x >= 1 && x < B.n[0] - 1 && yin;
     const RegionProg* P = nullptr;
-    if (!interior && yin && (x == 0 || x == B.n[0] - 1)) {
+    if (FOLD && !interior && yin && (x == 0 || x == B.n[0] - 1)) {
         P = &kp.progs[B.prog_base + (x == 0 ? 1 : 2)];
         if (!P->simple) P = nullptr;
     }
@@ -707,28 +716,25 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
             solid_residual<D, MODE>(kp, mnode, acc);
         } else {
             T f[Q];
+            // every lane loads all Q populations (an x-face lane's unpulled
+            // ones come from its halo neighbour, in bounds), so the loads
+            // stay one uniform burst; the pull mask applies afterwards
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 const bool bounce = OBST && (om >> q & 1u);
                 const E* p = bounce ? at_u(prev, (int64_t)opp(D, q) * B.sq, (uint32_t)node)
                                     : at_u(prev, (int64_t)q * B.sq - pull_delta<D>(B, q),
                                            (uint32_t)node);
-                T v = T(0);
-                if (pull >> q & 1u) {
-                    if constexpr (ST == MPLBHIP_STORE_F16)
-                        v = __half2float(*p) * kp.scale[bounce ? opp(D, q) : q];
-                    else
-                        v = *p;
-                }
-                f[q] = v;
+                if constexpr (ST == MPLBHIP_STORE_F16) f[q] = __half2float(*p) * kp.scale[q];
+                else f[q] = *p;
             }
-            if (P) {
-                // full values (unpulled populations start at zero; the EQM
+            if (FOLD && P) {
+                // full values; unpulled populations start at zero (the EQM
                 // applications set every one of them), then the boundary ops
-                if constexpr (ST != MPLBHIP_STORE_F64) {
 #pragma unroll
-                    for (int q = 0; q < Q; q++)
-                        if (pull >> q & 1u) f[q] += T(wgt(D, q));
+                for (int q = 0; q < Q; q++) {
+                    if constexpr (ST != MPLBHIP_STORE_F64) f[q] += T(wgt(D, q));
+                    f[q] = (pull >> q & 1u) ? f[q] : T(0);
                 }
                 for (int o = 0; o < P->nops; o++) apply_eqm<D, T>(f, P->op[o]);
             }
@@ -737,6 +743,74 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
         }
     }
     if constexpr (MODE == MODE_STEP_CHECK) reduce_store(acc, kp.partials + (int64_t)wg * kNAcc);
+}
+
+// ---- warm-start initial state ---------------------------------------------------
+// f = feq(rho, u) + f1, with f1 the first-order (Chapman-Enskog)
+// non-equilibrium part of BGK in lattice units (velocity u_L = u / CS,
+// unit node spacing and time step):
+//   f1_q = -3 tau_L w_q rho sum_ab (c_qa c_qb - delta_ab / 3) d_a u_L,b ,
+// tau_L = tau / dt + 1/2 of the block.  Gradients are central differences,
+// one-sided where a neighbour is outside the block or an obstacle node.
+// Obstacle nodes get f = 0 (ImmersedSolid).  Writes the host-layout array
+// fhost that MODE_INIT_F then collides.  No reference counterpart.
+template <int D>
+__global__ void k_lift(KParams kp, int64_t nodes) {
+    constexpr int Q = D == 2 ? 9 : 19;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nodes;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = kp.nblocks - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (kp.blocks[mid].mbase <= i) lo = mid; else hi = mid - 1;
+        }
+        const BlockParams& B = kp.blocks[lo];
+        const int64_t local = i - B.mbase;
+        const int64_t nn = (int64_t)B.n[0] * B.n[1] * B.n[2];
+        double* out = kp.fhost + B.fbase + local;
+        if (obst_mask(kp, B, i) & kObstSolid) {
+            for (int q = 0; q < Q; q++) out[q * nn] = 0.0;
+            continue;
+        }
+        const int c[3] = {(int)(local % B.n[0]), (int)((local / B.n[0]) % B.n[1]),
+                          (int)(local / ((int64_t)B.n[0] * B.n[1]))};
+        const int64_t stride[3] = {1, B.n[0], (int64_t)B.n[0] * B.n[1]};
+        double g[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // g[a][b] = d_a u_b
+        for (int a = 0; a < D; a++) {
+            const bool hp = c[a] + 1 < B.n[a] &&
+                            !(obst_mask(kp, B, i + stride[a]) & kObstSolid);
+            const bool hm = c[a] > 0 && !(obst_mask(kp, B, i - stride[a]) & kObstSolid);
+            for (int b = 0; b < D; b++) {
+                const double* u = kp.macro[1 + b];
+                const double u0 = u[i];
+                double d = 0.0;
+                if (hp && hm) d = 0.5 * (u[i + stride[a]] - u[i - stride[a]]);
+                else if (hp) d = u[i + stride[a]] - u0;
+                else if (hm) d = u0 - u[i - stride[a]];
+                g[a][b] = d / kCS;
+            }
+        }
+        const double rho = kp.macro[0][i];
+        const double tl = B.tau / B.dt + 0.5;
+        const double div = g[0][0] + g[1][1] + g[2][2];
+        for (int q = 0; q < Q; q++) {
+            double s = 0.0;
+            for (int a = 0; a < D; a++)
+                for (int b = 0; b < D; b++) s += cvel(D, q, a) * cvel(D, q, b) * g[a][b];
+            const double f1 = -3.0 * tl * wgt(D, q) * rho * (s - div / 3.0);
+            out[q * nn] = feq_ref<D, double>(q, rho, kp.macro[1][i], kp.macro[2][i],
+                                             D == 3 ? kp.macro[3][i] : 0.0) + f1;
+        }
+    }
+}
+
+hipError_t launch_lift(int dim, const KParams& kp, int64_t nodes, hipStream_t s) {
+    int64_t grid = (nodes + 255) / 256;
+    if (grid > 16384) grid = 16384;
+    if (grid < 1) grid = 1;
+    if (dim == 2) hipLaunchKernelGGL(k_lift<2>, dim3((unsigned)grid), dim3(256), 0, s, kp, nodes);
+    else hipLaunchKernelGGL(k_lift<3>, dim3((unsigned)grid), dim3(256), 0, s, kp, nodes);
+    return hipGetLastError();
 }
 
 // fStage halo slab copy of one block connection (Field::TransferHalos ->
@@ -810,10 +884,14 @@ template <int D, int ST, int MODE>
 static void launch_step(const KParams& kp, int part, hipStream_t s) {
     dim3 blk(kTX, kTY, 1);
     if (part != 2 && kp.nbulk > 0) {
-        if (kp.obst)
-            hipLaunchKernelGGL((k_bulk<D, ST, MODE, true>), dim3(kp.nbulk), blk, 0, s, kp);
-        else
-            hipLaunchKernelGGL((k_bulk<D, ST, MODE, false>), dim3(kp.nbulk), blk, 0, s, kp);
+        const dim3 g(kp.nbulk);
+        if (kp.obst) {
+            if (kp.fold) hipLaunchKernelGGL((k_bulk<D, ST, MODE, true, true>), g, blk, 0, s, kp);
+            else hipLaunchKernelGGL((k_bulk<D, ST, MODE, true, false>), g, blk, 0, s, kp);
+        } else {
+            if (kp.fold) hipLaunchKernelGGL((k_bulk<D, ST, MODE, false, true>), g, blk, 0, s, kp);
+            else hipLaunchKernelGGL((k_bulk<D, ST, MODE, false, false>), g, blk, 0, s, kp);
+        }
     }
     if (part != 1 && kp.nbnd > 0)
         hipLaunchKernelGGL((k_lattice<D, ST, MODE, SCOPE_BND>), dim3(kp.nbnd), blk, 0, s, kp);

@@ -93,6 +93,9 @@ struct KParams {
     int32_t want_residual;  // accumulate the residual against MacroVarsCopy
     const uint32_t* obst;   // obstacle link masks per interior node, or null
     const BndSeg* segs;     // boundary segments, blocks in order
+    int32_t fold;           // the bulk kernel runs simple x-face regions
+    int32_t xcd_remap;      // bulk tiles dealt to XCDs in contiguous runs
+    int32_t xgroup;         // >0: the xgroup tiles of a row run on one XCD
     int32_t nbulk;          // bulk workgroups (partials 0 .. nbulk-1)
     int32_t nbnd;           // boundary workgroups (partials nbulk ..)
 };

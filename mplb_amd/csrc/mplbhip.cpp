@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -24,6 +25,7 @@ hipError_t launch_reduce(const double* partials, const BlockParams* blocks,
                          int nblocks, int ntiles, int nbulk, int nbnd, int split,
                          double* out, hipStream_t s);
 hipError_t launch_fill(void* p, int64_t bytes, hipStream_t s);
+hipError_t launch_lift(int dim, const KParams& kp, int64_t nodes, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
 }  // namespace mplbhip
 
@@ -56,6 +58,7 @@ struct mplbhip_ctx {
     bool prepared = false;
     bool initialized = false;
     bool init_from_f = false;
+    bool init_lift = false;  // f^0 = feq + first-order non-equilibrium part
     long long tstep = 0;
     int cur = 0;  // parity of the latest fStage of every active block; also
                   // selects the per-block parity array read by the next step
@@ -68,6 +71,9 @@ struct mplbhip_ctx {
     int64_t interior_nodes = 0;
     int ntiles = 0;            // whole-lattice tiles
     int nbulk = 0, nbnd = 0;   // bulk tiles and boundary workgroups of a step
+    bool fold = true;          // simple x-face regions run inside the bulk kernel
+    bool xcd_remap = false;    // bulk tiles dealt to XCDs in contiguous runs
+    int xgroup = 0;            // tiles per row kept on one XCD (0: launch order)
     std::vector<BndSeg> segs;
     std::vector<std::vector<uint8_t>> solid;  // obstacle nodes per block (empty: none)
     std::vector<uint32_t> obst;               // link masks per interior node
@@ -183,6 +189,9 @@ static KParams base_params(mplbhip_ctx* c) {
     kp.ntiles = c->ntiles;
     kp.nbulk = c->nbulk;
     kp.nbnd = c->nbnd;
+    kp.fold = c->fold ? 1 : 0;
+    kp.xcd_remap = c->xcd_remap ? 1 : 0;
+    kp.xgroup = c->xgroup;
     kp.segs = static_cast<const BndSeg*>(c->dsegs.p);
     kp.obst = c->obst.empty() ? nullptr : static_cast<const uint32_t*>(c->dobst.p);
     kp.progs = static_cast<const RegionProg*>(c->progs.p);
@@ -280,6 +289,10 @@ static int materialize(mplbhip_ctx* c, bool want_f, bool residual,
         if (c->init_from_f) {
             HIPCHK(c, hipMemcpyAsync(c->fhost.p, c->host_init_f.data(), fbytes,
                                      hipMemcpyHostToDevice, c->stream));
+        } else if (c->init_lift) {
+            KParams ki = base_params(c);
+            ki.fhost = static_cast<double*>(c->fhost.p);
+            HIPCHK(c, launch_lift(c->dim, ki, c->interior_nodes, c->stream));
         } else {
             KParams ki = base_params(c);
             ki.fhost = static_cast<double*>(c->fhost.p);
@@ -722,14 +735,20 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     if (c->prepared) return fail(c, MPLBHIP_ERR_ARG, "case already prepared");
     if (c->plan.blocks.empty()) return fail(c, MPLBHIP_ERR_ARG, "no blocks defined");
     if (!c->plan.build()) return fail(c, MPLBHIP_ERR_UNSUPPORTED, "%s", c->plan.error.c_str());
-    // HBM layout: one halo node per axis, x = 0 aligned to 64 bytes
+    // HBM layout: one halo node per axis, x = 0 and every row aligned to a
+    // 128-byte cache line, so a wave's 64-node row segment spans whole lines
     const int nb = (int)c->plan.blocks.size();
     const int hz = c->dim == 3 ? 1 : 0;
-    const int align = (int)(64 / c->es);
+    const int align = (int)(128 / c->es);
     c->bp.assign(nb, BlockParams{});
     int64_t off = 0, moff = 0;
     int tiles = 0, bulk = 0, bnd = 0;
     c->segs.clear();
+    // measured at 256^3: folding pays for 4- and 8-byte populations; with
+    // 2-byte ones the extra lanes cost more than the strided boundary loads
+    c->fold = c->storage != MPLBHIP_STORE_F16;
+    if (const char* e = std::getenv("MPLBHIP_XFACE_FOLD")) c->fold = std::atoi(e) != 0;
+    if (const char* e = std::getenv("MPLBHIP_XCD_REMAP")) c->xcd_remap = std::atoi(e) != 0;
     for (int b = 0; b < nb; b++) {
         const int* n = c->plan.blocks[b].n;
         BlockParams& B = c->bp[b];
@@ -767,7 +786,8 @@ int mplbhip_prepare(mplbhip_ctx* c) {
             const int code[3] = {r % 3, (r / 3) % 3, r / 9};
             // x-face interiors with simple programs run inside the bulk
             // kernel, whose waves already load their rows
-            if (r < 3 && c->plan.progs[(size_t)b * c->plan.nregions + r].simple) continue;
+            if (c->fold && r < 3 && c->plan.progs[(size_t)b * c->plan.nregions + r].simple)
+                continue;
             BndSeg S;
             for (int a = 0; a < 3; a++) {
                 if (a == 2 && c->dim == 2) {
@@ -798,6 +818,16 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     c->ntiles = tiles;
     c->nbulk = bulk;
     c->nbnd = bnd;
+    // the x tiles of a row share their edge cache lines: deal each row's
+    // tiles to one XCD (hardware hands workgroups to the 8 XCDs round
+    // robin) when every block has the same row of tiles
+    c->xgroup = c->bp[0].tiles_x;
+    for (int b = 0; b < nb; b++)
+        if (c->bp[b].tiles_x != c->xgroup) c->xgroup = 0;
+    if (c->xgroup > 0 && bulk % (8 * c->xgroup) != 0) c->xgroup = 0;
+    if (const char* e = std::getenv("MPLBHIP_XGROUP")) {
+        if (std::atoi(e) == 0) c->xgroup = 0;
+    }
     if (int rc = setup_halos(c)) return rc;
     if (int rc = build_obstacles(c)) return rc;
     c->prepared = true;
@@ -899,9 +929,16 @@ int mplbhip_init_feq(mplbhip_ctx* c) {
     c->cur = 0;
     c->tstep = 0;
     c->init_from_f = false;
+    c->init_lift = false;
     c->initialized = true;
     // fStage^0 = collide(feq) is formed lazily at the first step, once the
     // time step is known (the Apps call SetTimeStep after this)
+    return 0;
+}
+
+int mplbhip_init_lift(mplbhip_ctx* c) {
+    if (int rc = mplbhip_init_feq(c)) return rc;
+    c->init_lift = true;
     return 0;
 }
 
@@ -920,6 +957,7 @@ int mplbhip_set_f(mplbhip_ctx* c, int block, const double* f) {
     c->cur = 0;
     c->tstep = 0;
     c->init_from_f = true;
+    c->init_lift = false;
     c->initialized = true;
     return 0;
 }
@@ -961,6 +999,13 @@ static int first_collision(mplbhip_ctx* c) {
         HIPCHK(c, hipMemcpyAsync(c->fhost.p, c->host_init_f.data(), fbytes,
                                  hipMemcpyHostToDevice, c->stream));
         kp.fhost = static_cast<double*>(c->fhost.p);
+        mode = MODE_INIT_F;
+    } else if (c->init_lift) {
+        const size_t fbytes = (size_t)c->Q * c->interior_nodes * sizeof(double);
+        if (c->fhost.bytes != fbytes)
+            if (int rc = dalloc(c, c->fhost, fbytes)) return rc;
+        kp.fhost = static_cast<double*>(c->fhost.p);
+        HIPCHK(c, launch_lift(c->dim, kp, c->interior_nodes, c->stream));
         mode = MODE_INIT_F;
     }
     HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->stream));

@@ -1,0 +1,479 @@
+// warmstart.hip -- kernels of the learned warm start (a small U-Net that
+// predicts the steady cavity flow from the obstacle mask and Re): 3x3
+// convolutions as implicit GEMMs on the gfx950 matrix cores
+// (v_mfma_f32_32x32x8_bf16: BF16 operands, FP32 accumulation), their data
+// and weight gradients, and the element-wise layers, loss and optimiser.
+// C ABI in include/mplbws.h.  No reference counterpart: the reference
+// starts every run from its initial macros.
+//
+// Tensors are FP32 in HBM, channels last: x[n][y][x][c].  Operands are
+// rounded to BF16 (nearest even) as they are loaded into the MFMA
+// fragments.  Convolution weights are laid out [tap = 3*ky + kx][cin][cout].
+#include "../../include/mplbws.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kWaves = 4;  // waves per workgroup
+
+__device__ __forceinline__ short to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+    return (short)(u >> 16);
+}
+
+// One wave: a 32 (pixels) x 32 (output channels) tile of
+//   out[m][co] = sum_{tap, ci} in[shift_tap(m)][ci] * w[tap][ci][co]
+// with zero padding at the image edges.  MFMA 32x32x8 lane layout: lane l
+// holds A[l%32][4(l/32) .. +3] and B[4(l/32) .. +3][l%32]; the accumulator
+// element 4i+j of lane l is C[8i + 4(l/32) + j][l%32].
+__global__ __launch_bounds__(64 * kWaves) void k_conv_fwd(const float* __restrict__ in,
+                                                          const float* __restrict__ w,
+                                                          const float* __restrict__ bias,
+                                                          float* __restrict__ out, int n,
+                                                          int h, int wd, int cin, int cout,
+                                                          int relu) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t M = (int64_t)n * h * wd;
+    const int64_t m0 = ((int64_t)blockIdx.x * kWaves + wave) * 32;
+    const int co0 = blockIdx.y * 32;
+    if (m0 >= M) return;
+    const int row = lane & 31, g = lane >> 5;
+    // the A row of this lane: one output pixel
+    const int64_t m = m0 + row;
+    const bool mvalid = m < M;
+    const int px = mvalid ? (int)(m % wd) : 0;
+    const int py = mvalid ? (int)((m / wd) % h) : 0;
+    const int64_t pn = mvalid ? m / ((int64_t)wd * h) : 0;
+    const int co = co0 + row;  // the B column of this lane
+    f32x16 acc = {};
+    for (int tap = 0; tap < 9; tap++) {
+        const int yy = py + tap / 3 - 1, xx = px + tap % 3 - 1;
+        const bool inside = mvalid && yy >= 0 && yy < h && xx >= 0 && xx < wd;
+        const float* src = in + ((pn * h + (inside ? yy : 0)) * wd + (inside ? xx : 0)) * cin;
+        const float* wt = w + (int64_t)tap * cin * cout;
+        for (int c0 = 0; c0 < cin; c0 += 8) {
+            const int ci = c0 + 4 * g;
+            bf16x4 a, b;
+            if (inside) {
+                const float4 v = *reinterpret_cast<const float4*>(src + ci);
+                a[0] = to_bf16(v.x); a[1] = to_bf16(v.y); a[2] = to_bf16(v.z); a[3] = to_bf16(v.w);
+            } else {
+                a = bf16x4{0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                b[t] = co < cout ? to_bf16(wt[(int64_t)(ci + t) * cout + co]) : (short)0;
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int64_t mo = m0 + 8 * i + 4 * g + j;
+            if (mo < M && co < cout) {
+                float v = acc[4 * i + j] + (bias ? bias[co] : 0.f);
+                if (relu) v = v > 0.f ? v : 0.f;
+                out[mo * cout + co] = v;
+            }
+        }
+}
+
+// weights of the data gradient: wt[tap][co][ci] = w[8 - tap][ci][co]
+__global__ void k_flip_weights(const float* __restrict__ w, float* __restrict__ wt, int cin,
+                               int cout) {
+    const int64_t total = 9LL * cin * cout;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ci = (int)(i % cin);
+        const int co = (int)((i / cin) % cout);
+        const int tap = (int)(i / ((int64_t)cin * cout));
+        wt[i] = w[((int64_t)(8 - tap) * cin + ci) * cout + co];
+    }
+}
+
+// Weight gradient partials of one pixel chunk: a wave computes a 32 (rows
+// k = tap*cin + ci) x 32 (co) tile of sum_m A[m][k] dout[m][co], reducing
+// over its pixels 8 at a time; partial[chunk][k][co].
+__global__ __launch_bounds__(64 * kWaves) void k_conv_wgrad(const float* __restrict__ in,
+                                                            const float* __restrict__ dout,
+                                                            float* __restrict__ partial,
+                                                            int n, int h, int wd, int cin,
+                                                            int cout, int chunk) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int K = 9 * cin;
+    const int k0 = blockIdx.x * 32, co0 = blockIdx.y * 32;
+    const int64_t M = (int64_t)n * h * wd;
+    const int64_t mbeg = (int64_t)blockIdx.z * chunk;
+    const int64_t mend = mbeg + chunk < M ? mbeg + chunk : M;
+    const int row = lane & 31, g = lane >> 5;
+    const int k = k0 + row;  // A row of this lane
+    const bool kvalid = k < K;
+    const int tap = kvalid ? k / cin : 0, ci = kvalid ? k % cin : 0;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int co = co0 + row;  // B column of this lane
+    f32x16 acc = {};
+    // the waves of the workgroup split the chunk in strides of 8 pixels
+    for (int64_t mb = mbeg + 8 * wave; mb < mend; mb += 8 * kWaves) {
+        bf16x4 a, b;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int64_t m = mb + 4 * g + t;
+            float av = 0.f, bv = 0.f;
+            if (m < mend) {
+                const int px = (int)(m % wd), py = (int)((m / wd) % h);
+                const int64_t pn = m / ((int64_t)wd * h);
+                const int yy = py + dy, xx = px + dx;
+                if (kvalid && yy >= 0 && yy < h && xx >= 0 && xx < wd)
+                    av = in[((pn * h + yy) * wd + xx) * cin + ci];
+                if (co < cout) bv = dout[m * cout + co];
+            }
+            a[t] = to_bf16(av);
+            b[t] = to_bf16(bv);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    // sum the waves' tiles through LDS, then write this chunk's partial
+    __shared__ float red[kWaves][32][33];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) red[wave][8 * i + 4 * g + j][row] = acc[4 * i + j];
+    __syncthreads();
+    for (int e = threadIdx.x; e < 32 * 32; e += 64 * kWaves) {
+        const int r = e / 32, cc = e % 32;
+        float s = 0.f;
+        for (int v = 0; v < kWaves; v++) s += red[v][r][cc];
+        const int kk = k0 + r, oc = co0 + cc;
+        if (kk < K && oc < cout) partial[((int64_t)blockIdx.z * K + kk) * cout + oc] = s;
+    }
+}
+
+// dw = sum over chunks (chunk order: deterministic)
+__global__ void k_sum_partials(const float* __restrict__ partial, float* __restrict__ dst,
+                               int64_t count, int nchunks) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (int c = 0; c < nchunks; c++) s += partial[(int64_t)c * count + i];
+        dst[i] = s;
+    }
+}
+
+// bias gradient partials: per chunk of pixels, per channel
+__global__ void k_bias_partial(const float* __restrict__ dout, float* __restrict__ partial,
+                               int64_t M, int cout, int chunk) {
+    const int co = threadIdx.x;
+    if (co >= cout) return;
+    const int64_t mbeg = (int64_t)blockIdx.x * chunk;
+    const int64_t mend = mbeg + chunk < M ? mbeg + chunk : M;
+    float s = 0.f;
+    for (int64_t m = mbeg; m < mend; m++) s += dout[m * cout + co];
+    partial[(int64_t)blockIdx.x * cout + co] = s;
+}
+
+__global__ void k_relu_bwd(const float* __restrict__ dout, const float* __restrict__ out,
+                           float* __restrict__ din, int64_t count) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x)
+        din[i] = out[i] > 0.f ? dout[i] : 0.f;
+}
+
+// 2x2 average pooling and its adjoint (h, wd: the fine size)
+__global__ void k_pool_fwd(const float* __restrict__ in, float* __restrict__ out, int n, int h,
+                           int wd, int c) {
+    const int ho = h / 2, wo = wd / 2;
+    const int64_t total = (int64_t)n * ho * wo * c;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % c);
+        const int64_t p = i / c;
+        const int x = (int)(p % wo), y = (int)((p / wo) % ho);
+        const int64_t b = p / ((int64_t)wo * ho);
+        const float* s = in + ((b * h + 2 * y) * wd + 2 * x) * c + ch;
+        out[i] = 0.25f * ((s[0] + s[c]) + (s[(int64_t)wd * c] + s[(int64_t)wd * c + c]));
+    }
+}
+
+__global__ void k_pool_bwd(const float* __restrict__ dout, float* __restrict__ din, int n,
+                           int h, int wd, int c) {
+    const int64_t total = (int64_t)n * h * wd * c;
+    const int ho = h / 2, wo = wd / 2;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % c);
+        const int64_t p = i / c;
+        const int x = (int)(p % wd), y = (int)((p / wd) % h);
+        const int64_t b = p / ((int64_t)wd * h);
+        din[i] = 0.25f * dout[((b * ho + y / 2) * wo + x / 2) * c + ch];
+    }
+}
+
+// nearest 2x upsampling (h, wd: the coarse size) into channels [c0, c0+c)
+// of a tensor with ctot channels, and its adjoint
+__global__ void k_up_fwd(const float* __restrict__ in, float* __restrict__ out, int n, int h,
+                         int wd, int c, int ctot, int c0) {
+    const int H = 2 * h, W = 2 * wd;
+    const int64_t total = (int64_t)n * H * W * c;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % c);
+        const int64_t p = i / c;
+        const int x = (int)(p % W), y = (int)((p / W) % H);
+        const int64_t b = p / ((int64_t)W * H);
+        out[p * ctot + c0 + ch] = in[((b * h + y / 2) * wd + x / 2) * c + ch];
+    }
+}
+
+__global__ void k_up_bwd(const float* __restrict__ dout, float* __restrict__ din, int n, int h,
+                         int wd, int c, int ctot, int c0) {
+    const int W = 2 * wd;
+    const int64_t total = (int64_t)n * h * wd * c;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % c);
+        const int64_t p = i / c;
+        const int x = (int)(p % wd), y = (int)((p / wd) % h);
+        const int64_t b = p / ((int64_t)wd * h);
+        const int64_t f = ((b * 2 * h + 2 * y) * W + 2 * x) * ctot + c0 + ch;
+        const int64_t r = (int64_t)W * ctot;
+        din[i] = (dout[f] + dout[f + ctot]) + (dout[f + r] + dout[f + r + ctot]);
+    }
+}
+
+// copy channels [0, c) of src (c channels) into channels [c0, c0 + c) of
+// dst (ctot channels) (dir = 0), or back (dir = 1), or add them back (dir = 2)
+__global__ void k_channels(const float* __restrict__ a, float* __restrict__ b, int64_t pixels,
+                           int c, int ctot, int c0, int dir) {
+    const int64_t total = pixels * c;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % c);
+        const int64_t p = i / c;
+        if (dir == 0) b[p * ctot + c0 + ch] = a[i];
+        else if (dir == 1) b[i] = a[p * ctot + c0 + ch];
+        else b[i] += a[p * ctot + c0 + ch];
+    }
+}
+
+// masked mean squared error over channels [0, cu) of pred (ctot channels):
+// per-block partial sums of the loss; grad = 2 (pred - target) / count on
+// fluid pixels (mask == 0), zero elsewhere and in channels >= cu
+__global__ void k_mse(const float* __restrict__ pred, const float* __restrict__ target,
+                      const float* __restrict__ mask, float* __restrict__ grad,
+                      float* __restrict__ partial, int64_t pixels, int cu, int ctot,
+                      float inv_count) {
+    __shared__ float red[256];
+    float s = 0.f;
+    const int64_t total = pixels * ctot;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % ctot);
+        const int64_t p = i / ctot;
+        float gv = 0.f;
+        if (ch < cu && mask[p] == 0.f) {
+            const float d = pred[i] - target[p * cu + ch];
+            s += d * d;
+            gv = 2.f * d * inv_count;
+        }
+        grad[i] = gv;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Adam (Kingma & Ba 2015) with bias correction folded into the step size
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int64_t count, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+    }
+}
+
+__global__ void k_zero(float* __restrict__ p, int64_t count) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.f;
+}
+
+int grid_for(int64_t total) {
+    int64_t g = (total + 255) / 256;
+    if (g > 8192) g = 8192;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+int status() { return hipGetLastError() == hipSuccess ? 0 : MPLBWS_ERR_HIP; }
+
+}  // namespace
+
+extern "C" {
+
+int mplbws_conv3x3_fwd(const float* in, const float* w, const float* bias, float* out, int n,
+                       int h, int wd, int cin, int cout, int relu) {
+    if (!in || !w || !out || n <= 0 || h <= 0 || wd <= 0 || cin <= 0 || cout <= 0 ||
+        cin % 8 != 0)
+        return MPLBWS_ERR_ARG;
+    const int64_t M = (int64_t)n * h * wd;
+    const dim3 grd((unsigned)((M + 32 * kWaves - 1) / (32 * kWaves)), (unsigned)((cout + 31) / 32));
+    hipLaunchKernelGGL(k_conv_fwd, grd, dim3(64 * kWaves), 0, 0, in, w, bias, out, n, h, wd, cin,
+                       cout, relu);
+    return status();
+}
+
+int mplbws_flip_weights(const float* w, float* wt, int cin, int cout) {
+    if (!w || !wt || cin <= 0 || cout <= 0) return MPLBWS_ERR_ARG;
+    hipLaunchKernelGGL(k_flip_weights, dim3(grid_for(9LL * cin * cout)), dim3(256), 0, 0, w, wt,
+                       cin, cout);
+    return status();
+}
+
+int64_t mplbws_wgrad_workspace(int n, int h, int wd, int cin, int cout) {
+    const int64_t M = (int64_t)n * h * wd;
+    const int64_t chunks = (M + MPLBWS_WGRAD_CHUNK - 1) / MPLBWS_WGRAD_CHUNK;
+    return chunks * 9LL * cin * cout + chunks * cout;
+}
+
+int mplbws_conv3x3_wgrad(const float* in, const float* dout, float* dw, float* dbias,
+                         float* workspace, int n, int h, int wd, int cin, int cout) {
+    if (!in || !dout || !dw || !workspace || n <= 0 || h <= 0 || wd <= 0 || cin <= 0 ||
+        cout <= 0)
+        return MPLBWS_ERR_ARG;
+    const int64_t M = (int64_t)n * h * wd;
+    const int chunks = (int)((M + MPLBWS_WGRAD_CHUNK - 1) / MPLBWS_WGRAD_CHUNK);
+    const int K = 9 * cin;
+    const dim3 grd((unsigned)((K + 31) / 32), (unsigned)((cout + 31) / 32), (unsigned)chunks);
+    hipLaunchKernelGGL(k_conv_wgrad, grd, dim3(64 * kWaves), 0, 0, in, dout, workspace, n, h, wd,
+                       cin, cout, MPLBWS_WGRAD_CHUNK);
+    const int64_t cnt = (int64_t)K * cout;
+    hipLaunchKernelGGL(k_sum_partials, dim3(grid_for(cnt)), dim3(256), 0, 0, workspace, dw, cnt,
+                       chunks);
+    if (dbias) {
+        float* bp = workspace + (int64_t)chunks * cnt;
+        if (cout > 256) return MPLBWS_ERR_ARG;
+        hipLaunchKernelGGL(k_bias_partial, dim3(chunks), dim3(256), 0, 0, dout, bp, M, cout,
+                           MPLBWS_WGRAD_CHUNK);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, 0, bp, dbias, (int64_t)cout,
+                           chunks);
+    }
+    return status();
+}
+
+int mplbws_relu_bwd(const float* dout, const float* out, float* din, int64_t count) {
+    if (!dout || !out || !din || count < 0) return MPLBWS_ERR_ARG;
+    hipLaunchKernelGGL(k_relu_bwd, dim3(grid_for(count)), dim3(256), 0, 0, dout, out, din, count);
+    return status();
+}
+
+int mplbws_pool2_fwd(const float* in, float* out, int n, int h, int wd, int c) {
+    if (!in || !out || h % 2 || wd % 2) return MPLBWS_ERR_ARG;
+    hipLaunchKernelGGL(k_pool_fwd, dim3(grid_for((int64_t)n * h * wd * c / 4)), dim3(256), 0, 0,
+                       in, out, n, h, wd, c);
+    return status();
+}
+
+int mplbws_pool2_bwd(const float* dout, float* din, int n, int h, int wd, int c) {
+    if (!dout || !din || h % 2 || wd % 2) return MPLBWS_ERR_ARG;
+    hipLaunchKernelGGL(k_pool_bwd, dim3(grid_for((int64_t)n * h * wd * c)), dim3(256), 0, 0, dout,
+                       din, n, h, wd, c);
+    return status();
+}
+
+int mplbws_up2_fwd(const float* in, float* out, int n, int h, int wd, int c, int ctot, int c0) {
+    if (!in || !out || c0 < 0 || c0 + c > ctot) return MPLBWS_ERR_ARG;
+    hipLaunchKernelGGL(k_up_fwd, dim3(grid_for((int64_t)n * 4 * h * wd * c)), dim3(256), 0, 0, in,
+                       out, n, h, wd, c, ctot, c0);
+    return status();
+}
+
+int mplbws_up2_bwd(const float* dout, float* din, int n, int h, int wd, int c, int ctot, int c0) {
+    if (!dout || !din || c0 < 0 || c0 + c > ctot) return MPLBWS_ERR_ARG;
+    hipLaunchKernelGGL(k_up_bwd, dim3(grid_for((int64_t)n * h * wd * c)), dim3(256), 0, 0, dout,
+                       din, n, h, wd, c, ctot, c0);
+    return status();
+}
+
+int mplbws_channels(const float* src, float* dst, int64_t pixels, int c, int ctot, int c0,
+                    int direction) {
+    if (!src || !dst || c0 < 0 || c0 + c > ctot) return MPLBWS_ERR_ARG;
+    hipLaunchKernelGGL(k_channels, dim3(grid_for(pixels * c)), dim3(256), 0, 0, src, dst, pixels,
+                       c, ctot, c0, direction);
+    return status();
+}
+
+int mplbws_mse(const float* pred, const float* target, const float* mask, float* grad,
+               float* partial, int npartial, int64_t pixels, int cu, int ctot, float inv_count) {
+    if (!pred || !target || !mask || !grad || !partial || npartial <= 0 || cu > ctot)
+        return MPLBWS_ERR_ARG;
+    hipLaunchKernelGGL(k_mse, dim3(npartial), dim3(256), 0, 0, pred, target, mask, grad, partial,
+                       pixels, cu, ctot, inv_count);
+    return status();
+}
+
+int mplbws_adam(float* p, const float* g, float* m, float* v, int64_t count, float lr, float b1,
+                float b2, float eps, int step) {
+    if (!p || !g || !m || !v || count < 0 || step < 1) return MPLBWS_ERR_ARG;
+    const float c1 = 1.f / (1.f - powf(b1, (float)step));
+    const float c2 = 1.f / (1.f - powf(b2, (float)step));
+    hipLaunchKernelGGL(k_adam, dim3(grid_for(count)), dim3(256), 0, 0, p, g, m, v, count, lr, b1,
+                       b2, eps, c1, c2);
+    return status();
+}
+
+int mplbws_sync(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : MPLBWS_ERR_HIP; }
+
+int mplbws_alloc(void** p, int64_t bytes) {
+    if (!p || bytes < 0) return MPLBWS_ERR_ARG;
+    *p = nullptr;
+    if (bytes == 0) return 0;
+    return hipMalloc(p, (size_t)bytes) == hipSuccess ? 0 : MPLBWS_ERR_HIP;
+}
+
+int mplbws_free(void* p) {
+    if (!p) return 0;
+    return hipFree(p) == hipSuccess ? 0 : MPLBWS_ERR_HIP;
+}
+
+int mplbws_upload(void* dst, const void* src, int64_t bytes) {
+    if (!dst || !src || bytes < 0) return MPLBWS_ERR_ARG;
+    return hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice) == hipSuccess
+               ? 0
+               : MPLBWS_ERR_HIP;
+}
+
+int mplbws_download(void* dst, const void* src, int64_t bytes) {
+    if (!dst || !src || bytes < 0) return MPLBWS_ERR_ARG;
+    return hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost) == hipSuccess
+               ? 0
+               : MPLBWS_ERR_HIP;
+}
+
+int mplbws_zero(void* p, int64_t bytes) {
+    if (!p || bytes < 0 || bytes % 4) return MPLBWS_ERR_ARG;
+    hipLaunchKernelGGL(k_zero, dim3(grid_for(bytes / 4)), dim3(256), 0, 0,
+                       static_cast<float*>(p), bytes / 4);
+    return status();
+}
+
+}  // extern "C"

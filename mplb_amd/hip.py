@@ -62,6 +62,7 @@ def load():
         "mplbhip_node_info": (I, [P, I, I, IP]),
         "mplbhip_set_macros": (I, [P, I, DP, DP, DP, DP]),
         "mplbhip_init_feq": (I, [P]),
+        "mplbhip_init_lift": (I, [P]),
         "mplbhip_set_f": (I, [P, I, DP]),
         "mplbhip_set_timestep": (I, [P, D]),
         "mplbhip_set_accel": (I, [P, I, DP]),
@@ -186,6 +187,11 @@ class LatticeCase:
 
     def init_feq(self):
         self._ck(self.lib.mplbhip_init_feq(self.h))
+
+    def init_lift(self):
+        """feq of the uploaded macros plus the first-order non-equilibrium
+        part (warm start from a predicted flow)."""
+        self._ck(self.lib.mplbhip_init_lift(self.h))
 
     def set_f(self, block, f):
         f = np.ascontiguousarray(np.asarray(f, dtype=np.float64))
