@@ -527,11 +527,17 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
         active = x < B.n[0] && y < B.n[1] && B.active;
     } else {
         wg = blockIdx.x;
-        b = find_block<&BlockParams::bnd_wg_start>(kp, wg);
+        const int v = kp.fold;
+        int lo = 0, hi = kp.nblocks - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (kp.blocks[mid].bnd_wg_start[v] <= wg) lo = mid; else hi = mid - 1;
+        }
+        b = lo;
         const BlockParams& B = kp.blocks[b];
-        const int i = (wg - B.bnd_wg_start) * kBndWG + threadIdx.y * kTX + threadIdx.x;
-        int s = B.seg_start;
-        const int send = B.seg_start + B.seg_count;
+        const int i = (wg - B.bnd_wg_start[v]) * kBndWG + threadIdx.y * kTX + threadIdx.x;
+        int s = B.seg_start[v];
+        const int send = B.seg_start[v] + B.seg_count[v];
         while (s < send && i >= kp.segs[s].start + kp.segs[s].count) s++;
         active = s < send && B.active;
         x = y = z = 0;
@@ -846,7 +852,7 @@ __global__ void k_halo(HaloParams hp) {
 // a whole-lattice launch, its tiles.
 __global__ void k_reduce_partials(const double* partials, const BlockParams* blocks,
                                   int nblocks, int ntiles, int nbulk, int nbnd, int split,
-                                  double* out) {
+                                  int fold, double* out) {
     const int a = threadIdx.x;
     if (a >= kNAcc) return;
     double total = 0.0;
@@ -857,8 +863,8 @@ __global__ void k_reduce_partials(const double* partials, const BlockParams* blo
             const int t0 = blocks[b].bulk_tile_start;
             const int t1 = last ? nbulk : blocks[b + 1].bulk_tile_start;
             for (int t = t0; t < t1; t++) s += partials[(int64_t)t * kNAcc + a];
-            const int w0 = blocks[b].bnd_wg_start;
-            const int w1 = last ? nbnd : blocks[b + 1].bnd_wg_start;
+            const int w0 = blocks[b].bnd_wg_start[fold];
+            const int w1 = last ? nbnd : blocks[b + 1].bnd_wg_start[fold];
             for (int w = w0; w < w1; w++) s += partials[(int64_t)(nbulk + w) * kNAcc + a];
         } else {
             const int t0 = blocks[b].tile_start;
@@ -949,10 +955,10 @@ hipError_t launch_halo(const HaloParams& hp, hipStream_t s) {
 }
 
 hipError_t launch_reduce(const double* partials, const BlockParams* blocks, int nblocks,
-                         int ntiles, int nbulk, int nbnd, int split, double* out,
+                         int ntiles, int nbulk, int nbnd, int split, int fold, double* out,
                          hipStream_t s) {
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partials, blocks,
-                       nblocks, ntiles, nbulk, nbnd, split, out);
+                       nblocks, ntiles, nbulk, nbnd, split, fold, out);
     return hipGetLastError();
 }
 

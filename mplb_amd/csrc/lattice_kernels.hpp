@@ -36,8 +36,10 @@ struct BlockParams {
     // n-2 in 3D) in tiles_x * tiles_y tiles per plane; the boundary kernel
     // walks the block's boundary segments, 256 nodes per workgroup
     int32_t bulk_tile_start;  // first bulk workgroup of this block
-    int32_t bnd_wg_start;     // first boundary workgroup of this block
-    int32_t seg_start, seg_count;  // boundary segments of this block
+    // boundary workgroups and segments, [fold]: 0 = every boundary region,
+    // 1 = without the x-face regions the bulk kernel runs inline
+    int32_t bnd_wg_start[2];
+    int32_t seg_start[2], seg_count[2];
     int32_t has_obst;         // the block has obstacle link masks
 };
 

@@ -22,7 +22,7 @@ hipError_t launch_lattice(int dim, int storage, int mode, const KParams& kp,
                           hipStream_t s, int part = 0);
 hipError_t launch_halo(const HaloParams& hp, hipStream_t s);
 hipError_t launch_reduce(const double* partials, const BlockParams* blocks,
-                         int nblocks, int ntiles, int nbulk, int nbnd, int split,
+                         int nblocks, int ntiles, int nbulk, int nbnd, int split, int fold,
                          double* out, hipStream_t s);
 hipError_t launch_fill(void* p, int64_t bytes, hipStream_t s);
 hipError_t launch_lift(int dim, const KParams& kp, int64_t nodes, hipStream_t s);
@@ -70,10 +70,14 @@ struct mplbhip_ctx {
     int64_t total_elems = 0;
     int64_t interior_nodes = 0;
     int ntiles = 0;            // whole-lattice tiles
-    int nbulk = 0, nbnd = 0;   // bulk tiles and boundary workgroups of a step
+    int nbulk = 0;             // bulk tiles of a step
+    int nbnd_v[2] = {0, 0};    // boundary workgroups of a step, [fold]
+    bool tuned = false;        // launch configuration chosen (autotune)
     bool fold = true;          // simple x-face regions run inside the bulk kernel
     bool xcd_remap = false;    // bulk tiles dealt to XCDs in contiguous runs
     int xgroup = 0;            // tiles per row kept on one XCD (0: launch order)
+    int xgroup_ok = 0;         // the row grouping this case allows (0: none)
+    bool fold_env = false, xgroup_env = false;  // fixed by the environment
     std::vector<BndSeg> segs;
     std::vector<std::vector<uint8_t>> solid;  // obstacle nodes per block (empty: none)
     std::vector<uint32_t> obst;               // link masks per interior node
@@ -188,7 +192,7 @@ static KParams base_params(mplbhip_ctx* c) {
     kp.nblocks = (int)c->bp.size();
     kp.ntiles = c->ntiles;
     kp.nbulk = c->nbulk;
-    kp.nbnd = c->nbnd;
+    kp.nbnd = c->nbnd_v[c->fold ? 1 : 0];
     kp.fold = c->fold ? 1 : 0;
     kp.xcd_remap = c->xcd_remap ? 1 : 0;
     kp.xgroup = c->xgroup;
@@ -308,7 +312,7 @@ static int materialize(mplbhip_ctx* c, bool want_f, bool residual,
     if (acc_out) {
         HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
                                 static_cast<const BlockParams*>(c->blocks.p),
-                                (int)c->bp.size(), c->ntiles, c->nbulk, c->nbnd, 0,
+                                (int)c->bp.size(), c->ntiles, c->nbulk, 0, 0, 0,
                                 static_cast<double*>(c->red.p), c->stream));
         acc_out->resize((c->bp.size() + 1) * kNAcc);
         HIPCHK(c, hipMemcpyAsync(acc_out->data(), c->red.p,
@@ -742,12 +746,15 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     const int align = (int)(128 / c->es);
     c->bp.assign(nb, BlockParams{});
     int64_t off = 0, moff = 0;
-    int tiles = 0, bulk = 0, bnd = 0;
+    int tiles = 0, bulk = 0, bnd_v[2] = {0, 0};
     c->segs.clear();
     // measured at 256^3: folding pays for 4- and 8-byte populations; with
     // 2-byte ones the extra lanes cost more than the strided boundary loads
     c->fold = c->storage != MPLBHIP_STORE_F16;
-    if (const char* e = std::getenv("MPLBHIP_XFACE_FOLD")) c->fold = std::atoi(e) != 0;
+    if (const char* e = std::getenv("MPLBHIP_XFACE_FOLD")) {
+        c->fold = std::atoi(e) != 0;
+        c->fold_env = true;
+    }
     if (const char* e = std::getenv("MPLBHIP_XCD_REMAP")) c->xcd_remap = std::atoi(e) != 0;
     for (int b = 0; b < nb; b++) {
         const int* n = c->plan.blocks[b].n;
@@ -780,33 +787,36 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         // boundary regions as segments, 256 nodes per workgroup
         B.bulk_tile_start = bulk;
         bulk += B.tiles_x * B.tiles_y * (c->dim == 3 ? n[2] - 2 : 1);
-        B.seg_start = (int)c->segs.size();
-        int count = 0;
-        for (int r = 1; r < c->plan.nregions; r++) {
-            const int code[3] = {r % 3, (r / 3) % 3, r / 9};
-            // x-face interiors with simple programs run inside the bulk
-            // kernel, whose waves already load their rows
-            if (c->fold && r < 3 && c->plan.progs[(size_t)b * c->plan.nregions + r].simple)
-                continue;
-            BndSeg S;
-            for (int a = 0; a < 3; a++) {
-                if (a == 2 && c->dim == 2) {
-                    S.org[a] = 0;
-                    S.ext[a] = 1;
-                } else {
-                    S.org[a] = code[a] == 0 ? 1 : code[a] == 1 ? 0 : n[a] - 1;
-                    S.ext[a] = code[a] == 0 ? n[a] - 2 : 1;
+        // two segment lists: [0] every boundary region, [1] without the
+        // x-face interiors whose programs are simple (the bulk kernel
+        // runs those inline when folding)
+        for (int v = 0; v < 2; v++) {
+            B.seg_start[v] = (int)c->segs.size();
+            int count = 0;
+            for (int r = 1; r < c->plan.nregions; r++) {
+                const int code[3] = {r % 3, (r / 3) % 3, r / 9};
+                if (v == 1 && r < 3 && c->plan.progs[(size_t)b * c->plan.nregions + r].simple)
+                    continue;
+                BndSeg S;
+                for (int a = 0; a < 3; a++) {
+                    if (a == 2 && c->dim == 2) {
+                        S.org[a] = 0;
+                        S.ext[a] = 1;
+                    } else {
+                        S.org[a] = code[a] == 0 ? 1 : code[a] == 1 ? 0 : n[a] - 1;
+                        S.ext[a] = code[a] == 0 ? n[a] - 2 : 1;
+                    }
                 }
+                S.start = count;
+                S.count = S.ext[0] * S.ext[1] * S.ext[2];
+                if (S.count <= 0) continue;
+                count += S.count;
+                c->segs.push_back(S);
             }
-            S.start = count;
-            S.count = S.ext[0] * S.ext[1] * S.ext[2];
-            if (S.count <= 0) continue;
-            count += S.count;
-            c->segs.push_back(S);
+            B.seg_count[v] = (int)c->segs.size() - B.seg_start[v];
+            B.bnd_wg_start[v] = bnd_v[v];
+            bnd_v[v] += (count + kBndWG - 1) / kBndWG;
         }
-        B.seg_count = (int)c->segs.size() - B.seg_start;
-        B.bnd_wg_start = bnd;
-        bnd += (count + kBndWG - 1) / kBndWG;
         B.prog_base = b * c->plan.nregions;
         B.mbase = moff;
         B.fbase = moff * c->Q;
@@ -817,7 +827,8 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     c->interior_nodes = moff;
     c->ntiles = tiles;
     c->nbulk = bulk;
-    c->nbnd = bnd;
+    c->nbnd_v[0] = bnd_v[0];
+    c->nbnd_v[1] = bnd_v[1];
     // the x tiles of a row share their edge cache lines: deal each row's
     // tiles to one XCD (hardware hands workgroups to the 8 XCDs round
     // robin) when every block has the same row of tiles
@@ -825,8 +836,10 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     for (int b = 0; b < nb; b++)
         if (c->bp[b].tiles_x != c->xgroup) c->xgroup = 0;
     if (c->xgroup > 0 && bulk % (8 * c->xgroup) != 0) c->xgroup = 0;
+    c->xgroup_ok = c->xgroup;
     if (const char* e = std::getenv("MPLBHIP_XGROUP")) {
         if (std::atoi(e) == 0) c->xgroup = 0;
+        c->xgroup_env = true;
     }
     if (int rc = setup_halos(c)) return rc;
     if (int rc = build_obstacles(c)) return rc;
@@ -859,7 +872,7 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         HIPCHK(c, hipMemcpyAsync(c->dobst.p, c->obst.data(), c->dobst.bytes,
                                  hipMemcpyHostToDevice, c->stream));
     }
-    const size_t nslots = (size_t)std::max(tiles, bulk + bnd);
+    const size_t nslots = (size_t)std::max(tiles, bulk + std::max(bnd_v[0], bnd_v[1]));
     if (int rc = dalloc(c, c->partials, nslots * kNAcc * sizeof(double))) return rc;
     if (int rc = dalloc(c, c->dsegs, std::max<size_t>(1, c->segs.size()) * sizeof(BndSeg)))
         return rc;
@@ -984,6 +997,54 @@ int mplbhip_set_accel(mplbhip_ctx* c, int block, const double* g) {
     return 0;
 }
 
+// Launch-configuration autotuning, once per context on its first step:
+// folding the simple x faces into the bulk kernel or not, and keeping a
+// row's tiles on one XCD or not.  Each candidate runs the step kernels a
+// few times from the current state into the other buffer; those writes are
+// what the next real step writes again, so the state is unchanged.  Cases
+// under 2^20 nodes keep the defaults (their steps are launch-bound).
+static int autotune(mplbhip_ctx* c) {
+    if (c->tuned) return 0;
+    c->tuned = true;
+    if (c->interior_nodes < (1 << 20)) return 0;
+    std::vector<std::pair<bool, int>> cand;
+    std::vector<bool> folds;
+    if (c->fold_env || c->nbnd_v[0] == c->nbnd_v[1]) folds = {c->fold};
+    else folds = {true, false};
+    std::vector<int> groups;
+    if (c->xgroup_env || c->xgroup_ok == 0) groups = {c->xgroup};
+    else groups = {c->xgroup_ok, 0};
+    for (bool f : folds)
+        for (int g : groups) cand.push_back({f, g});
+    if (cand.size() < 2) return 0;
+    float best = 1e30f;
+    std::pair<bool, int> pick = cand[0];
+    for (const auto& cf : cand) {
+        c->fold = cf.first;
+        c->xgroup = cf.second;
+        KParams kp = base_params(c);
+        HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP, kp, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        for (int i = 0; i < 3; i++)
+            HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP, kp, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev[1]));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        if (ms < best) {
+            best = ms;
+            pick = cf;
+        }
+    }
+    c->fold = pick.first;
+    c->xgroup = pick.second;
+    drop_graphs(c);
+    if (std::getenv("MPLBHIP_VERBOSE"))
+        std::fprintf(stderr, "mplbhip: autotune picked fold=%d xgroup=%d (%.4f ms/step)\n",
+                     (int)c->fold, c->xgroup, best / 3);
+    return 0;
+}
+
 // the collision of f^0 happens here, with the time step then in force
 static int first_collision(mplbhip_ctx* c) {
     // every block starts from buffer 0
@@ -1010,7 +1071,7 @@ static int first_collision(mplbhip_ctx* c) {
     }
     HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->stream));
     if (int rc = exchange(c, 0)) return rc;
-    return 0;
+    return autotune(c);
 }
 
 static int run_steps(mplbhip_ctx* c, int nsteps, bool timed, double* ms_total,
@@ -1110,7 +1171,8 @@ int mplbhip_iterate(mplbhip_ctx* c, double criteria, int check_period,
             double acc[kNAcc];
             HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
                                     static_cast<const BlockParams*>(c->blocks.p),
-                                    (int)c->bp.size(), c->ntiles, c->nbulk, c->nbnd, 1,
+                                    (int)c->bp.size(), c->ntiles, c->nbulk,
+                                    c->nbnd_v[c->fold ? 1 : 0], 1, c->fold ? 1 : 0,
                                     static_cast<double*>(c->red.p), c->stream));
             HIPCHK(c, hipMemcpyAsync(acc, static_cast<double*>(c->red.p) +
                                               c->bp.size() * kNAcc,
@@ -1185,7 +1247,8 @@ int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
         done++;
         HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
                                 static_cast<const BlockParams*>(c->blocks.p), nb,
-                                c->ntiles, c->nbulk, c->nbnd, 1,
+                                c->ntiles, c->nbulk, c->nbnd_v[c->fold ? 1 : 0], 1,
+                                c->fold ? 1 : 0,
                                 static_cast<double*>(c->red.p), c->stream));
         HIPCHK(c, hipMemcpyAsync(acc.data(), c->red.p, acc.size() * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));
