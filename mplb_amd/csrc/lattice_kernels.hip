@@ -681,11 +681,13 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
     using T = typename Store<ST>::T;
     using E = typename Store<ST>::E;
     constexpr int Q = D == 2 ? 9 : 19;
-    const int wg = kp.xgroup ? row_group_wg(blockIdx.x, kp.xgroup)
-                   : kp.xcd_remap ? logical_wg(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int wg = kp.xcd_remap ? logical_wg(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int b = find_block<&BlockParams::bulk_tile_start>(kp, wg);
     const BlockParams& B = kp.blocks[b];
     int t = wg - B.bulk_tile_start;
+    // a block whose first tile sits on XCD 0 and whose tile count is a
+    // multiple of 8 rows keeps each row's tiles on one XCD
+    if (kp.xgroup && B.xgroup) t = row_group_wg(t, B.xgroup);
     const bool first = t == 0;
     const int tx = t % B.tiles_x;
     t /= B.tiles_x;

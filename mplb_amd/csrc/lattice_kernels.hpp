@@ -40,6 +40,9 @@ struct BlockParams {
     // 1 = without the x-face regions the bulk kernel runs inline
     int32_t bnd_wg_start[2];
     int32_t seg_start[2], seg_count[2];
+    int32_t xgroup;           // tiles_x when this block's bulk tiles can be
+                              // dealt a row per XCD, else 0
+    int32_t pad2;
     int32_t has_obst;         // the block has obstacle link masks
 };
 
@@ -97,7 +100,7 @@ struct KParams {
     const BndSeg* segs;     // boundary segments, blocks in order
     int32_t fold;           // the bulk kernel runs simple x-face regions
     int32_t xcd_remap;      // bulk tiles dealt to XCDs in contiguous runs
-    int32_t xgroup;         // >0: the xgroup tiles of a row run on one XCD
+    int32_t xgroup;         // 1: deal each block's rows of tiles to one XCD
     int32_t nbulk;          // bulk workgroups (partials 0 .. nbulk-1)
     int32_t nbnd;           // boundary workgroups (partials nbulk ..)
 };

@@ -75,8 +75,8 @@ struct mplbhip_ctx {
     bool tuned = false;        // launch configuration chosen (autotune)
     bool fold = true;          // simple x-face regions run inside the bulk kernel
     bool xcd_remap = false;    // bulk tiles dealt to XCDs in contiguous runs
-    int xgroup = 0;            // tiles per row kept on one XCD (0: launch order)
-    int xgroup_ok = 0;         // the row grouping this case allows (0: none)
+    int xgroup = 0;            // 1: rows of bulk tiles dealt to one XCD each
+    int xgroup_ok = 0;         // 1 when some block allows that grouping
     bool fold_env = false, xgroup_env = false;  // fixed by the environment
     std::vector<BndSeg> segs;
     std::vector<std::vector<uint8_t>> solid;  // obstacle nodes per block (empty: none)
@@ -832,10 +832,16 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     // the x tiles of a row share their edge cache lines: deal each row's
     // tiles to one XCD (hardware hands workgroups to the 8 XCDs round
     // robin) when every block has the same row of tiles
-    c->xgroup = c->bp[0].tiles_x;
-    for (int b = 0; b < nb; b++)
-        if (c->bp[b].tiles_x != c->xgroup) c->xgroup = 0;
-    if (c->xgroup > 0 && bulk % (8 * c->xgroup) != 0) c->xgroup = 0;
+    int grouped = 0;
+    for (int b = 0; b < nb; b++) {
+        BlockParams& B = c->bp[b];
+        const int next = b + 1 < nb ? c->bp[b + 1].bulk_tile_start : bulk;
+        const int cnt = next - B.bulk_tile_start;
+        const bool ok = B.bulk_tile_start % 8 == 0 && cnt % (8 * B.tiles_x) == 0 && !c->xcd_remap;
+        B.xgroup = ok ? B.tiles_x : 0;
+        grouped += ok ? 1 : 0;
+    }
+    c->xgroup = grouped > 0 ? 1 : 0;
     c->xgroup_ok = c->xgroup;
     if (const char* e = std::getenv("MPLBHIP_XGROUP")) {
         if (std::atoi(e) == 0) c->xgroup = 0;
