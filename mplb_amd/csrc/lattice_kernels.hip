@@ -502,7 +502,8 @@ __device__ __forceinline__ void finish_node(const KParams& kp, const BlockParams
 // ---- the boundary / whole-lattice kernel --------------------------------------
 // SCOPE_ALL: every node of every block, tiles of kTX x kTY nodes per z plane
 // (initialisation and MATERIALIZE).  SCOPE_BND: the boundary segments only
-// (the step's second kernel; the bulk kernel covers region 0).
+// (the step's second kernel; the bulk kernel covers region 0).  Small
+// cases run the whole step as one SCOPE_ALL launch (launch-bound there).
 enum Scope : int { SCOPE_ALL = 0, SCOPE_BND = 1 };
 
 template <int D, int ST, int MODE, int SCOPE>
@@ -525,6 +526,10 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
         x = tx * kTX + threadIdx.x;
         y = ty * kTY + threadIdx.y;
         active = x < B.n[0] && y < B.n[1] && B.active;
+        if constexpr (MODE == MODE_STEP || MODE == MODE_STEP_CHECK) {
+            if (wg == B.tile_start && threadIdx.x == 0 && threadIdx.y == 0)
+                kp.cur_out[b] = B.active ? 1 - kp.cur_in[b] : kp.cur_in[b];
+        }
     } else {
         wg = blockIdx.x;
         const int v = kp.fold;
@@ -672,10 +677,13 @@ reduce:
 
 // ---- the bulk kernel ------------------------------------------------------------
 // Region 0 of every block: Fluid nodes whose Q populations all stream in
-// (KerStream3D Fluid branch), no boundary program, so the register budget
-// is the bulk's alone.  Tiles of kTX x kTY nodes, z planes 1 .. n-2 (3D).
-// The block's first bulk workgroup also records the block's parity after
-// the step.
+// (KerStream3D Fluid branch).  With FOLD it also runs the simple boundary
+// regions (masked pull + EQM, nothing persisted) of the planes it covers,
+// whose rows share cache lines with the interior; the other boundary
+// regions stay in the boundary kernel, which keeps this one's register
+// budget small.  Tiles of kTX x kTY nodes, z planes 1 .. n-2 (3D).  The
+// block's first bulk workgroup also records the block's parity after the
+// step.
 template <int D, int ST, int MODE, bool OBST, bool FOLD>
 __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
     using T = typename Store<ST>::T;
@@ -700,14 +708,13 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
     double acc[kNAcc];
 #pragma unroll
     for (int a = 0; a < kNAcc; a++) acc[a] = 0.0;
-    // lanes: interior nodes, and the x-face nodes of rows y = 1 .. n-2 when
-    // their region program is simple (masked pull + EQM): their loads share
-    // the wave's cache lines, which a separate kernel would fetch again
-    const bool yin = y >= 1 && y < B.n[1] - 1;
-    const bool interior = x >= 1 && x < B.n[0] - 1 && yin;
+    // lanes: interior nodes, and (FOLD) the face and edge nodes of this
+    // plane whose region program is simple (masked pull + EQM): their loads
+    // share the wave's cache lines, which a separate kernel would fetch again
+    const bool interior = x >= 1 && x < B.n[0] - 1 && y >= 1 && y < B.n[1] - 1;
     const RegionProg* P = nullptr;
-    if (FOLD && !interior && yin && (x == 0 || x == B.n[0] - 1)) {
-        P = &kp.progs[B.prog_base + (x == 0 ? 1 : 2)];
+    if (FOLD && !interior && x < B.n[0] && y < B.n[1]) {
+        P = &kp.progs[B.prog_base + region_code(x, B.n[0]) + 3 * region_code(y, B.n[1])];
         if (!P->simple) P = nullptr;
     }
     const bool active = B.active && (interior || P);
@@ -891,6 +898,12 @@ __global__ void k_fill_bytes(uint32_t* p, int64_t n, uint32_t v) {
 template <int D, int ST, int MODE>
 static void launch_step(const KParams& kp, int part, hipStream_t s) {
     dim3 blk(kTX, kTY, 1);
+    if (kp.fused) {
+        if (part != 2)
+            hipLaunchKernelGGL((k_lattice<D, ST, MODE, SCOPE_ALL>), dim3(kp.ntiles), blk, 0, s,
+                               kp);
+        return;
+    }
     if (part != 2 && kp.nbulk > 0) {
         const dim3 g(kp.nbulk);
         if (kp.obst) {

@@ -98,9 +98,10 @@ struct KParams {
     int32_t want_residual;  // accumulate the residual against MacroVarsCopy
     const uint32_t* obst;   // obstacle link masks per interior node, or null
     const BndSeg* segs;     // boundary segments, blocks in order
-    int32_t fold;           // the bulk kernel runs simple x-face regions
+    int32_t fold;           // the bulk kernel runs the simple boundary regions of its grid
     int32_t xcd_remap;      // bulk tiles dealt to XCDs in contiguous runs
     int32_t xgroup;         // 1: deal each block's rows of tiles to one XCD
+    int32_t fused;          // 1: the step is one whole-lattice launch
     int32_t nbulk;          // bulk workgroups (partials 0 .. nbulk-1)
     int32_t nbnd;           // boundary workgroups (partials nbulk ..)
 };

@@ -73,6 +73,7 @@ struct mplbhip_ctx {
     int nbulk = 0;             // bulk tiles of a step
     int nbnd_v[2] = {0, 0};    // boundary workgroups of a step, [fold]
     bool tuned = false;        // launch configuration chosen (autotune)
+    bool fused = false;        // small case: one whole-lattice launch per step
     bool fold = true;          // simple x-face regions run inside the bulk kernel
     bool xcd_remap = false;    // bulk tiles dealt to XCDs in contiguous runs
     int xgroup = 0;            // 1: rows of bulk tiles dealt to one XCD each
@@ -196,6 +197,7 @@ static KParams base_params(mplbhip_ctx* c) {
     kp.fold = c->fold ? 1 : 0;
     kp.xcd_remap = c->xcd_remap ? 1 : 0;
     kp.xgroup = c->xgroup;
+    kp.fused = c->fused ? 1 : 0;
     kp.segs = static_cast<const BndSeg*>(c->dsegs.p);
     kp.obst = c->obst.empty() ? nullptr : static_cast<const uint32_t*>(c->dobst.p);
     kp.progs = static_cast<const RegionProg*>(c->progs.p);
@@ -788,14 +790,16 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         B.bulk_tile_start = bulk;
         bulk += B.tiles_x * B.tiles_y * (c->dim == 3 ? n[2] - 2 : 1);
         // two segment lists: [0] every boundary region, [1] without the
-        // x-face interiors whose programs are simple (the bulk kernel
-        // runs those inline when folding)
+        // simple regions the bulk kernel runs inline when folding
         for (int v = 0; v < 2; v++) {
             B.seg_start[v] = (int)c->segs.size();
             int count = 0;
             for (int r = 1; r < c->plan.nregions; r++) {
                 const int code[3] = {r % 3, (r / 3) % 3, r / 9};
-                if (v == 1 && r < 3 && c->plan.progs[(size_t)b * c->plan.nregions + r].simple)
+                // list 1 leaves out the simple regions inside the bulk
+                // grid (every region in 2D; rz = 0, i.e. r < 9, in 3D)
+                if (v == 1 && (c->dim == 2 || r < 9) &&
+                    c->plan.progs[(size_t)b * c->plan.nregions + r].simple)
                     continue;
                 BndSeg S;
                 for (int a = 0; a < 3; a++) {
@@ -843,6 +847,9 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     }
     c->xgroup = grouped > 0 ? 1 : 0;
     c->xgroup_ok = c->xgroup;
+    // below a quarter million nodes a step is launch-bound: one launch
+    c->fused = c->interior_nodes < (1 << 18);
+    if (const char* e = std::getenv("MPLBHIP_FUSED")) c->fused = std::atoi(e) != 0;
     if (const char* e = std::getenv("MPLBHIP_XGROUP")) {
         if (std::atoi(e) == 0) c->xgroup = 0;
         c->xgroup_env = true;
@@ -1178,7 +1185,8 @@ int mplbhip_iterate(mplbhip_ctx* c, double criteria, int check_period,
             HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
                                     static_cast<const BlockParams*>(c->blocks.p),
                                     (int)c->bp.size(), c->ntiles, c->nbulk,
-                                    c->nbnd_v[c->fold ? 1 : 0], 1, c->fold ? 1 : 0,
+                                    c->nbnd_v[c->fold ? 1 : 0], c->fused ? 0 : 1,
+                                    c->fold ? 1 : 0,
                                     static_cast<double*>(c->red.p), c->stream));
             HIPCHK(c, hipMemcpyAsync(acc, static_cast<double*>(c->red.p) +
                                               c->bp.size() * kNAcc,
@@ -1253,8 +1261,8 @@ int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
         done++;
         HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
                                 static_cast<const BlockParams*>(c->blocks.p), nb,
-                                c->ntiles, c->nbulk, c->nbnd_v[c->fold ? 1 : 0], 1,
-                                c->fold ? 1 : 0,
+                                c->ntiles, c->nbulk, c->nbnd_v[c->fold ? 1 : 0],
+                                c->fused ? 0 : 1, c->fold ? 1 : 0,
                                 static_cast<double*>(c->red.p), c->stream));
         HIPCHK(c, hipMemcpyAsync(acc.data(), c->red.p, acc.size() * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));

@@ -18,9 +18,14 @@
  * shear-wave decay rate, body-force Poiseuille flow, the Ghia et al.
  * cavity profile, and the reference quirks (frozen 3D corner populations,
  * last-definition-wins at shared edges, the extrapolation rescale that
- * skips the last population).  The halo slab extents at multi-block
- * junction corners and FDPeriodic are "parity unpinned" (no fixture and
- * no closed form reaches them); FDPeriodic is not restated.
+ * skips the last population).  Extensions with no reference counterpart
+ * are pinned too: the rotated connection by rotation equivariance
+ * (tests/test_rotated_link.py) and the obstacle bounce-back by exact
+ * Poiseuille flow at tau = 1/2 + sqrt(3/16) and exact mass conservation
+ * (tests/test_obstacles.py).  The halo slab extents at multi-block
+ * junction corners, the reduction-handle reset between the two residual
+ * sums, and FDPeriodic are "parity unpinned" (no fixture and no closed form
+ * reaches them); FDPeriodic is not restated.
  */
 #include "mplb_oracle.h"
 

@@ -9,7 +9,7 @@ import pytest
 from mplb_amd import cases, hip
 from oracle.oracle import OracleCase
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("step_form")]
 
 
 def log(m):

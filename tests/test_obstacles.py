@@ -179,6 +179,7 @@ def _same(g, o, blocks=(0,)):
 
 
 @pytest.mark.gpu
+@pytest.mark.usefixtures("step_form")
 def test_gpu_plate_channel_bitexact():
     print("obstacle plates through a periodic link, 600 steps", flush=True)
     o, g = OracleCase(2), _gpu(2)
@@ -190,6 +191,7 @@ def test_gpu_plate_channel_bitexact():
 
 
 @pytest.mark.gpu
+@pytest.mark.usefixtures("step_form")
 @pytest.mark.parametrize("dim,n", [(2, 16), (3, 14)])
 def test_gpu_obstacle_shell_bitexact(dim, n):
     print("obstacle shell %dD n=%d, 120 steps" % (dim, n), flush=True)
@@ -204,6 +206,7 @@ def test_gpu_obstacle_shell_bitexact(dim, n):
 
 
 @pytest.mark.gpu
+@pytest.mark.usefixtures("step_form")
 def test_gpu_cavity_with_rectangles_bitexact_and_residual():
     from mplb_amd import cases
     rects = [(5, 6, 9, 4), (18, 20, 8, 9)]

@@ -18,7 +18,7 @@ from mplb_amd.enums import BoundarySurface as BS
 from mplb_amd.enums import VertexType as VT
 from oracle.oracle import OracleCase
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("step_form")]
 
 F_TOL = 1e-12
 MASS_TOL = 1e-13

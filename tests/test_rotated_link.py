@@ -197,6 +197,7 @@ def _gpu(dim, **kw):
 
 
 @pytest.mark.gpu
+@pytest.mark.usefixtures("step_form")
 @pytest.mark.parametrize("dim,R", ROTATIONS)
 def test_gpu_rotated_pair_bitexact(dim, R):
     print("rotated pair dim=%d R=%s" % (dim, R.tolist()), flush=True)
@@ -209,6 +210,7 @@ def test_gpu_rotated_pair_bitexact(dim, R):
 
 
 @pytest.mark.gpu
+@pytest.mark.usefixtures("step_form")
 def test_gpu_lchannel_periodic_bitexact():
     from mplb_amd import cases
     print("closed L-channel a=9 length=14, 80 steps", flush=True)
