@@ -26,7 +26,10 @@ WS_PATH = os.path.join(_HERE, "libmplbws.so")
 _ws = None
 
 LID = 0.05                     # lattice lid speed of the cavities
-RHO_SCALE = 1.0 / (3 * LID * LID)  # scale of the density-deviation target
+# scale of the density-deviation target: a tenth of 1/(3 U^2), so that the
+# density channel (dominated by the lid-corner singularities, and only a
+# small part of the lift) weighs less than the velocity in the loss
+RHO_SCALE = 1.0 / (30 * LID * LID)
 IN_CH = 8                      # input channels (6 used, padded to 8)
 OUT_CH = 8                     # output channels (3 used)
 
@@ -127,28 +130,57 @@ def _p(t):
 
 
 # ---- the network -----------------------------------------------------------
-# (name, cin, cout, relu); the forward graph is fixed in UNet.forward
-LAYERS = [("e1a", IN_CH, 16, 1), ("e1b", 16, 16, 1), ("e2a", 16, 32, 1), ("e2b", 32, 32, 1),
-          ("e3a", 32, 64, 1), ("e3b", 64, 64, 1), ("ba", 64, 64, 1), ("bb", 64, 64, 1),
-          ("d3a", 128, 32, 1), ("d3b", 32, 32, 1), ("d2a", 64, 16, 1), ("d2b", 16, 16, 1),
-          ("d1a", 32, 16, 1), ("d1b", 16, 16, 1), ("out", 16, OUT_CH, 0)]
+# Encoder channels per level (each level: two 3x3 conv + ReLU, then 2x2
+# average pooling), a bottleneck of two conv + ReLU, and a mirrored decoder
+# (nearest 2x upsampling, concatenation with the encoder level, two conv +
+# ReLU), then a linear 3x3 output conv.  Four levels give a receptive field
+# wider than a 128^2 cavity, whose flow is global.
+ENC = (16, 32, 48, 48)
+BOTTLENECK = 48
 
 
-def param_count():
-    return sum(9 * ci * co + co for _, ci, co, _ in LAYERS)
+def layer_specs(enc=ENC, bott=BOTTLENECK):
+    """[(name, cin, cout, relu, level)]; level = image size divisor 2^level."""
+    L = []
+    cin = IN_CH
+    for k, c in enumerate(enc):
+        L += [("e%da" % k, cin, c, 1, k), ("e%db" % k, c, c, 1, k)]
+        cin = c
+    nl = len(enc)
+    L += [("ba", cin, bott, 1, nl), ("bb", bott, bott, 1, nl)]
+    cin = bott
+    for k in reversed(range(nl)):
+        c = enc[k]
+        L += [("d%da" % k, cin + c, c, 1, k), ("d%db" % k, c, c, 1, k)]
+        cin = c
+    L.append(("out", cin, OUT_CH, 0, 0))
+    return L
+
+
+LAYERS = layer_specs()
+
+
+def param_count(layers=None):
+    return sum(9 * ci * co + co for _, ci, co, _, _ in (layers or LAYERS))
 
 
 class UNet:
-    """Three-level U-Net on n x n images (n divisible by 8), batch `batch`;
+    """U-Net on n x n images (n divisible by 2^levels), batch `batch`;
     parameters in FP32 with Adam state; activations kept for backward."""
 
-    def __init__(self, n=128, batch=32, seed=0):
+    def __init__(self, n=128, batch=32, seed=0, enc=ENC, bott=BOTTLENECK):
         self.lib = load()
         self.n, self.batch = n, batch
+        self.enc, self.bott = tuple(enc), bott
+        self.layers = layer_specs(self.enc, bott)
+        nl = len(self.enc)
+        if n % (1 << nl):
+            raise WarmStartError("image size %d is not divisible by %d" % (n, 1 << nl))
         rng = np.random.default_rng(seed)
         self.w, self.b = {}, {}
         self.gw, self.gb, self.state = {}, {}, {}
-        for name, ci, co, relu in LAYERS:
+        self.spec = {}
+        for name, ci, co, relu, lev in self.layers:
             std = math.sqrt(2.0 / (9 * ci)) if relu else math.sqrt(1.0 / (9 * ci))
             w = (rng.standard_normal((9, ci, co)) * std).astype(np.float32)
             self.w[name] = DeviceArray.from_numpy(w)
@@ -157,41 +189,35 @@ class UNet:
             self.gb[name] = DeviceArray((co,), zero=True)
             self.state[name] = [DeviceArray(w.shape, zero=True), DeviceArray(w.shape, zero=True),
                                 DeviceArray((co,), zero=True), DeviceArray((co,), zero=True)]
-        self.spec = {name: (ci, co, relu) for name, ci, co, relu in LAYERS}
+            self.spec[name] = (ci, co, relu, n >> lev)
         self.step_count = 0
         self._alloc(batch)
 
     def _alloc(self, batch):
-        n = self.n
+        n, nl = self.n, len(self.enc)
         self.batch = batch
         e = lambda s, c: DeviceArray((batch, s, s, c))
-        self.act = {
-            "x": e(n, IN_CH), "e1a": e(n, 16), "e1b": e(n, 16), "p1": e(n // 2, 16),
-            "e2a": e(n // 2, 32), "e2b": e(n // 2, 32), "p2": e(n // 4, 32),
-            "e3a": e(n // 4, 64), "e3b": e(n // 4, 64), "p3": e(n // 8, 64),
-            "ba": e(n // 8, 64), "bb": e(n // 8, 64), "c3": e(n // 4, 128),
-            "d3a": e(n // 4, 32), "d3b": e(n // 4, 32), "c2": e(n // 2, 64),
-            "d2a": e(n // 2, 16), "d2b": e(n // 2, 16), "c1": e(n, 32),
-            "d1a": e(n, 16), "d1b": e(n, 16), "out": e(n, OUT_CH)}
-        self.grad = {k: DeviceArray(v.shape) for k, v in self.act.items()}
-        self.gout = DeviceArray(self.act["out"].shape)
-        ws = 0
-        for name, (ci, co, _) in self.spec.items():
-            s = self._size(name)
+        a = {"x": e(n, IN_CH)}
+        for name, ci, co, relu, lev in self.layers:
+            a[name] = e(n >> lev, co)
+        for k, c in enumerate(self.enc):
+            a["p%d" % k] = e(n >> (k + 1), c)            # pooled level k
+            below = self.bott if k == nl - 1 else self.enc[k + 1]
+            a["c%d" % k] = e(n >> k, below + c)           # decoder input of level k
+        self.act = a
+        self.grad = {k: DeviceArray(v.shape) for k, v in a.items()}
+        self.gout = DeviceArray(a["out"].shape)
+        ws, wf = 0, 0
+        for name, (ci, co, _, s) in self.spec.items():
             ws = max(ws, self.lib.mplbws_wgrad_workspace(batch, s, s, ci, co))
+            wf = max(wf, 9 * ci * co)
         self.workspace = DeviceArray((int(ws),))
-        self.wflip = DeviceArray((9 * 128 * 64,))
+        self.wflip = DeviceArray((wf,))
         self.lossbuf = DeviceArray((1024,), zero=True)
-
-    def _size(self, name):
-        n = self.n
-        return {"e1": n, "e2": n // 2, "e3": n // 4, "ba": n // 8, "bb": n // 8,
-                "d3": n // 4, "d2": n // 2, "d1": n, "ou": n}[name[:2]]
 
     # -- building blocks
     def _conv(self, name, src, dst):
-        ci, co, relu = self.spec[name]
-        s = self._size(name)
+        ci, co, relu, s = self.spec[name]
         _ck(self.lib.mplbws_conv3x3_fwd(_p(src), _p(self.w[name]), _p(self.b[name]), _p(dst),
                                         self.batch, s, s, ci, co, relu), name)
 
@@ -199,8 +225,7 @@ class UNet:
         """dout: gradient at the layer output (overwritten with the
         pre-activation gradient for ReLU layers); din: input gradient (or
         None for the first layer)."""
-        ci, co, relu = self.spec[name]
-        s = self._size(name)
+        ci, co, relu, s = self.spec[name]
         if relu:
             _ck(self.lib.mplbws_relu_bwd(_p(dout), _p(out), _p(dout), dout.numel()), "relu")
         _ck(self.lib.mplbws_conv3x3_wgrad(_p(src), _p(dout), _p(self.gw[name]), _p(self.gb[name]),
@@ -213,75 +238,60 @@ class UNet:
     def forward(self, x):
         """x: (batch, n, n, IN_CH) host array; returns the output activation
         (device, (batch, n, n, OUT_CH), channels 0..2 used)."""
-        a, L, n, B = self.act, self.lib, self.n, self.batch
+        a, L, n, B, nl = self.act, self.lib, self.n, self.batch, len(self.enc)
         a["x"].upload(x)
-        self._conv("e1a", a["x"], a["e1a"])
-        self._conv("e1b", a["e1a"], a["e1b"])
-        _ck(L.mplbws_pool2_fwd(_p(a["e1b"]), _p(a["p1"]), B, n, n, 16), "pool")
-        self._conv("e2a", a["p1"], a["e2a"])
-        self._conv("e2b", a["e2a"], a["e2b"])
-        _ck(L.mplbws_pool2_fwd(_p(a["e2b"]), _p(a["p2"]), B, n // 2, n // 2, 32), "pool")
-        self._conv("e3a", a["p2"], a["e3a"])
-        self._conv("e3b", a["e3a"], a["e3b"])
-        _ck(L.mplbws_pool2_fwd(_p(a["e3b"]), _p(a["p3"]), B, n // 4, n // 4, 64), "pool")
-        self._conv("ba", a["p3"], a["ba"])
+        src = a["x"]
+        for k, c in enumerate(self.enc):
+            self._conv("e%da" % k, src, a["e%da" % k])
+            self._conv("e%db" % k, a["e%da" % k], a["e%db" % k])
+            s = n >> k
+            _ck(L.mplbws_pool2_fwd(_p(a["e%db" % k]), _p(a["p%d" % k]), B, s, s, c), "pool")
+            src = a["p%d" % k]
+        self._conv("ba", src, a["ba"])
         self._conv("bb", a["ba"], a["bb"])
-        s = n // 8
-        _ck(L.mplbws_up2_fwd(_p(a["bb"]), _p(a["c3"]), B, s, s, 64, 128, 0), "up")
-        _ck(L.mplbws_channels(_p(a["e3b"]), _p(a["c3"]), B * 4 * s * s, 64, 128, 64, 0), "cat")
-        self._conv("d3a", a["c3"], a["d3a"])
-        self._conv("d3b", a["d3a"], a["d3b"])
-        s = n // 4
-        _ck(L.mplbws_up2_fwd(_p(a["d3b"]), _p(a["c2"]), B, s, s, 32, 64, 0), "up")
-        _ck(L.mplbws_channels(_p(a["e2b"]), _p(a["c2"]), B * 4 * s * s, 32, 64, 32, 0), "cat")
-        self._conv("d2a", a["c2"], a["d2a"])
-        self._conv("d2b", a["d2a"], a["d2b"])
-        s = n // 2
-        _ck(L.mplbws_up2_fwd(_p(a["d2b"]), _p(a["c1"]), B, s, s, 16, 32, 0), "up")
-        _ck(L.mplbws_channels(_p(a["e1b"]), _p(a["c1"]), B * 4 * s * s, 16, 32, 16, 0), "cat")
-        self._conv("d1a", a["c1"], a["d1a"])
-        self._conv("d1b", a["d1a"], a["d1b"])
-        self._conv("out", a["d1b"], a["out"])
+        cur, cc = a["bb"], self.bott
+        for k in reversed(range(nl)):
+            c, s = self.enc[k], n >> (k + 1)
+            cat = a["c%d" % k]
+            _ck(L.mplbws_up2_fwd(_p(cur), _p(cat), B, s, s, cc, cc + c, 0), "up")
+            _ck(L.mplbws_channels(_p(a["e%db" % k]), _p(cat), B * 4 * s * s, c, cc + c, cc, 0),
+                "cat")
+            self._conv("d%da" % k, cat, a["d%da" % k])
+            self._conv("d%db" % k, a["d%da" % k], a["d%db" % k])
+            cur, cc = a["d%db" % k], c
+        self._conv("out", cur, a["out"])
         return a["out"]
 
     def backward(self, gout):
         """gout: gradient at the output activation; fills self.gw/self.gb."""
-        a, g, L, n, B = self.act, self.grad, self.lib, self.n, self.batch
-        self._conv_bwd("out", a["d1b"], a["out"], gout, g["d1b"])
-        self._conv_bwd("d1b", a["d1a"], a["d1b"], g["d1b"], g["d1a"])
-        self._conv_bwd("d1a", a["c1"], a["d1a"], g["d1a"], g["c1"])
-        s = n // 2
-        _ck(L.mplbws_up2_bwd(_p(g["c1"]), _p(g["d2b"]), B, s, s, 16, 32, 0), "up")
-        self._conv_bwd("d2b", a["d2a"], a["d2b"], g["d2b"], g["d2a"])
-        self._conv_bwd("d2a", a["c2"], a["d2a"], g["d2a"], g["c2"])
-        s = n // 4
-        _ck(L.mplbws_up2_bwd(_p(g["c2"]), _p(g["d3b"]), B, s, s, 32, 64, 0), "up")
-        self._conv_bwd("d3b", a["d3a"], a["d3b"], g["d3b"], g["d3a"])
-        self._conv_bwd("d3a", a["c3"], a["d3a"], g["d3a"], g["c3"])
-        s = n // 8
-        _ck(L.mplbws_up2_bwd(_p(g["c3"]), _p(g["bb"]), B, s, s, 64, 128, 0), "up")
+        a, g, L, n, B, nl = self.act, self.grad, self.lib, self.n, self.batch, len(self.enc)
+        self._conv_bwd("out", a["d0b"], a["out"], gout, g["d0b"])
+        for k in range(nl):
+            c, s = self.enc[k], n >> (k + 1)
+            cc = self.bott if k == nl - 1 else self.enc[k + 1]
+            self._conv_bwd("d%db" % k, a["d%da" % k], a["d%db" % k], g["d%db" % k], g["d%da" % k])
+            self._conv_bwd("d%da" % k, a["c%d" % k], a["d%da" % k], g["d%da" % k], g["c%d" % k])
+            below = "bb" if k == nl - 1 else "d%db" % (k + 1)
+            _ck(L.mplbws_up2_bwd(_p(g["c%d" % k]), _p(g[below]), B, s, s, cc, cc + c, 0), "up")
         self._conv_bwd("bb", a["ba"], a["bb"], g["bb"], g["ba"])
-        self._conv_bwd("ba", a["p3"], a["ba"], g["ba"], g["p3"])
-        # e3b feeds the pool and the skip into c3
-        _ck(L.mplbws_pool2_bwd(_p(g["p3"]), _p(g["e3b"]), B, n // 4, n // 4, 64), "pool")
-        _ck(L.mplbws_channels(_p(g["c3"]), _p(g["e3b"]), B * (n // 4) ** 2, 64, 128, 64, 2),
-            "cat")
-        self._conv_bwd("e3b", a["e3a"], a["e3b"], g["e3b"], g["e3a"])
-        self._conv_bwd("e3a", a["p2"], a["e3a"], g["e3a"], g["p2"])
-        _ck(L.mplbws_pool2_bwd(_p(g["p2"]), _p(g["e2b"]), B, n // 2, n // 2, 32), "pool")
-        _ck(L.mplbws_channels(_p(g["c2"]), _p(g["e2b"]), B * (n // 2) ** 2, 32, 64, 32, 2),
-            "cat")
-        self._conv_bwd("e2b", a["e2a"], a["e2b"], g["e2b"], g["e2a"])
-        self._conv_bwd("e2a", a["p1"], a["e2a"], g["e2a"], g["p1"])
-        _ck(L.mplbws_pool2_bwd(_p(g["p1"]), _p(g["e1b"]), B, n, n, 16), "pool")
-        _ck(L.mplbws_channels(_p(g["c1"]), _p(g["e1b"]), B * n * n, 16, 32, 16, 2), "cat")
-        self._conv_bwd("e1b", a["e1a"], a["e1b"], g["e1b"], g["e1a"])
-        self._conv_bwd("e1a", a["x"], a["e1a"], g["e1a"], None)
+        self._conv_bwd("ba", a["p%d" % (nl - 1)], a["ba"], g["ba"], g["p%d" % (nl - 1)])
+        for k in reversed(range(nl)):
+            c, s = self.enc[k], n >> k
+            cc = self.bott if k == nl - 1 else self.enc[k + 1]
+            # e_kb feeds the pool and the skip into c_k
+            _ck(L.mplbws_pool2_bwd(_p(g["p%d" % k]), _p(g["e%db" % k]), B, s, s, c), "pool")
+            _ck(L.mplbws_channels(_p(g["c%d" % k]), _p(g["e%db" % k]), B * s * s, c, cc + c, cc,
+                                  2), "cat")
+            self._conv_bwd("e%db" % k, a["e%da" % k], a["e%db" % k], g["e%db" % k], g["e%da" % k])
+            src = a["x"] if k == 0 else a["p%d" % (k - 1)]
+            gin = None if k == 0 else g["p%d" % (k - 1)]
+            self._conv_bwd("e%da" % k, src, a["e%da" % k], g["e%da" % k], gin)
 
     def loss_grad(self, target, mask):
-        """Masked MSE over the 3 used output channels.  target (batch, n, n,
-        3) and mask (batch, n, n) are host arrays.  Returns (loss, gradient
-        at the output, device)."""
+        """Masked MSE over the 3 used output channels (the density channel
+        weighted by RHO_WEIGHT through its target scale).  target (batch, n,
+        n, 3) and mask (batch, n, n) are host arrays.  Returns (loss,
+        gradient at the output, device)."""
         pixels = self.batch * self.n * self.n
         fluid = float((np.asarray(mask) == 0).sum())
         inv = 1.0 / max(fluid * 3.0, 1.0)
@@ -451,7 +461,7 @@ def velocity_l2(g_a, g_b, count):
     return np.array(out)
 
 
-def run(n=128, n_train=4096, n_test=512, epochs=30, criteria=1e-8, check_period=1000,
+def run(n=128, n_train=4096, n_test=512, epochs=60, criteria=1e-8, check_period=1000,
         seed=0x4D504C42, batch=32, chunk=512, storage=hip.STORE_F32, log=print):
     """Time-to-residual of learned versus rest initialisation.
 
@@ -533,6 +543,8 @@ def run(n=128, n_train=4096, n_test=512, epochs=30, criteria=1e-8, check_period=
         "rest_steps_median": float(np.median(s_rest)), "rest_steps_max": int(s_rest.max()),
         "learned_steps_median": float(np.median(s_warm)),
         "learned_steps_max": int(s_warm.max()),
+        "per_case_step_ratio_median": float(np.median(s_rest / np.maximum(s_warm, 1))),
+        "work_ratio_sum_steps": float(s_rest.sum() / max(s_warm.sum(), 1)),
         "prediction_rel_l2_median": float(np.median(pred_err)),
         "velocity_rel_l2_learned_vs_rest_median": float(np.median(l2)),
         "velocity_rel_l2_learned_vs_rest_max": float(l2.max()),

@@ -61,7 +61,10 @@ def test_oracle_bf16_rounding():
 
 
 def test_network_size():
-    assert 200_000 <= ws.param_count() <= 220_000
+    assert 250_000 <= ws.param_count() <= 350_000
+    # every convolution input has a multiple of 8 channels (the MFMA K step)
+    for name, ci, co, relu, lev in ws.LAYERS:
+        assert ci % 8 == 0, name
 
 
 def test_library_exports_header():
@@ -151,47 +154,44 @@ def test_gpu_pool_up_channels():
     assert abs(float((nno.up2(y) * z[..., 2:6]).sum()) - float((y * _np(gy)).sum())) < 1e-4
 
 
-def _cpu_unet(params, x, target, mask):
+def _cpu_unet(params, x, target, mask, enc, bott, rb=True):
     """float64 forward/backward of the U-Net from the oracle layers (BF16
-    rounding of conv operands as on the GPU); returns (loss, grads)."""
+    rounding of conv operands as on the GPU when rb); returns (loss, grads)."""
     P = params
-    act = {}
+    spec = {nm: (ci, co, r) for nm, ci, co, r, _ in ws.layer_specs(enc, bott)}
+    act, inp = {}, {}
 
     def conv(name, src):
-        ci, co, relu = [(c_i, c_o, r) for nm, c_i, c_o, r in ws.LAYERS if nm == name][0]
-        act[name] = nno.conv3x3(src, P["w_" + name], P["b_" + name], relu=bool(relu))
+        inp[name] = src
+        act[name] = nno.conv3x3(src, P["w_" + name], P["b_" + name], relu=bool(spec[name][2]),
+                                round_bf16=rb)
         return act[name]
 
-    a = conv("e1a", x)
-    e1b = conv("e1b", a)
-    e2b = conv("e2b", conv("e2a", nno.pool2(e1b)))
-    e3b = conv("e3b", conv("e3a", nno.pool2(e2b)))
-    bb = conv("bb", conv("ba", nno.pool2(e3b)))
-    c3 = np.concatenate([nno.up2(bb), e3b], axis=3)
-    d3b = conv("d3b", conv("d3a", c3))
-    c2 = np.concatenate([nno.up2(d3b), e2b], axis=3)
-    d2b = conv("d2b", conv("d2a", c2))
-    c1 = np.concatenate([nno.up2(d2b), e1b], axis=3)
-    d1b = conv("d1b", conv("d1a", c1))
-    out = conv("out", d1b)
+    nl = len(enc)
+    skips = []
+    cur = x
+    for k in range(nl):
+        e = conv("e%db" % k, conv("e%da" % k, cur))
+        skips.append(e)
+        cur = nno.pool2(e)
+    cur = conv("bb", conv("ba", cur))
+    for k in reversed(range(nl)):
+        cat = np.concatenate([nno.up2(cur), skips[k]], axis=3)
+        cur = conv("d%db" % k, conv("d%da" % k, cat))
+    out = conv("out", cur)
     fluid = (mask == 0)[..., None]
     cnt = fluid.sum() * 3
     diff = (out[..., :3] - target) * fluid
     loss = float((diff ** 2).sum() / cnt)
     gout = np.zeros_like(out)
     gout[..., :3] = 2 * diff / cnt
-    inputs = {"e1a": x, "e1b": act["e1a"], "e2a": nno.pool2(e1b), "e2b": act["e2a"],
-              "e3a": nno.pool2(e2b), "e3b": act["e3a"], "ba": nno.pool2(e3b), "bb": act["ba"],
-              "d3a": c3, "d3b": act["d3a"], "d2a": c2, "d2b": act["d2a"], "d1a": c1,
-              "d1b": act["d1a"], "out": d1b}
     grads = {}
 
     def back(name, g):
-        relu = [r for nm, _, _, r in ws.LAYERS if nm == name][0]
-        if relu:
+        if spec[name][2]:
             g = np.where(act[name] > 0, g, 0.0)
-        grads["w_" + name], grads["b_" + name] = nno.conv3x3_wgrad(inputs[name], g)
-        return nno.conv3x3_dgrad(g, P["w_" + name])
+        grads["w_" + name], grads["b_" + name] = nno.conv3x3_wgrad(inp[name], g, round_bf16=rb)
+        return nno.conv3x3_dgrad(g, P["w_" + name], round_bf16=rb)
 
     def pool_t(g):
         return nno.up2(g) / 4.0
@@ -200,24 +200,53 @@ def _cpu_unet(params, x, target, mask):
         n, h, w, c = g.shape
         return g.reshape(n, h // 2, 2, w // 2, 2, c).sum(axis=(2, 4))
 
-    g = back("d1a", back("d1b", back("out", gout)))
-    g_e1b = g[..., 16:]
-    g = back("d2a", back("d2b", up_t(g[..., :16])))
-    g_e2b = g[..., 32:]
-    g = back("d3a", back("d3b", up_t(g[..., :32])))
-    g_e3b = g[..., 64:]
-    g = back("ba", back("bb", up_t(g[..., :64])))
-    g = back("e3a", back("e3b", pool_t(g) + g_e3b))
-    g = back("e2a", back("e2b", pool_t(g) + g_e2b))
-    back("e1a", back("e1b", pool_t(g) + g_e1b))
+    g = back("out", gout)
+    gskip = [None] * nl
+    for k in range(nl):
+        c = enc[k]
+        gcat = back("d%da" % k, back("d%db" % k, g))
+        cc = gcat.shape[3] - c
+        gskip[k] = gcat[..., cc:]
+        g = up_t(gcat[..., :cc])
+    g = back("ba", back("bb", g))
+    for k in reversed(range(nl)):
+        g = pool_t(g) + gskip[k]
+        g = back("e%da" % k, back("e%db" % k, g))
     return loss, grads
+
+
+def test_cpu_unet_gradient_matches_finite_differences():
+    """The CPU composition used to check the GPU U-Net, in plain float64,
+    against central differences of its own loss."""
+    enc, bott, n = (8, 8), 8, 8
+    rng = np.random.default_rng(4)
+    P = {}
+    for name, ci, co, relu, _ in ws.layer_specs(enc, bott):
+        P["w_" + name] = rng.standard_normal((9, ci, co)) / math.sqrt(9 * ci)
+        P["b_" + name] = 0.1 * rng.standard_normal(co)
+    x = rng.standard_normal((1, n, n, ws.IN_CH))
+    target = rng.standard_normal((1, n, n, 3))
+    mask = np.zeros((1, n, n))
+    mask[0, 2:4, 3:5] = 1
+    loss, grads = _cpu_unet(P, x, target, mask, enc, bott, rb=False)
+    e = 1e-6
+    for key, idx in [("w_e0a", (4, 2, 3)), ("w_ba", (0, 1, 5)), ("w_d1a", (8, 9, 2)),
+                     ("w_out", (4, 3, 1)), ("b_e1b", (2,)), ("w_d0b", (1, 7, 0))]:
+        Pp = {k: v.copy() for k, v in P.items()}
+        Pm = {k: v.copy() for k, v in P.items()}
+        Pp[key][idx] += e
+        Pm[key][idx] -= e
+        fd = (_cpu_unet(Pp, x, target, mask, enc, bott, rb=False)[0]
+              - _cpu_unet(Pm, x, target, mask, enc, bott, rb=False)[0]) / (2 * e)
+        assert abs(fd - grads[key][idx]) <= 1e-6 * max(1.0, abs(fd)), (key, fd, grads[key][idx])
 
 
 @pytest.mark.gpu
 def test_gpu_unet_gradients_match_cpu():
-    print("U-Net 16x16 batch 2 gradients vs CPU", flush=True)
+    print("U-Net 16x16 batch 2 (3 levels) gradients vs CPU", flush=True)
     n, B = 16, 2
-    net = ws.UNet(n=n, batch=B, seed=5)
+    enc, bott = (8, 16, 16), 16
+    net = ws.UNet(n=n, batch=B, seed=5, enc=enc, bott=bott)
     res, rects = [150.0, 700.0], [[(3, 4, 5, 6)], []]
     x = ws.inputs(n, res, rects)
     rng = np.random.default_rng(2)
@@ -227,9 +256,9 @@ def test_gpu_unet_gradients_match_cpu():
     loss_gpu, gout = net.loss_grad(target, mask)
     net.backward(gout)
     params = net.state_dict()
-    loss_cpu, grads = _cpu_unet(params, x, target.astype(np.float64), mask)
+    loss_cpu, grads = _cpu_unet(params, x, target.astype(np.float64), mask, enc, bott)
     assert abs(loss_gpu - loss_cpu) <= 1e-4 * abs(loss_cpu)
-    for name, _, _, _ in ws.LAYERS:
+    for name, _, _, _, _ in ws.layer_specs(enc, bott):
         gw = net.gw[name].numpy()
         gb = net.gb[name].numpy()
         rw, rb = grads["w_" + name], grads["b_" + name]
@@ -243,7 +272,7 @@ def test_gpu_unet_gradients_match_cpu():
 def test_gpu_training_reduces_loss():
     print("U-Net 32x32 training steps", flush=True)
     n, B = 32, 8
-    net = ws.UNet(n=n, batch=B, seed=1)
+    net = ws.UNet(n=n, batch=B, seed=1, enc=(16, 32, 32), bott=32)
     res, rects = ws.case_params(B, seed=9, n=n)
     x = ws.inputs(n, res, rects)
     yy, xx = np.meshgrid(np.linspace(0, 1, n), np.linspace(0, 1, n), indexing="ij")
