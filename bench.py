@@ -254,7 +254,7 @@ def main():
     ap.add_argument("--ws-n", type=int, default=128)
     ap.add_argument("--ws-train", type=int, default=4096)
     ap.add_argument("--ws-test", type=int, default=512)
-    ap.add_argument("--ws-epochs", type=int, default=60)
+    ap.add_argument("--ws-epochs", type=int, default=100)
     ap.add_argument("--workload", choices=("cavity3d", "cavity2d", "lchannel", "batch",
                                            "warmstart"),
                     default="cavity3d",

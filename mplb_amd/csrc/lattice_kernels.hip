@@ -684,7 +684,7 @@ reduce:
 // budget small.  Tiles of kTX x kTY nodes, z planes 1 .. n-2 (3D).  The
 // block's first bulk workgroup also records the block's parity after the
 // step.
-template <int D, int ST, int MODE, bool OBST, bool FOLD>
+template <int D, int ST, int MODE, bool OBST, int FOLD>
 __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
     using T = typename Store<ST>::T;
     using E = typename Store<ST>::E;
@@ -714,8 +714,11 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
     const bool interior = x >= 1 && x < B.n[0] - 1 && y >= 1 && y < B.n[1] - 1;
     const RegionProg* P = nullptr;
     if (FOLD && !interior && x < B.n[0] && y < B.n[1]) {
-        P = &kp.progs[B.prog_base + region_code(x, B.n[0]) + 3 * region_code(y, B.n[1])];
-        if (!P->simple) P = nullptr;
+        const bool yin = y >= 1 && y < B.n[1] - 1;
+        if (FOLD == 2 || yin) {
+            P = &kp.progs[B.prog_base + region_code(x, B.n[0]) + 3 * region_code(y, B.n[1])];
+            if (!P->simple) P = nullptr;
+        }
     }
     const bool active = B.active && (interior || P);
     if (active) {
@@ -731,15 +734,29 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
             solid_residual<D, MODE>(kp, mnode, acc);
         } else {
             T f[Q];
-            // every lane loads all Q populations (an x-face lane's unpulled
-            // ones come from its halo neighbour, in bounds), so the loads
-            // stay one uniform burst; the pull mask applies afterwards
+            // every lane loads all Q populations, so the loads stay one
+            // uniform burst; the pull mask applies afterwards.  A boundary
+            // lane's unpulled source is moved back inside the block along
+            // the axes where it left it: that row is one the wave (or its
+            // workgroup) loads anyway, so no halo line is fetched for it.
+            int32_t adj[3] = {0, 0, 0};
+            if (FOLD && P) {
+                adj[0] = x == 0 ? 1 : (x == B.n[0] - 1 ? -1 : 0);
+                adj[1] = y == 0 ? B.sx : (y == B.n[1] - 1 ? -B.sx : 0);
+            }
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 const bool bounce = OBST && (om >> q & 1u);
+                uint32_t nd = (uint32_t)node;
+                if (FOLD && P && !(pull >> q & 1u)) {
+                    // only the components that point out of the block
+                    if (cvel(D, q, 0) > 0 && adj[0] > 0) nd += 1;
+                    if (cvel(D, q, 0) < 0 && adj[0] < 0) nd -= 1;
+                    if (cvel(D, q, 1) > 0 && adj[1] > 0) nd += (uint32_t)B.sx;
+                    if (cvel(D, q, 1) < 0 && adj[1] < 0) nd -= (uint32_t)B.sx;
+                }
                 const E* p = bounce ? at_u(prev, (int64_t)opp(D, q) * B.sq, (uint32_t)node)
-                                    : at_u(prev, (int64_t)q * B.sq - pull_delta<D>(B, q),
-                                           (uint32_t)node);
+                                    : at_u(prev, (int64_t)q * B.sq - pull_delta<D>(B, q), nd);
                 if constexpr (ST == MPLBHIP_STORE_F16) f[q] = __half2float(*p) * kp.scale[q];
                 else f[q] = *p;
             }
@@ -907,11 +924,15 @@ static void launch_step(const KParams& kp, int part, hipStream_t s) {
     if (part != 2 && kp.nbulk > 0) {
         const dim3 g(kp.nbulk);
         if (kp.obst) {
-            if (kp.fold) hipLaunchKernelGGL((k_bulk<D, ST, MODE, true, true>), g, blk, 0, s, kp);
-            else hipLaunchKernelGGL((k_bulk<D, ST, MODE, true, false>), g, blk, 0, s, kp);
+            if (kp.fold == 2) hipLaunchKernelGGL((k_bulk<D, ST, MODE, true, 2>), g, blk, 0, s, kp);
+            else if (kp.fold == 1)
+                hipLaunchKernelGGL((k_bulk<D, ST, MODE, true, 1>), g, blk, 0, s, kp);
+            else hipLaunchKernelGGL((k_bulk<D, ST, MODE, true, 0>), g, blk, 0, s, kp);
         } else {
-            if (kp.fold) hipLaunchKernelGGL((k_bulk<D, ST, MODE, false, true>), g, blk, 0, s, kp);
-            else hipLaunchKernelGGL((k_bulk<D, ST, MODE, false, false>), g, blk, 0, s, kp);
+            if (kp.fold == 2) hipLaunchKernelGGL((k_bulk<D, ST, MODE, false, 2>), g, blk, 0, s, kp);
+            else if (kp.fold == 1)
+                hipLaunchKernelGGL((k_bulk<D, ST, MODE, false, 1>), g, blk, 0, s, kp);
+            else hipLaunchKernelGGL((k_bulk<D, ST, MODE, false, 0>), g, blk, 0, s, kp);
         }
     }
     if (part != 1 && kp.nbnd > 0)

@@ -36,10 +36,11 @@ struct BlockParams {
     // n-2 in 3D) in tiles_x * tiles_y tiles per plane; the boundary kernel
     // walks the block's boundary segments, 256 nodes per workgroup
     int32_t bulk_tile_start;  // first bulk workgroup of this block
-    // boundary workgroups and segments, [fold]: 0 = every boundary region,
-    // 1 = without the x-face regions the bulk kernel runs inline
-    int32_t bnd_wg_start[2];
-    int32_t seg_start[2], seg_count[2];
+    // boundary workgroups and segments per fold level: [0] every boundary
+    // region, [1] without the simple x-face interiors, [2] without every
+    // simple region of the bulk grid (those run inside the bulk kernel)
+    int32_t bnd_wg_start[3];
+    int32_t seg_start[3], seg_count[3];
     int32_t xgroup;           // tiles_x when this block's bulk tiles can be
                               // dealt a row per XCD, else 0
     int32_t pad2;
@@ -98,7 +99,8 @@ struct KParams {
     int32_t want_residual;  // accumulate the residual against MacroVarsCopy
     const uint32_t* obst;   // obstacle link masks per interior node, or null
     const BndSeg* segs;     // boundary segments, blocks in order
-    int32_t fold;           // the bulk kernel runs the simple boundary regions of its grid
+    int32_t fold;           // 0 none, 1 simple x faces, 2 every simple region of
+                            // the bulk grid run inside the bulk kernel
     int32_t xcd_remap;      // bulk tiles dealt to XCDs in contiguous runs
     int32_t xgroup;         // 1: deal each block's rows of tiles to one XCD
     int32_t fused;          // 1: the step is one whole-lattice launch

@@ -71,10 +71,11 @@ struct mplbhip_ctx {
     int64_t interior_nodes = 0;
     int ntiles = 0;            // whole-lattice tiles
     int nbulk = 0;             // bulk tiles of a step
-    int nbnd_v[2] = {0, 0};    // boundary workgroups of a step, [fold]
+    int nbnd_v[3] = {0, 0, 0}; // boundary workgroups of a step, [fold]
     bool tuned = false;        // launch configuration chosen (autotune)
     bool fused = false;        // small case: one whole-lattice launch per step
-    bool fold = true;          // simple x-face regions run inside the bulk kernel
+    int fold = 1;              // simple regions the bulk kernel runs inline:
+                               // 0 none, 1 x faces, 2 every one in its grid
     bool xcd_remap = false;    // bulk tiles dealt to XCDs in contiguous runs
     int xgroup = 0;            // 1: rows of bulk tiles dealt to one XCD each
     int xgroup_ok = 0;         // 1 when some block allows that grouping
@@ -193,8 +194,8 @@ static KParams base_params(mplbhip_ctx* c) {
     kp.nblocks = (int)c->bp.size();
     kp.ntiles = c->ntiles;
     kp.nbulk = c->nbulk;
-    kp.nbnd = c->nbnd_v[c->fold ? 1 : 0];
-    kp.fold = c->fold ? 1 : 0;
+    kp.nbnd = c->nbnd_v[c->fold];
+    kp.fold = c->fold;
     kp.xcd_remap = c->xcd_remap ? 1 : 0;
     kp.xgroup = c->xgroup;
     kp.fused = c->fused ? 1 : 0;
@@ -748,13 +749,15 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     const int align = (int)(128 / c->es);
     c->bp.assign(nb, BlockParams{});
     int64_t off = 0, moff = 0;
-    int tiles = 0, bulk = 0, bnd_v[2] = {0, 0};
+    int tiles = 0, bulk = 0, bnd_v[3] = {0, 0, 0};
     c->segs.clear();
-    // measured at 256^3: folding pays for 4- and 8-byte populations; with
-    // 2-byte ones the extra lanes cost more than the strided boundary loads
-    c->fold = c->storage != MPLBHIP_STORE_F16;
+    // measured at 256^3: folding the x faces pays for 4- and 8-byte
+    // populations; with 2-byte ones the extra lanes cost more than the
+    // strided boundary loads.  The first step's autotuning decides for
+    // large cases.
+    c->fold = c->storage != MPLBHIP_STORE_F16 ? 1 : 0;
     if (const char* e = std::getenv("MPLBHIP_XFACE_FOLD")) {
-        c->fold = std::atoi(e) != 0;
+        c->fold = std::min(2, std::max(0, std::atoi(e)));
         c->fold_env = true;
     }
     if (const char* e = std::getenv("MPLBHIP_XCD_REMAP")) c->xcd_remap = std::atoi(e) != 0;
@@ -789,18 +792,18 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         // boundary regions as segments, 256 nodes per workgroup
         B.bulk_tile_start = bulk;
         bulk += B.tiles_x * B.tiles_y * (c->dim == 3 ? n[2] - 2 : 1);
-        // two segment lists: [0] every boundary region, [1] without the
-        // simple regions the bulk kernel runs inline when folding
-        for (int v = 0; v < 2; v++) {
+        // three segment lists: [0] every boundary region, [1] without the
+        // simple x-face interiors, [2] without every simple region of the
+        // bulk grid (the bulk kernel runs those inline at that fold level)
+        for (int v = 0; v < 3; v++) {
             B.seg_start[v] = (int)c->segs.size();
             int count = 0;
             for (int r = 1; r < c->plan.nregions; r++) {
                 const int code[3] = {r % 3, (r / 3) % 3, r / 9};
-                // list 1 leaves out the simple regions inside the bulk
-                // grid (every region in 2D; rz = 0, i.e. r < 9, in 3D)
-                if (v == 1 && (c->dim == 2 || r < 9) &&
-                    c->plan.progs[(size_t)b * c->plan.nregions + r].simple)
-                    continue;
+                const bool simple = c->plan.progs[(size_t)b * c->plan.nregions + r].simple;
+                if (v == 1 && r < 3 && simple) continue;
+                // every region in 2D; rz = 0, i.e. r < 9, in 3D
+                if (v == 2 && (c->dim == 2 || r < 9) && simple) continue;
                 BndSeg S;
                 for (int a = 0; a < 3; a++) {
                     if (a == 2 && c->dim == 2) {
@@ -831,8 +834,7 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     c->interior_nodes = moff;
     c->ntiles = tiles;
     c->nbulk = bulk;
-    c->nbnd_v[0] = bnd_v[0];
-    c->nbnd_v[1] = bnd_v[1];
+    for (int v = 0; v < 3; v++) c->nbnd_v[v] = bnd_v[v];
     // the x tiles of a row share their edge cache lines: deal each row's
     // tiles to one XCD (hardware hands workgroups to the 8 XCDs round
     // robin) when every block has the same row of tiles
@@ -885,7 +887,8 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         HIPCHK(c, hipMemcpyAsync(c->dobst.p, c->obst.data(), c->dobst.bytes,
                                  hipMemcpyHostToDevice, c->stream));
     }
-    const size_t nslots = (size_t)std::max(tiles, bulk + std::max(bnd_v[0], bnd_v[1]));
+    const size_t nslots =
+        (size_t)std::max(tiles, bulk + std::max(bnd_v[0], std::max(bnd_v[1], bnd_v[2])));
     if (int rc = dalloc(c, c->partials, nslots * kNAcc * sizeof(double))) return rc;
     if (int rc = dalloc(c, c->dsegs, std::max<size_t>(1, c->segs.size()) * sizeof(BndSeg)))
         return rc;
@@ -1011,8 +1014,8 @@ int mplbhip_set_accel(mplbhip_ctx* c, int block, const double* g) {
 }
 
 // Launch-configuration autotuning, once per context on its first step:
-// folding the simple x faces into the bulk kernel or not, and keeping a
-// row's tiles on one XCD or not.  Each candidate runs the step kernels a
+// which simple boundary regions the bulk kernel runs inline (none, the x
+// faces, all of its grid's), and keeping a row's tiles on one XCD or not.  Each candidate runs the step kernels a
 // few times from the current state into the other buffer; those writes are
 // what the next real step writes again, so the state is unchanged.  Cases
 // under 2^20 nodes keep the defaults (their steps are launch-bound).
@@ -1020,18 +1023,23 @@ static int autotune(mplbhip_ctx* c) {
     if (c->tuned) return 0;
     c->tuned = true;
     if (c->interior_nodes < (1 << 20)) return 0;
-    std::vector<std::pair<bool, int>> cand;
-    std::vector<bool> folds;
-    if (c->fold_env || c->nbnd_v[0] == c->nbnd_v[1]) folds = {c->fold};
-    else folds = {true, false};
+    std::vector<std::pair<int, int>> cand;
+    std::vector<int> folds;
+    if (c->fold_env) {
+        folds = {c->fold};
+    } else {
+        folds = {0};
+        if (c->nbnd_v[1] != c->nbnd_v[0]) folds.push_back(1);
+        if (c->nbnd_v[2] != c->nbnd_v[1]) folds.push_back(2);
+    }
     std::vector<int> groups;
     if (c->xgroup_env || c->xgroup_ok == 0) groups = {c->xgroup};
     else groups = {c->xgroup_ok, 0};
-    for (bool f : folds)
+    for (int f : folds)
         for (int g : groups) cand.push_back({f, g});
     if (cand.size() < 2) return 0;
     float best = 1e30f;
-    std::pair<bool, int> pick = cand[0];
+    std::pair<int, int> pick = cand[0];
     for (const auto& cf : cand) {
         c->fold = cf.first;
         c->xgroup = cf.second;
@@ -1054,7 +1062,7 @@ static int autotune(mplbhip_ctx* c) {
     drop_graphs(c);
     if (std::getenv("MPLBHIP_VERBOSE"))
         std::fprintf(stderr, "mplbhip: autotune picked fold=%d xgroup=%d (%.4f ms/step)\n",
-                     (int)c->fold, c->xgroup, best / 3);
+                     c->fold, c->xgroup, best / 3);
     return 0;
 }
 
@@ -1185,8 +1193,7 @@ int mplbhip_iterate(mplbhip_ctx* c, double criteria, int check_period,
             HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
                                     static_cast<const BlockParams*>(c->blocks.p),
                                     (int)c->bp.size(), c->ntiles, c->nbulk,
-                                    c->nbnd_v[c->fold ? 1 : 0], c->fused ? 0 : 1,
-                                    c->fold ? 1 : 0,
+                                    c->nbnd_v[c->fold], c->fused ? 0 : 1, c->fold,
                                     static_cast<double*>(c->red.p), c->stream));
             HIPCHK(c, hipMemcpyAsync(acc, static_cast<double*>(c->red.p) +
                                               c->bp.size() * kNAcc,
@@ -1261,8 +1268,8 @@ int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
         done++;
         HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
                                 static_cast<const BlockParams*>(c->blocks.p), nb,
-                                c->ntiles, c->nbulk, c->nbnd_v[c->fold ? 1 : 0],
-                                c->fused ? 0 : 1, c->fold ? 1 : 0,
+                                c->ntiles, c->nbulk, c->nbnd_v[c->fold],
+                                c->fused ? 0 : 1, c->fold,
                                 static_cast<double*>(c->red.p), c->stream));
         HIPCHK(c, hipMemcpyAsync(acc.data(), c->red.p, acc.size() * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));

@@ -461,7 +461,7 @@ def velocity_l2(g_a, g_b, count):
     return np.array(out)
 
 
-def run(n=128, n_train=4096, n_test=512, epochs=60, criteria=1e-8, check_period=1000,
+def run(n=128, n_train=4096, n_test=512, epochs=100, criteria=1e-8, check_period=1000,
         seed=0x4D504C42, batch=32, chunk=512, storage=hip.STORE_F32, log=print):
     """Time-to-residual of learned versus rest initialisation.
 
@@ -549,6 +549,11 @@ def run(n=128, n_train=4096, n_test=512, epochs=60, criteria=1e-8, check_period=
         "velocity_rel_l2_learned_vs_rest_median": float(np.median(l2)),
         "velocity_rel_l2_learned_vs_rest_max": float(l2.max()),
         "total_seconds": round(time.perf_counter() - t_all, 1),
+        # the cases that set the batch times
+        "slowest_cases": [{"Re": round(tr[b], 1), "rects": len(tq[b]),
+                           "rest_steps": int(s_rest[b]), "learned_steps": int(s_warm[b]),
+                           "prediction_rel_l2": round(pred_err[b], 4)}
+                          for b in np.argsort(-s_rest)[:5]],
     }
     g_rest.close()
     g_warm.close()

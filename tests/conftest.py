@@ -13,11 +13,15 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: long CPU oracle runs")
 
 
-@pytest.fixture(params=["split", "fused"])
+@pytest.fixture(params=["split-fold0", "split-fold1", "split-fold2", "fused"])
 def step_form(request, monkeypatch):
-    """Run a test with both forms of the step: bulk + boundary kernels
-    (what cases of 2^18 nodes and more use) and the single whole-lattice
-    launch (what smaller cases use).  The library reads MPLBHIP_FUSED when
-    a case is prepared."""
-    monkeypatch.setenv("MPLBHIP_FUSED", "1" if request.param == "fused" else "0")
+    """Run a test with every form of the step: bulk + boundary kernels with
+    the bulk kernel running none, the x-face, or all simple boundary regions
+    of its grid (the forms cases of 2^18 nodes and more choose between), and
+    the single whole-lattice launch (what smaller cases use).  The library
+    reads these variables when a case is prepared."""
+    fused = request.param == "fused"
+    monkeypatch.setenv("MPLBHIP_FUSED", "1" if fused else "0")
+    if not fused:
+        monkeypatch.setenv("MPLBHIP_XFACE_FOLD", request.param[-1])
     return request.param
