@@ -162,6 +162,15 @@ int mplbhip_define_block_collision(mplbhip_ctx* ctx, int block, double tau_ref,
 int mplbhip_iterate_batch(mplbhip_ctx* ctx, double criteria, int check_period,
                           long long max_steps, long long* steps_per_block,
                           double* residual_per_block, long long* steps_done);
+/* No reference counterpart: the batched solver's fallback for warm-started
+ * instances.  With factor > 0, an instance whose velocity residual at a
+ * check exceeds factor times the lowest it has reached is restarted once at
+ * rest (both fStage buffers = feq(1, 0), macros rho = 1, u = 0) and goes on
+ * from there.  0 (the default) turns it off.  Refused for cases whose
+ * boundaries persist state (3D corners). */
+int mplbhip_set_batch_restart(mplbhip_ctx* ctx, double factor);
+/* restarts per block during the last mplbhip_iterate_batch */
+int mplbhip_batch_restarts(mplbhip_ctx* ctx, int* restarts);
 
 /* UpdateMacroVars[3D] (model_wrapper.cpp:112-275): macros of the current f */
 int mplbhip_update_macros(mplbhip_ctx* ctx);

@@ -845,6 +845,179 @@ hipError_t launch_lift(int dim, const KParams& kp, int64_t nodes, hipStream_t s)
     return hipGetLastError();
 }
 
+// ---- FP16 storage: two nodes per lane ---------------------------------------------
+// The BGK update of one node in deviation form, FP32 arithmetic, as in
+// finish_node's step path: g in (deviations), h out (storage units).
+template <int D>
+__device__ __forceinline__ void collide_dev(const KParams& kp, const BlockParams& B,
+                                            const float* g, float* h) {
+    constexpr int Q = D == 2 ? 9 : 19;
+    float drho = 0.f, j[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        drho += g[q];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            if (cvel(D, q, d) > 0) j[d] += g[q];
+            if (cvel(D, q, d) < 0) j[d] -= g[q];
+        }
+    }
+    const float rho = 1.f + drho;
+    const float inv = 1.f / rho;
+    float ul[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        ul[d] = j[d] * inv;
+        if (kp.force_macros) ul[d] += B.fhalf_dtg[d];
+    }
+    const float usq = ul[0] * ul[0] + ul[1] * ul[1] + ul[2] * ul[2];
+    const float om = B.fomega, om1 = B.fone_minus_omega;
+    const bool addf = kp.force_type == 1;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        float cu = 0.f;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            if (cvel(D, q, d) > 0) cu += ul[d];
+            if (cvel(D, q, d) < 0) cu -= ul[d];
+        }
+        const float wq = float(wgt(D, q));
+        const float geq = wq * (drho + rho * (3.f * cu + 4.5f * cu * cu - 1.5f * usq));
+        float gn = om1 * g[q] + om * geq;
+        if (addf) {
+            float cg = 0.f;
+#pragma unroll
+            for (int d = 0; d < D; d++) cg += float(cvel(D, q, d)) * float(B.g[d]);
+            gn += B.fforce * wq * rho * cg;
+        }
+        h[q] = gn * kp.inv_scale[q];
+    }
+}
+
+__device__ __forceinline__ uint32_t ld_u32(const __half* p) {
+    return *reinterpret_cast<const uint32_t*>(p);
+}
+__device__ __forceinline__ float h_lo(uint32_t v) {
+    return __half2float(__ushort_as_half((unsigned short)(v & 0xFFFFu)));
+}
+__device__ __forceinline__ float h_hi(uint32_t v) {
+    return __half2float(__ushort_as_half((unsigned short)(v >> 16)));
+}
+
+// FP16 bulk step with two neighbouring nodes (x0, x0 + 1; x0 even) per
+// lane: every population moves as one 4-byte half pair per lane.  A pull
+// along x by one node crosses the pair boundary; the missing half comes
+// from the neighbouring lane (shuffle), and the lanes at the wave's ends
+// load it themselves.  Interior nodes only: the boundary kernel runs every
+// boundary region (fold level 0).  Rows of 128 nodes per wave.
+template <int D>
+__global__ __launch_bounds__(kTX * kTY) void k_bulk_h2(KParams kp) {
+    constexpr int Q = D == 2 ? 9 : 19;
+    const int wg = (int)blockIdx.x;
+    const int b = find_block<&BlockParams::h2_tile_start>(kp, wg);
+    const BlockParams& B = kp.blocks[b];
+    int t = wg - B.h2_tile_start;
+    const bool first = t == 0;
+    const int tx = t % B.tiles_x2;
+    t /= B.tiles_x2;
+    const int ty = t % B.tiles_y;
+    const int z = D == 3 ? 1 + t / B.tiles_y : 0;
+    const int lane = threadIdx.x;
+    const int x0 = tx * 2 * kTX + 2 * lane;
+    const int y = ty * kTY + threadIdx.y;
+    const int cur = kp.cur_in[b];
+    if (first && lane == 0 && threadIdx.y == 0) kp.cur_out[b] = B.active ? 1 - cur : cur;
+    if (!B.active) return;
+    const bool yin = y >= 1 && y < B.n[1] - 1;
+    const bool a0 = yin && x0 >= 1 && x0 < B.n[0] - 1;
+    const bool a1 = yin && x0 + 1 >= 1 && x0 + 1 < B.n[0] - 1;
+    // every lane loads (the shuffles need all of them).  FP16 rows are
+    // padded so that x0 - 1 .. x0 + 2 of every lane of the grid lie in the
+    // row; padding rows load the last interior row instead.  An active node
+    // only ever uses x - 1 .. x + 1 of its own row.
+    const int yc = y < B.n[1] - 1 ? y : B.n[1] - 1;
+    const int32_t node = z * B.sz + yc * B.sx + x0;
+    const __half* prev = static_cast<const __half*>(kp.fs_buf[cur]) + B.base;
+    __half* next = static_cast<__half*>(kp.fs_buf[1 - cur]) + B.base;
+    float g0[Q], g1[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int cx = cvel(D, q, 0);
+        // the aligned pair of the source row, before the x shift
+        const int32_t dyz = cvel(D, q, 1) * B.sx + (D == 3 ? cvel(D, q, 2) * B.sz : 0);
+        const __half* row = prev + (int64_t)q * B.sq - dyz;
+        const uint32_t P = ld_u32(row + node);
+        float lo, hi;
+        if (cx == 0) {
+            lo = h_lo(P);
+            hi = h_hi(P);
+        } else if (cx > 0) {
+            // (x0 - 1, x0): the low half of the pair is the left lane's high
+            const uint32_t L = (uint32_t)__shfl_up((int)P, 1, 64);
+            const float left = lane == 0 ? __half2float(row[node - 1]) : h_hi(L);
+            lo = left;
+            hi = h_lo(P);
+        } else {
+            // (x0 + 1, x0 + 2): the high half is the right lane's low
+            const uint32_t R = (uint32_t)__shfl_down((int)P, 1, 64);
+            const float right = lane == 63 ? __half2float(row[node + 2]) : h_lo(R);
+            lo = h_hi(P);
+            hi = right;
+        }
+        g0[q] = lo * kp.scale[q];
+        g1[q] = hi * kp.scale[q];
+    }
+    if (!(a0 || a1)) return;
+    float h0[Q], h1[Q];
+    collide_dev<D>(kp, B, g0, h0);
+    collide_dev<D>(kp, B, g1, h1);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        __half* p = next + (int64_t)q * B.sq + node;
+        if (a0 && a1) {
+            const uint32_t v = (uint32_t)__half_as_ushort(__float2half_rn(h0[q])) |
+                               ((uint32_t)__half_as_ushort(__float2half_rn(h1[q])) << 16);
+            __builtin_nontemporal_store(v, reinterpret_cast<uint32_t*>(p));
+        } else if (a0) {
+            *p = __float2half_rn(h0[q]);
+        } else {
+            p[1] = __float2half_rn(h1[q]);
+        }
+    }
+}
+
+hipError_t launch_bulk_h2(int dim, const KParams& kp, hipStream_t s) {
+    if (kp.nbulk2 <= 0) return hipSuccess;
+    dim3 blk(kTX, kTY, 1);
+    if (dim == 2) hipLaunchKernelGGL(k_bulk_h2<2>, dim3(kp.nbulk2), blk, 0, s, kp);
+    else hipLaunchKernelGGL(k_bulk_h2<3>, dim3(kp.nbulk2), blk, 0, s, kp);
+    return hipGetLastError();
+}
+
+// ---- restart of an instance at rest (batched solver fallback) --------------------
+// The macros of block b and their residual copies become rho = 1, u = 0, as
+// set_macros leaves them before a rest start; an INIT_FEQ launch in which
+// only the restarting blocks are active then forms their fStage from them.
+__global__ void k_rest_macros(KParams kp, int b) {
+    const BlockParams& B = kp.blocks[b];
+    const int64_t nn = (int64_t)B.n[0] * B.n[1] * B.n[2];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nn;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        for (int v = 0; v < 4; v++) {
+            kp.macro[v][B.mbase + i] = v == 0 ? 1.0 : 0.0;
+            kp.macro_copy[v][B.mbase + i] = v == 0 ? 1.0 : 0.0;
+        }
+    }
+}
+
+hipError_t launch_rest_macros(const KParams& kp, int b, int64_t nodes, hipStream_t s) {
+    int64_t grid = (nodes + 255) / 256;
+    if (grid > 4096) grid = 4096;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_rest_macros, dim3((unsigned)grid), dim3(256), 0, s, kp, b);
+    return hipGetLastError();
+}
+
 // fStage halo slab copy of one block connection (Field::TransferHalos ->
 // ops_halo_transfer, field.h:94-99, slabs of field.h:215-395)
 template <typename E>
@@ -915,6 +1088,12 @@ __global__ void k_fill_bytes(uint32_t* p, int64_t n, uint32_t v) {
 template <int D, int ST, int MODE>
 static void launch_step(const KParams& kp, int part, hipStream_t s) {
     dim3 blk(kTX, kTY, 1);
+    if (ST == MPLBHIP_STORE_F16 && MODE == MODE_STEP && kp.h2 && !kp.fused) {
+        if (part != 2) (void)launch_bulk_h2(D, kp, s);
+        if (part != 1 && kp.nbnd > 0)
+            hipLaunchKernelGGL((k_lattice<D, ST, MODE, SCOPE_BND>), dim3(kp.nbnd), blk, 0, s, kp);
+        return;
+    }
     if (kp.fused) {
         if (part != 2)
             hipLaunchKernelGGL((k_lattice<D, ST, MODE, SCOPE_ALL>), dim3(kp.ntiles), blk, 0, s,

@@ -43,6 +43,8 @@ struct BlockParams {
     int32_t seg_start[3], seg_count[3];
     int32_t xgroup;           // tiles_x when this block's bulk tiles can be
                               // dealt a row per XCD, else 0
+    int32_t tiles_x2;         // FP16 two-node bulk kernel: 128-node tiles per row
+    int32_t h2_tile_start;    // its first workgroup of this block
     int32_t pad2;
     int32_t has_obst;         // the block has obstacle link masks
 };
@@ -104,6 +106,8 @@ struct KParams {
     int32_t xcd_remap;      // bulk tiles dealt to XCDs in contiguous runs
     int32_t xgroup;         // 1: deal each block's rows of tiles to one XCD
     int32_t fused;          // 1: the step is one whole-lattice launch
+    int32_t h2;             // 1: FP16 steps use the two-node bulk kernel
+    int32_t nbulk2;         // its workgroups
     int32_t nbulk;          // bulk workgroups (partials 0 .. nbulk-1)
     int32_t nbnd;           // boundary workgroups (partials nbulk ..)
 };

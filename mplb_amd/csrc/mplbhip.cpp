@@ -26,6 +26,7 @@ hipError_t launch_reduce(const double* partials, const BlockParams* blocks,
                          double* out, hipStream_t s);
 hipError_t launch_fill(void* p, int64_t bytes, hipStream_t s);
 hipError_t launch_lift(int dim, const KParams& kp, int64_t nodes, hipStream_t s);
+hipError_t launch_rest_macros(const KParams& kp, int b, int64_t nodes, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
 }  // namespace mplbhip
 
@@ -74,6 +75,15 @@ struct mplbhip_ctx {
     int nbnd_v[3] = {0, 0, 0}; // boundary workgroups of a step, [fold]
     bool tuned = false;        // launch configuration chosen (autotune)
     bool fused = false;        // small case: one whole-lattice launch per step
+    bool h2 = false;           // FP16: two-node-per-lane bulk kernel (fold level 0)
+    bool h2_ok = false, h2_env = false;
+    int nbulk2 = 0;            // its workgroups
+    double restart_factor = 0.0;     // batched solver: restart a rising instance at rest
+    std::vector<int> batch_restarts;  // per block, last iterate_batch
+    std::vector<BlockParams> restart_bp;  // block parameters of a restart launch
+    bool fused_env = false, fused_full = false;  // MPLBHIP_FUSED given; prepare's choice
+    int64_t grid_nodes = 0;    // nodes of the blocks the step launches cover
+    bool batch_compact = true;  // iterate_batch shrinks the launches to live blocks
     int fold = 1;              // simple regions the bulk kernel runs inline:
                                // 0 none, 1 x faces, 2 every one in its grid
     bool xcd_remap = false;    // bulk tiles dealt to XCDs in contiguous runs
@@ -194,8 +204,12 @@ static KParams base_params(mplbhip_ctx* c) {
     kp.nblocks = (int)c->bp.size();
     kp.ntiles = c->ntiles;
     kp.nbulk = c->nbulk;
-    kp.nbnd = c->nbnd_v[c->fold];
-    kp.fold = c->fold;
+    // the two-node FP16 kernel covers interior nodes only: fold level 0
+    const int fold = c->h2 ? 0 : c->fold;
+    kp.nbnd = c->nbnd_v[fold];
+    kp.fold = fold;
+    kp.h2 = c->h2 ? 1 : 0;
+    kp.nbulk2 = c->nbulk2;
     kp.xcd_remap = c->xcd_remap ? 1 : 0;
     kp.xgroup = c->xgroup;
     kp.fused = c->fused ? 1 : 0;
@@ -737,6 +751,59 @@ static int build_obstacles(mplbhip_ctx* c) {
     return 0;
 }
 
+// Workgroup ranges of the blocks in the step launches (whole tiles, the
+// two-node FP16 tiles, the bulk tiles and the three boundary segment
+// lists) and the per-block XCD row grouping.  With only_active, frozen
+// blocks get empty ranges that start where the next block's range does,
+// so the launches' block search (the last block whose start <= the
+// workgroup) never lands on them.  Returns the number of row-grouped blocks.
+static int assign_tiles(mplbhip_ctx* c, bool only_active) {
+    int tiles = 0, bulk = 0, bulk2 = 0, bnd_v[3] = {0, 0, 0};
+    int64_t nodes = 0;
+    for (BlockParams& B : c->bp) {
+        const bool on = !only_active || B.active;
+        const int zb = c->dim == 3 ? B.n[2] - 2 : 1;
+        B.tile_start = tiles;
+        B.bulk_tile_start = bulk;
+        B.h2_tile_start = bulk2;
+        if (on) {
+            tiles += B.tiles_x * B.tiles_y * B.n[2];
+            bulk += B.tiles_x * B.tiles_y * zb;
+            bulk2 += B.tiles_x2 * B.tiles_y * zb;
+            nodes += (int64_t)B.n[0] * B.n[1] * B.n[2];
+        }
+        for (int v = 0; v < 3; v++) {
+            B.bnd_wg_start[v] = bnd_v[v];
+            if (!on) continue;
+            int count = 0;
+            for (int k = B.seg_start[v]; k < B.seg_start[v] + B.seg_count[v]; k++)
+                count += c->segs[(size_t)k].count;
+            bnd_v[v] += (count + kBndWG - 1) / kBndWG;
+        }
+    }
+    c->ntiles = tiles;
+    c->nbulk = bulk;
+    c->nbulk2 = bulk2;
+    for (int v = 0; v < 3; v++) c->nbnd_v[v] = bnd_v[v];
+    c->grid_nodes = nodes;
+    // the x tiles of a row share their edge cache lines: deal each row's
+    // tiles to one XCD (hardware hands workgroups to the 8 XCDs round
+    // robin) where the block's tiles start on a multiple of 8 and fill
+    // whole groups of 8 rows
+    const int nb = (int)c->bp.size();
+    int grouped = 0;
+    for (int b = 0; b < nb; b++) {
+        BlockParams& B = c->bp[b];
+        const int next = b + 1 < nb ? c->bp[b + 1].bulk_tile_start : bulk;
+        const int cnt = next - B.bulk_tile_start;
+        const bool ok = cnt > 0 && B.bulk_tile_start % 8 == 0 && cnt % (8 * B.tiles_x) == 0 &&
+                        !c->xcd_remap;
+        B.xgroup = ok ? B.tiles_x : 0;
+        grouped += ok ? 1 : 0;
+    }
+    return grouped;
+}
+
 int mplbhip_prepare(mplbhip_ctx* c) {
     if (!c) return MPLBHIP_ERR_ARG;
     if (c->prepared) return fail(c, MPLBHIP_ERR_ARG, "case already prepared");
@@ -749,7 +816,7 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     const int align = (int)(128 / c->es);
     c->bp.assign(nb, BlockParams{});
     int64_t off = 0, moff = 0;
-    int tiles = 0, bulk = 0, bnd_v[3] = {0, 0, 0};
+    int tiles = 0, bulk = 0, bulk2 = 0, bnd_v[3] = {0, 0, 0};
     c->segs.clear();
     // measured at 256^3: folding the x faces pays for 4- and 8-byte
     // populations; with 2-byte ones the extra lanes cost more than the
@@ -772,7 +839,12 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         // contiguous slab (fewer DRAM pages and TLB entries per wave than
         // Q whole-lattice arrays)
         const int ox = align;
-        const int64_t sx = ((int64_t)ox + n[0] + 1 + align - 1) / align * align;
+        int64_t row = (int64_t)ox + n[0] + 1;
+        // FP16: the two-node bulk kernel's lanes load x0 - 1 .. x0 + 2 over
+        // whole 128-node tiles; keep all of that inside the row
+        if (c->storage == MPLBHIP_STORE_F16)
+            row = std::max<int64_t>(row, (int64_t)ox + 2 * kTX * ((n[0] + 2 * kTX - 1) / (2 * kTX)) + 2);
+        const int64_t sx = (row + align - 1) / align * align;
         const int64_t sq = sx * (n[1] + 2);
         const int64_t sz = sq * c->Q;
         const int64_t elems = sz * (n[2] + 2 * hz);
@@ -792,6 +864,9 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         // boundary regions as segments, 256 nodes per workgroup
         B.bulk_tile_start = bulk;
         bulk += B.tiles_x * B.tiles_y * (c->dim == 3 ? n[2] - 2 : 1);
+        B.tiles_x2 = (n[0] + 2 * kTX - 1) / (2 * kTX);
+        B.h2_tile_start = bulk2;
+        bulk2 += B.tiles_x2 * B.tiles_y * (c->dim == 3 ? n[2] - 2 : 1);
         // three segment lists: [0] every boundary region, [1] without the
         // simple x-face interiors, [2] without every simple region of the
         // bulk grid (the bulk kernel runs those inline at that fold level)
@@ -835,29 +910,31 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     c->ntiles = tiles;
     c->nbulk = bulk;
     for (int v = 0; v < 3; v++) c->nbnd_v[v] = bnd_v[v];
-    // the x tiles of a row share their edge cache lines: deal each row's
-    // tiles to one XCD (hardware hands workgroups to the 8 XCDs round
-    // robin) when every block has the same row of tiles
-    int grouped = 0;
-    for (int b = 0; b < nb; b++) {
-        BlockParams& B = c->bp[b];
-        const int next = b + 1 < nb ? c->bp[b + 1].bulk_tile_start : bulk;
-        const int cnt = next - B.bulk_tile_start;
-        const bool ok = B.bulk_tile_start % 8 == 0 && cnt % (8 * B.tiles_x) == 0 && !c->xcd_remap;
-        B.xgroup = ok ? B.tiles_x : 0;
-        grouped += ok ? 1 : 0;
-    }
+    const int grouped = assign_tiles(c, false);
+    (void)bulk2;
     c->xgroup = grouped > 0 ? 1 : 0;
     c->xgroup_ok = c->xgroup;
     // below a quarter million nodes a step is launch-bound: one launch
     c->fused = c->interior_nodes < (1 << 18);
-    if (const char* e = std::getenv("MPLBHIP_FUSED")) c->fused = std::atoi(e) != 0;
+    if (const char* e = std::getenv("MPLBHIP_FUSED")) {
+        c->fused = std::atoi(e) != 0;
+        c->fused_env = true;
+    }
+    c->fused_full = c->fused;
+    if (const char* e = std::getenv("MPLBHIP_BATCH_COMPACT")) c->batch_compact = std::atoi(e) != 0;
     if (const char* e = std::getenv("MPLBHIP_XGROUP")) {
         if (std::atoi(e) == 0) c->xgroup = 0;
         c->xgroup_env = true;
     }
     if (int rc = setup_halos(c)) return rc;
     if (int rc = build_obstacles(c)) return rc;
+    // the two-node FP16 kernel has no obstacle path
+    c->h2_ok = c->storage == MPLBHIP_STORE_F16 && c->obst.empty();
+    c->h2 = c->h2_ok;
+    if (const char* e = std::getenv("MPLBHIP_H2")) {
+        c->h2 = c->h2_ok && std::atoi(e) != 0;
+        c->h2_env = true;
+    }
     c->prepared = true;
     if (c->device < 0) return 0;
     HIPCHK(c, hipSetDevice(c->device));
@@ -1023,7 +1100,7 @@ static int autotune(mplbhip_ctx* c) {
     if (c->tuned) return 0;
     c->tuned = true;
     if (c->interior_nodes < (1 << 20)) return 0;
-    std::vector<std::pair<int, int>> cand;
+    std::vector<std::array<int, 3>> cand;  // fold, xgroup, h2
     std::vector<int> folds;
     if (c->fold_env) {
         folds = {c->fold};
@@ -1035,14 +1112,17 @@ static int autotune(mplbhip_ctx* c) {
     std::vector<int> groups;
     if (c->xgroup_env || c->xgroup_ok == 0) groups = {c->xgroup};
     else groups = {c->xgroup_ok, 0};
-    for (int f : folds)
-        for (int g : groups) cand.push_back({f, g});
+    if (!(c->h2_env && c->h2))
+        for (int f : folds)
+            for (int g : groups) cand.push_back({f, g, 0});
+    if (c->h2_ok && !(c->h2_env && !c->h2)) cand.push_back({0, 0, 1});
     if (cand.size() < 2) return 0;
     float best = 1e30f;
-    std::pair<int, int> pick = cand[0];
+    std::array<int, 3> pick = cand[0];
     for (const auto& cf : cand) {
-        c->fold = cf.first;
-        c->xgroup = cf.second;
+        c->fold = cf[0];
+        c->xgroup = cf[1];
+        c->h2 = cf[2] != 0;
         KParams kp = base_params(c);
         HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP, kp, c->stream));
         HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
@@ -1057,12 +1137,13 @@ static int autotune(mplbhip_ctx* c) {
             pick = cf;
         }
     }
-    c->fold = pick.first;
-    c->xgroup = pick.second;
+    c->fold = pick[0];
+    c->xgroup = pick[1];
+    c->h2 = pick[2] != 0;
     drop_graphs(c);
     if (std::getenv("MPLBHIP_VERBOSE"))
-        std::fprintf(stderr, "mplbhip: autotune picked fold=%d xgroup=%d (%.4f ms/step)\n",
-                     c->fold, c->xgroup, best / 3);
+        std::fprintf(stderr, "mplbhip: autotune picked fold=%d xgroup=%d h2=%d (%.4f ms/step)\n",
+                     c->fold, c->xgroup, (int)c->h2, best / 3);
     return 0;
 }
 
@@ -1193,7 +1274,8 @@ int mplbhip_iterate(mplbhip_ctx* c, double criteria, int check_period,
             HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
                                     static_cast<const BlockParams*>(c->blocks.p),
                                     (int)c->bp.size(), c->ntiles, c->nbulk,
-                                    c->nbnd_v[c->fold], c->fused ? 0 : 1, c->fold,
+                                    c->nbnd_v[c->h2 ? 0 : c->fold], c->fused ? 0 : 1,
+                                    c->h2 ? 0 : c->fold,
                                     static_cast<double*>(c->red.p), c->stream));
             HIPCHK(c, hipMemcpyAsync(acc, static_cast<double*>(c->red.p) +
                                               c->bp.size() * kNAcc,
@@ -1232,6 +1314,46 @@ int mplbhip_define_block_collision(mplbhip_ctx* c, int block, double tau_ref, do
     return 0;
 }
 
+// Shrink the step launches to the active blocks (on) or restore them to
+// every block (off).  Blocks outside the launches carry no parity forward,
+// so both parity arrays first take the current parities; the launches'
+// grids change, so captured graphs go.  The caller uploads c->bp.
+static int compact_grid(mplbhip_ctx* c, bool on) {
+    const size_t nb = c->bp.size();
+    if (on)
+        HIPCHK(c, hipMemcpyAsync(c->parity[1 - c->cur].p, c->parity[c->cur].p,
+                                 nb * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
+    assign_tiles(c, on);
+    // a small remainder runs as one launch per step, as small cases do
+    c->fused = c->fused_env ? c->fused_full
+                            : (on ? c->grid_nodes < (1 << 18) : c->fused_full);
+    drop_graphs(c);
+    return 0;
+}
+
+// Restart the listed blocks at rest, as a rest start forms its first
+// fStage: macros (and residual copies) rho = 1, u = 0, then an INIT_FEQ
+// launch in which only these blocks are active.  INIT_FEQ rewrites each
+// node's latest fStage in place from its own macros, so the parities stay.
+// The caller re-uploads the block parameters afterwards.
+static int restart_at_rest(mplbhip_ctx* c, const std::vector<int>& blocks) {
+    KParams kp = base_params(c);
+    for (int b : blocks) {
+        const BlockParams& B = c->bp[b];
+        HIPCHK(c, launch_rest_macros(kp, b, (int64_t)B.n[0] * B.n[1] * B.n[2], c->stream));
+    }
+    c->restart_bp = c->bp;
+    for (auto& B : c->restart_bp) B.active = 0;
+    for (int b : blocks) c->restart_bp[b].active = 1;
+    HIPCHK(c, hipMemcpyAsync(c->blocks.p, c->restart_bp.data(),
+                             c->restart_bp.size() * sizeof(BlockParams), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_INIT_FEQ, kp, c->stream));
+    // the host copy must outlive the upload
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
                           long long max_steps, long long* steps_b, double* res_b,
                           long long* steps_done) {
@@ -1243,9 +1365,14 @@ int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
     if (!c->plan.links.empty())
         return fail(c, MPLBHIP_ERR_UNSUPPORTED,
                     "instances with block connections cannot be frozen one by one");
+    if (c->restart_factor > 0 && c->plan.persist_total > 0)
+        return fail(c, MPLBHIP_ERR_UNSUPPORTED,
+                    "instances with persisted boundary state cannot restart at rest");
     const int nb = (int)c->bp.size();
     std::vector<long long> conv(nb, -1);
     std::vector<double> res(nb, -1.0);
+    std::vector<double> minres(nb, INFINITY);
+    c->batch_restarts.assign(nb, 0);
     for (int b = 0; b < nb; b++) c->bp[b].active = 1;
     if (int rc = upload_blocks(c)) return rc;
     long long done = 0;
@@ -1268,13 +1395,14 @@ int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
         done++;
         HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
                                 static_cast<const BlockParams*>(c->blocks.p), nb,
-                                c->ntiles, c->nbulk, c->nbnd_v[c->fold],
-                                c->fused ? 0 : 1, c->fold,
+                                c->ntiles, c->nbulk, c->nbnd_v[c->h2 ? 0 : c->fold],
+                                c->fused ? 0 : 1, c->h2 ? 0 : c->fold,
                                 static_cast<double*>(c->red.p), c->stream));
         HIPCHK(c, hipMemcpyAsync(acc.data(), c->red.p, acc.size() * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         bool changed = false;
+        std::vector<int> restart;
         for (int b = 0; b < nb; b++) {
             if (!c->bp[b].active) continue;
             const double* a = acc.data() + (size_t)b * kNAcc;
@@ -1284,6 +1412,16 @@ int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
                 if (mx <= rel) mx = rel;
             }
             res[b] = mx;
+            // fallback: an instance whose residual climbs well above the
+            // lowest it reached (or is no longer a number) restarts once at rest
+            if (c->restart_factor > 0 && c->batch_restarts[b] == 0 &&
+                !(mx <= c->restart_factor * minres[b])) {
+                restart.push_back(b);
+                c->batch_restarts[b]++;
+                minres[b] = INFINITY;
+                continue;
+            }
+            if (mx < minres[b]) minres[b] = mx;
             if (mx < criteria) {
                 c->bp[b].active = 0;
                 conv[b] = c->tstep;
@@ -1291,7 +1429,19 @@ int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
                 changed = true;
             }
         }
-        if (changed)
+        if (!restart.empty())
+            if (int rc = restart_at_rest(c, restart)) return rc;
+        if (changed && remaining > 0) {
+            // compaction: once the launches cover a quarter more nodes than
+            // the unconverged instances hold, they shrink to those
+            int64_t live = 0;
+            for (int b = 0; b < nb; b++)
+                if (c->bp[b].active)
+                    live += (int64_t)c->bp[b].n[0] * c->bp[b].n[1] * c->bp[b].n[2];
+            if (c->batch_compact && c->grid_nodes > live + live / 4)
+                if (int rc = compact_grid(c, true)) return rc;
+        }
+        if (changed || !restart.empty())
             HIPCHK(c, hipMemcpyAsync(c->blocks.p, c->bp.data(), nb * sizeof(BlockParams),
                                      hipMemcpyHostToDevice, c->stream));
     }
@@ -1302,6 +1452,8 @@ int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
         frozen = frozen || !c->bp[b].active;
         c->bp[b].active = 1;
     }
+    if (c->grid_nodes != c->interior_nodes)
+        if (int rc = compact_grid(c, false)) return rc;
     if (frozen)
         HIPCHK(c, hipMemcpyAsync(c->blocks.p, c->bp.data(), nb * sizeof(BlockParams),
                                  hipMemcpyHostToDevice, c->stream));
@@ -1311,6 +1463,22 @@ int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
     if (res_b)
         for (int b = 0; b < nb; b++) res_b[b] = res[b];
     if (steps_done) *steps_done = done;
+    return 0;
+}
+
+int mplbhip_set_batch_restart(mplbhip_ctx* c, double factor) {
+    if (!c || factor < 0) return MPLBHIP_ERR_ARG;
+    if (factor > 0 && c->prepared && c->plan.persist_total > 0)
+        return fail(c, MPLBHIP_ERR_UNSUPPORTED,
+                    "instances with persisted boundary state cannot restart at rest");
+    c->restart_factor = factor;
+    return 0;
+}
+
+int mplbhip_batch_restarts(mplbhip_ctx* c, int* restarts) {
+    if (!c || !restarts) return MPLBHIP_ERR_ARG;
+    for (size_t b = 0; b < c->bp.size(); b++)
+        restarts[b] = b < c->batch_restarts.size() ? c->batch_restarts[b] : 0;
     return 0;
 }
 

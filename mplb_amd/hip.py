@@ -81,6 +81,8 @@ def load():
         "mplbhip_last_residual": (I, [P, DP, DP]),
         "mplbhip_define_block_collision": (I, [P, I, D, D]),
         "mplbhip_iterate_batch": (I, [P, D, I, ctypes.c_longlong, LLP, DP, LLP]),
+        "mplbhip_set_batch_restart": (I, [P, D]),
+        "mplbhip_batch_restarts": (I, [P, IP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -253,6 +255,17 @@ class LatticeCase:
             steps.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _dp(res),
             ctypes.byref(done)))
         return steps, res, done.value
+
+    def set_batch_restart(self, factor):
+        """Restart, once, a batched instance whose residual climbs above
+        factor times its lowest (0: off)."""
+        self._ck(self.lib.mplbhip_set_batch_restart(self.h, float(factor)))
+
+    def batch_restarts(self):
+        out = np.zeros(len(self.sizes), dtype=np.int32)
+        self._ck(self.lib.mplbhip_batch_restarts(
+            self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return out
 
     def update_macros(self):
         self._ck(self.lib.mplbhip_update_macros(self.h))

@@ -88,3 +88,60 @@ def test_batch_c4_fp32_runs():
     g.update_macros()
     for b in (0, 100, 255):
         assert np.all(np.isfinite(g.get_macro(b, 1)))
+
+
+def test_batch_restart_at_rest_replays_a_fresh_run():
+    """The fallback restart: with a factor that any residual exceeds, every
+    instance restarts at rest at its second check (step 2 * period) and then
+    follows a fresh rest run exactly (FP64): it converges 2 * period steps
+    later than the fresh run, with the same state and residual."""
+    sizes, res = small_batch()
+    crit, period = 1e-4, 50
+    fresh = hip.LatticeCase(2, storage=hip.STORE_F64, device=0)
+    cases.batched_cavities(fresh, sizes, res)
+    s_fresh, r_fresh, _ = fresh.iterate_batch(crit, period, 20000)
+    g = hip.LatticeCase(2, storage=hip.STORE_F64, device=0)
+    cases.batched_cavities(g, sizes, res)
+    g.set_batch_restart(1e-3)
+    steps, resid, _ = g.iterate_batch(crit, period, 20000)
+    log("restarted steps %s fresh %s" % (list(steps), list(s_fresh)))
+    assert np.all(s_fresh > 2 * period)
+    assert list(g.batch_restarts()) == [1] * len(sizes)
+    assert np.array_equal(steps, s_fresh + 2 * period)
+    assert np.array_equal(resid, r_fresh)
+    for b in range(len(sizes)):
+        assert np.array_equal(g.get_f(b), fresh.get_f(b)), b
+    # off by default and when set to 0
+    g.set_batch_restart(0)
+    g.iterate_batch(crit, period, 10 * period)
+    assert list(g.batch_restarts()) == [0] * len(sizes)
+    fresh.close()
+    g.close()
+
+
+def test_batch_compaction_matches_full_launches(monkeypatch):
+    """Shrinking the step launches to the unconverged instances (and, below
+    2^18 nodes, to one launch per step) changes neither any instance's
+    convergence step nor its state, and the launches over every instance
+    come back afterwards: later steps match too (FP64, bit-exact f)."""
+    monkeypatch.delenv("MPLBHIP_FUSED", raising=False)
+    sizes, res = cases.batch_spec(32, seed=777, sizes=(64, 96, 128, 160), re_range=(20.0, 300.0))
+    log("compaction batch: %d nodes" % sum(n * n for n in sizes))
+    assert sum(n * n for n in sizes) >= 1 << 18
+    out = {}
+    for compact in (0, 1):
+        monkeypatch.setenv("MPLBHIP_BATCH_COMPACT", str(compact))
+        g = hip.LatticeCase(2, storage=hip.STORE_F64, device=0)
+        cases.batched_cavities(g, sizes, res)
+        steps, resid, done = g.iterate_batch(1e-4, 50, 50000)
+        g.step(7)
+        out[compact] = (steps, resid, done, [g.get_f(b) for b in range(len(sizes))])
+        g.close()
+    (s0, r0, d0, f0), (s1, r1, d1, f1) = out[0], out[1]
+    log("steps %s" % list(s1))
+    assert np.all(s0 > 0)
+    assert np.array_equal(s0, s1) and d0 == d1
+    # the per-block partial sums are grouped differently in the two forms
+    assert np.allclose(r0, r1, rtol=1e-12, atol=0)
+    for b in range(len(sizes)):
+        assert np.array_equal(f0[b], f1[b]), b

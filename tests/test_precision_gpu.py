@@ -108,3 +108,36 @@ def test_c2_full_size_mirror_symmetry(storage, tol):
             float(np.max(np.abs(w + _mirror_z(w))))) / lid
     log("mirror asymmetry / lid = %.3e" % e)
     assert e <= tol
+
+
+@pytest.mark.parametrize("dim,size", [(3, (67, 40, 31)), (3, (128, 36, 20)), (2, (201, 201))])
+def test_fp16_two_node_kernel_matches_one_node(monkeypatch, dim, size):
+    """FP16 storage: the bulk kernel that moves two neighbouring nodes per
+    lane as one half pair (x shifts through lane shuffles) against the
+    one-node kernel, on odd and even row lengths.  Both do the same FP32
+    arithmetic on the same inputs, up to how the compiler fuses it, so they
+    agree to a few FP16 steps of the stored values, and both stay as close
+    to an FP64 run."""
+    log("FP16 two-node vs one-node kernel, %dD %s" % (dim, size))
+    monkeypatch.setenv("MPLBHIP_FUSED", "0")
+    monkeypatch.setenv("MPLBHIP_XFACE_FOLD", "0")
+
+    def run(storage, h2):
+        monkeypatch.setenv("MPLBHIP_H2", h2)
+        g = hip.LatticeCase(dim, storage=storage, device=0)
+        if dim == 3:
+            cases.cavity3d(g, size[0], re=300.0, ulid=0.05, noise=True, size=size)
+        else:
+            cases.cavity2d(g, size[0], re=300.0, ulid=0.05)
+        g.step(300)
+        f = g.get_f(0)
+        g.close()
+        return f
+
+    f_two, f_one, f_64 = run(hip.STORE_F16, "1"), run(hip.STORE_F16, "0"), run(hip.STORE_F64, "0")
+    d = float(np.max(np.abs(f_two - f_one)))
+    e_two = float(np.max(np.abs(f_two - f_64)))
+    e_one = float(np.max(np.abs(f_one - f_64)))
+    log("max |two - one| = %.3e; error vs FP64: two %.3e, one %.3e" % (d, e_two, e_one))
+    assert d <= 1e-4
+    assert e_two <= 1.5 * e_one + 1e-6
