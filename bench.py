@@ -223,8 +223,10 @@ def bench_warmstart(args):
     cores, then time-to-residual 1e-8 of learned versus rest starts on the
     held-out cases."""
     from mplb_amd import warmstart
+    enc = tuple(int(c) for c in args.ws_enc.split(","))
     res, _ = warmstart.run(n=args.ws_n, n_train=args.ws_train, n_test=args.ws_test,
-                           epochs=args.ws_epochs, log=log)
+                           epochs=args.ws_epochs, enc=enc, bott=args.ws_bott,
+                           diagnose=args.ws_diagnose, log=log)
     log("bench: warm start %s" % res)
     return {
         "metric": "time-to-residual 1e-8, learned vs rest initialisation",
@@ -254,7 +256,11 @@ def main():
     ap.add_argument("--ws-n", type=int, default=128)
     ap.add_argument("--ws-train", type=int, default=4096)
     ap.add_argument("--ws-test", type=int, default=512)
-    ap.add_argument("--ws-epochs", type=int, default=100)
+    ap.add_argument("--ws-epochs", type=int, default=200)
+    ap.add_argument("--ws-enc", default="24,48,64,64", help="encoder widths per level")
+    ap.add_argument("--ws-bott", type=int, default=64, help="bottleneck width")
+    ap.add_argument("--ws-diagnose", action="store_true",
+                    help="also solve from exact velocity / exact density splits")
     ap.add_argument("--workload", choices=("cavity3d", "cavity2d", "lchannel", "batch",
                                            "warmstart"),
                     default="cavity3d",

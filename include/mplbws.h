@@ -25,7 +25,8 @@ extern "C" {
 
 #define MPLBWS_ERR_ARG (-1)
 #define MPLBWS_ERR_HIP (-2)
-/* pixels per weight-gradient partial */
+/* pixels per weight-gradient partial (whole rows of one image: at least
+ * one row, at most one image) */
 #define MPLBWS_WGRAD_CHUNK 4096
 
 /* out = [relu](conv3x3(in, w) + bias); cin a multiple of 8; bias may be 0 */
@@ -53,6 +54,9 @@ int mplbws_up2_bwd(const float* dout, float* din, int n, int h, int wd, int c, i
  * 2: dst += src[:, c0:c0+c] */
 int mplbws_channels(const float* src, float* dst, int64_t pixels, int c, int ctot, int c0,
                     int direction);
+/* dst[i][0 .. elems) = src[idx[i]][0 .. elems) for i < count (idx on the
+ * device; the caller keeps every idx[i] inside src) */
+int mplbws_gather(const float* src, float* dst, const int32_t* idx, int count, int64_t elems);
 /* masked squared error over channels [0, cu) of pred (ctot channels) and
  * target (cu channels), pixels with mask == 0: grad = 2 (pred - target) *
  * inv_count there, 0 elsewhere; partial[npartial] receives block sums of the

@@ -695,14 +695,26 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
     int t = wg - B.bulk_tile_start;
     // a block whose first tile sits on XCD 0 and whose tile count is a
     // multiple of 8 rows keeps each row's tiles on one XCD
-    if (kp.xgroup && B.xgroup) t = row_group_wg(t, B.xgroup);
+    if (kp.xgroup == 1 && B.xgroup) t = row_group_wg(t, B.xgroup);
     const bool first = t == 0;
-    const int tx = t % B.tiles_x;
-    t /= B.tiles_x;
-    const int ty = t % B.tiles_y;
-    const int z = D == 3 ? 1 + t / B.tiles_y : 0;
-    const int x = tx * kTX + threadIdx.x;
-    const int y = ty * kTY + threadIdx.y;
+    int x, y, z;
+    if (kp.xgroup == 2 && B.rowwg) {
+        // kTY consecutive x tiles of one row: the x-shifted pulls' straddled
+        // lines are shared by waves of the same workgroup
+        const int per_row = B.tiles_x / kTY;
+        const int tr = t % per_row;
+        t /= per_row;
+        y = t % B.n[1];
+        z = D == 3 ? 1 + t / B.n[1] : 0;
+        x = (tr * kTY + (int)threadIdx.y) * kTX + threadIdx.x;
+    } else {
+        const int tx = t % B.tiles_x;
+        t /= B.tiles_x;
+        const int ty = t % B.tiles_y;
+        z = D == 3 ? 1 + t / B.tiles_y : 0;
+        x = tx * kTX + threadIdx.x;
+        y = ty * kTY + threadIdx.y;
+    }
     const int cur = kp.cur_in[b];
     if (first && threadIdx.x == 0 && threadIdx.y == 0) kp.cur_out[b] = B.active ? 1 - cur : cur;
     double acc[kNAcc];

@@ -45,7 +45,8 @@ struct BlockParams {
                               // dealt a row per XCD, else 0
     int32_t tiles_x2;         // FP16 two-node bulk kernel: 128-node tiles per row
     int32_t h2_tile_start;    // its first workgroup of this block
-    int32_t pad2;
+    int32_t rowwg;            // 1: tiles_x is a multiple of kTY and n[1] of kTY,
+                              // so a bulk workgroup can take kTY tiles of one row
     int32_t has_obst;         // the block has obstacle link masks
 };
 
@@ -104,7 +105,10 @@ struct KParams {
     int32_t fold;           // 0 none, 1 simple x faces, 2 every simple region of
                             // the bulk grid run inside the bulk kernel
     int32_t xcd_remap;      // bulk tiles dealt to XCDs in contiguous runs
-    int32_t xgroup;         // 1: deal each block's rows of tiles to one XCD
+    int32_t xgroup;         // bulk workgroup shape: 0 kTX x kTY tiles in order,
+                            // 1 the same with each block's rows of tiles dealt
+                            // to one XCD, 2 kTY x-tiles of one row (blocks with
+                            // rowwg; the others as 0)
     int32_t fused;          // 1: the step is one whole-lattice launch
     int32_t h2;             // 1: FP16 steps use the two-node bulk kernel
     int32_t nbulk2;         // its workgroups

@@ -89,6 +89,7 @@ struct mplbhip_ctx {
     bool xcd_remap = false;    // bulk tiles dealt to XCDs in contiguous runs
     int xgroup = 0;            // 1: rows of bulk tiles dealt to one XCD each
     int xgroup_ok = 0;         // 1 when some block allows that grouping
+    bool rowwg_ok = false;     // some block allows row workgroups (xgroup 2)
     bool fold_env = false, xgroup_env = false;  // fixed by the environment
     std::vector<BndSeg> segs;
     std::vector<std::vector<uint8_t>> solid;  // obstacle nodes per block (empty: none)
@@ -800,6 +801,7 @@ static int assign_tiles(mplbhip_ctx* c, bool only_active) {
                         !c->xcd_remap;
         B.xgroup = ok ? B.tiles_x : 0;
         grouped += ok ? 1 : 0;
+        B.rowwg = B.tiles_x % kTY == 0 && B.n[1] % kTY == 0 ? 1 : 0;
     }
     return grouped;
 }
@@ -914,6 +916,8 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     (void)bulk2;
     c->xgroup = grouped > 0 ? 1 : 0;
     c->xgroup_ok = c->xgroup;
+    c->rowwg_ok = false;
+    for (const BlockParams& B : c->bp) c->rowwg_ok = c->rowwg_ok || B.rowwg;
     // below a quarter million nodes a step is launch-bound: one launch
     c->fused = c->interior_nodes < (1 << 18);
     if (const char* e = std::getenv("MPLBHIP_FUSED")) {
@@ -923,7 +927,9 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     c->fused_full = c->fused;
     if (const char* e = std::getenv("MPLBHIP_BATCH_COMPACT")) c->batch_compact = std::atoi(e) != 0;
     if (const char* e = std::getenv("MPLBHIP_XGROUP")) {
-        if (std::atoi(e) == 0) c->xgroup = 0;
+        const int v = std::atoi(e);
+        if (v == 0) c->xgroup = 0;
+        if (v == 2 && c->rowwg_ok) c->xgroup = 2;
         c->xgroup_env = true;
     }
     if (int rc = setup_halos(c)) return rc;
@@ -1110,40 +1116,54 @@ static int autotune(mplbhip_ctx* c) {
         if (c->nbnd_v[2] != c->nbnd_v[1]) folds.push_back(2);
     }
     std::vector<int> groups;
-    if (c->xgroup_env || c->xgroup_ok == 0) groups = {c->xgroup};
-    else groups = {c->xgroup_ok, 0};
+    if (c->xgroup_env) {
+        groups = {c->xgroup};
+    } else {
+        if (c->xgroup_ok) groups.push_back(1);
+        groups.push_back(0);
+        if (c->rowwg_ok) groups.push_back(2);
+    }
     if (!(c->h2_env && c->h2))
         for (int f : folds)
             for (int g : groups) cand.push_back({f, g, 0});
     if (c->h2_ok && !(c->h2_env && !c->h2)) cand.push_back({0, 0, 1});
     if (cand.size() < 2) return 0;
-    float best = 1e30f;
-    std::array<int, 3> pick = cand[0];
-    for (const auto& cf : cand) {
-        c->fold = cf[0];
-        c->xgroup = cf[1];
-        c->h2 = cf[2] != 0;
-        KParams kp = base_params(c);
-        HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP, kp, c->stream));
-        HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-        for (int i = 0; i < 3; i++)
+    // three interleaved rounds of 5 dry steps per candidate; each keeps its
+    // fastest round (clock and neighbour noise only ever add time)
+    constexpr int kRounds = 3, kSteps = 5;
+    std::vector<float> tmin(cand.size(), 1e30f);
+    for (int r = 0; r < kRounds; r++)
+        for (size_t i = 0; i < cand.size(); i++) {
+            c->fold = cand[i][0];
+            c->xgroup = cand[i][1];
+            c->h2 = cand[i][2] != 0;
+            KParams kp = base_params(c);
             HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP, kp, c->stream));
-        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-        HIPCHK(c, hipEventSynchronize(c->ev[1]));
-        float ms = 0.f;
-        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-        if (ms < best) {
-            best = ms;
-            pick = cf;
+            HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+            for (int k = 0; k < kSteps; k++)
+                HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP, kp, c->stream));
+            HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+            HIPCHK(c, hipEventSynchronize(c->ev[1]));
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            tmin[i] = std::min(tmin[i], ms / kSteps);
         }
-    }
+    size_t ipick = 0;
+    for (size_t i = 1; i < cand.size(); i++)
+        if (tmin[i] < tmin[ipick]) ipick = i;
+    const std::array<int, 3> pick = cand[ipick];
+    const float best = tmin[ipick];
+    if (std::getenv("MPLBHIP_VERBOSE"))
+        for (size_t i = 0; i < cand.size(); i++)
+            std::fprintf(stderr, "mplbhip: autotune fold=%d xgroup=%d h2=%d: %.4f ms/step\n",
+                         cand[i][0], cand[i][1], cand[i][2], tmin[i]);
     c->fold = pick[0];
     c->xgroup = pick[1];
     c->h2 = pick[2] != 0;
     drop_graphs(c);
     if (std::getenv("MPLBHIP_VERBOSE"))
         std::fprintf(stderr, "mplbhip: autotune picked fold=%d xgroup=%d h2=%d (%.4f ms/step)\n",
-                     c->fold, c->xgroup, (int)c->h2, best / 3);
+                     c->fold, c->xgroup, (int)c->h2, best);
     return 0;
 }
 

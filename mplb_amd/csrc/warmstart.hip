@@ -100,47 +100,63 @@ __global__ void k_flip_weights(const float* __restrict__ w, float* __restrict__ 
     }
 }
 
-// Weight gradient partials of one pixel chunk: a wave computes a 32 (rows
-// k = tap*cin + ci) x 32 (co) tile of sum_m A[m][k] dout[m][co], reducing
-// over its pixels 8 at a time; partial[chunk][k][co].
+// Pixel chunks of the weight gradient: whole rows of one image, about
+// MPLBWS_WGRAD_CHUNK pixels each, so that a chunk's pixel coordinates need
+// no division.
+__host__ __device__ inline int wgrad_rows(int h, int wd) {
+    int r = MPLBWS_WGRAD_CHUNK / wd;
+    if (r < 1) r = 1;
+    if (r > h) r = h;
+    return r;
+}
+
+// Weight gradient partials of one chunk: a wave computes a 32 (rows k =
+// tap*cin + ci) x 32 (co) tile of sum_m A[m][k] dout[m][co] over the
+// chunk's pixels, 8 per product (MFMA reduction dimension = pixels); the
+// waves take the chunk's rows in turn and their tiles are summed through
+// LDS in wave order.  partial[chunk][k][co].
 __global__ __launch_bounds__(64 * kWaves) void k_conv_wgrad(const float* __restrict__ in,
                                                             const float* __restrict__ dout,
                                                             float* __restrict__ partial,
-                                                            int n, int h, int wd, int cin,
-                                                            int cout, int chunk) {
+                                                            int h, int wd, int cin, int cout,
+                                                            int rows) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int K = 9 * cin;
     const int k0 = blockIdx.x * 32, co0 = blockIdx.y * 32;
-    const int64_t M = (int64_t)n * h * wd;
-    const int64_t mbeg = (int64_t)blockIdx.z * chunk;
-    const int64_t mend = mbeg + chunk < M ? mbeg + chunk : M;
+    const int per_image = (h + rows - 1) / rows;
+    const int pn = (int)blockIdx.z / per_image;
+    const int r0 = ((int)blockIdx.z % per_image) * rows;
+    const int r1 = r0 + rows < h ? r0 + rows : h;
     const int row = lane & 31, g = lane >> 5;
     const int k = k0 + row;  // A row of this lane
     const bool kvalid = k < K;
     const int tap = kvalid ? k / cin : 0, ci = kvalid ? k % cin : 0;
     const int dy = tap / 3 - 1, dx = tap % 3 - 1;
     const int co = co0 + row;  // B column of this lane
+    const bool cvalid = co < cout;
+    const float* inb = in + (int64_t)pn * h * wd * cin;
+    const float* db = dout + (int64_t)pn * h * wd * cout;
     f32x16 acc = {};
-    // the waves of the workgroup split the chunk in strides of 8 pixels
-    for (int64_t mb = mbeg + 8 * wave; mb < mend; mb += 8 * kWaves) {
-        bf16x4 a, b;
+    for (int py = r0 + wave; py < r1; py += kWaves) {
+        const int yy = py + dy;
+        const bool yin = kvalid && yy >= 0 && yy < h;
+        for (int x0 = 0; x0 < wd; x0 += 8) {
+            bf16x4 a, b;
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const int64_t m = mb + 4 * g + t;
-            float av = 0.f, bv = 0.f;
-            if (m < mend) {
-                const int px = (int)(m % wd), py = (int)((m / wd) % h);
-                const int64_t pn = m / ((int64_t)wd * h);
-                const int yy = py + dy, xx = px + dx;
-                if (kvalid && yy >= 0 && yy < h && xx >= 0 && xx < wd)
-                    av = in[((pn * h + yy) * wd + xx) * cin + ci];
-                if (co < cout) bv = dout[m * cout + co];
+            for (int t = 0; t < 4; t++) {
+                const int px = x0 + 4 * g + t;
+                const int xx = px + dx;
+                float av = 0.f, bv = 0.f;
+                if (px < wd) {
+                    if (yin && xx >= 0 && xx < wd) av = inb[((int64_t)yy * wd + xx) * cin + ci];
+                    if (cvalid) bv = db[((int64_t)py * wd + px) * cout + co];
+                }
+                a[t] = to_bf16(av);
+                b[t] = to_bf16(bv);
             }
-            a[t] = to_bf16(av);
-            b[t] = to_bf16(bv);
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
     }
     // sum the waves' tiles through LDS, then write this chunk's partial
     __shared__ float red[kWaves][32][33];
@@ -169,16 +185,31 @@ __global__ void k_sum_partials(const float* __restrict__ partial, float* __restr
     }
 }
 
-// bias gradient partials: per chunk of pixels, per channel
-__global__ void k_bias_partial(const float* __restrict__ dout, float* __restrict__ partial,
-                               int64_t M, int cout, int chunk) {
-    const int co = threadIdx.x;
-    if (co >= cout) return;
-    const int64_t mbeg = (int64_t)blockIdx.x * chunk;
-    const int64_t mend = mbeg + chunk < M ? mbeg + chunk : M;
+// bias gradient partials over the same chunks: 256 / cout groups of cout
+// threads take the chunk's pixels in turn; the groups' sums are added in
+// group order
+__global__ __launch_bounds__(256) void k_bias_partial(const float* __restrict__ dout,
+                                                      float* __restrict__ partial, int h, int wd,
+                                                      int cout, int rows) {
+    const int t = threadIdx.x;
+    const int groups = 256 / cout;
+    const int grp = t / cout, co = t % cout;
+    const int per_image = (h + rows - 1) / rows;
+    const int pn = (int)blockIdx.x / per_image;
+    const int r0 = ((int)blockIdx.x % per_image) * rows;
+    const int r1 = r0 + rows < h ? r0 + rows : h;
+    const float* db = dout + (int64_t)pn * h * wd * cout;
     float s = 0.f;
-    for (int64_t m = mbeg; m < mend; m++) s += dout[m * cout + co];
-    partial[(int64_t)blockIdx.x * cout + co] = s;
+    if (grp < groups)
+        for (int m = r0 * wd + grp; m < r1 * wd; m += groups) s += db[(int64_t)m * cout + co];
+    __shared__ float red[256];
+    red[t] = s;
+    __syncthreads();
+    if (t < cout) {
+        float tot = 0.f;
+        for (int q = 0; q < groups; q++) tot += red[q * cout + t];
+        partial[(int64_t)blockIdx.x * cout + t] = tot;
+    }
 }
 
 __global__ void k_relu_bwd(const float* __restrict__ dout, const float* __restrict__ out,
@@ -262,6 +293,18 @@ __global__ void k_channels(const float* __restrict__ a, float* __restrict__ b, i
         if (dir == 0) b[p * ctot + c0 + ch] = a[i];
         else if (dir == 1) b[i] = a[p * ctot + c0 + ch];
         else b[i] += a[p * ctot + c0 + ch];
+    }
+}
+
+// dst[i][:] = src[idx[i]][:] for rows of `elems` floats (a mini-batch
+// from a device-resident training set)
+__global__ void k_gather(const float* __restrict__ src, float* __restrict__ dst,
+                         const int32_t* __restrict__ idx, int count, int64_t elems) {
+    const int64_t total = (int64_t)count * elems;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / elems);
+        dst[i] = src[(int64_t)idx[r] * elems + (i - (int64_t)r * elems)];
     }
 }
 
@@ -350,32 +393,33 @@ int mplbws_flip_weights(const float* w, float* wt, int cin, int cout) {
 }
 
 int64_t mplbws_wgrad_workspace(int n, int h, int wd, int cin, int cout) {
-    const int64_t M = (int64_t)n * h * wd;
-    const int64_t chunks = (M + MPLBWS_WGRAD_CHUNK - 1) / MPLBWS_WGRAD_CHUNK;
+    if (n <= 0 || h <= 0 || wd <= 0) return 0;
+    const int rows = wgrad_rows(h, wd);
+    const int64_t chunks = (int64_t)n * ((h + rows - 1) / rows);
     return chunks * 9LL * cin * cout + chunks * cout;
 }
 
 int mplbws_conv3x3_wgrad(const float* in, const float* dout, float* dw, float* dbias,
                          float* workspace, int n, int h, int wd, int cin, int cout) {
     if (!in || !dout || !dw || !workspace || n <= 0 || h <= 0 || wd <= 0 || cin <= 0 ||
-        cout <= 0)
+        cout <= 0 || (dbias && cout > 256))
         return MPLBWS_ERR_ARG;
-    const int64_t M = (int64_t)n * h * wd;
-    const int chunks = (int)((M + MPLBWS_WGRAD_CHUNK - 1) / MPLBWS_WGRAD_CHUNK);
+    const int rows = wgrad_rows(h, wd);
+    const int64_t chunks = (int64_t)n * ((h + rows - 1) / rows);
+    if (chunks > 65535) return MPLBWS_ERR_ARG;
     const int K = 9 * cin;
     const dim3 grd((unsigned)((K + 31) / 32), (unsigned)((cout + 31) / 32), (unsigned)chunks);
-    hipLaunchKernelGGL(k_conv_wgrad, grd, dim3(64 * kWaves), 0, 0, in, dout, workspace, n, h, wd,
-                       cin, cout, MPLBWS_WGRAD_CHUNK);
+    hipLaunchKernelGGL(k_conv_wgrad, grd, dim3(64 * kWaves), 0, 0, in, dout, workspace, h, wd,
+                       cin, cout, rows);
     const int64_t cnt = (int64_t)K * cout;
     hipLaunchKernelGGL(k_sum_partials, dim3(grid_for(cnt)), dim3(256), 0, 0, workspace, dw, cnt,
-                       chunks);
+                       (int)chunks);
     if (dbias) {
-        float* bp = workspace + (int64_t)chunks * cnt;
-        if (cout > 256) return MPLBWS_ERR_ARG;
-        hipLaunchKernelGGL(k_bias_partial, dim3(chunks), dim3(256), 0, 0, dout, bp, M, cout,
-                           MPLBWS_WGRAD_CHUNK);
+        float* bp = workspace + chunks * cnt;
+        hipLaunchKernelGGL(k_bias_partial, dim3((unsigned)chunks), dim3(256), 0, 0, dout, bp, h,
+                           wd, cout, rows);
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, 0, bp, dbias, (int64_t)cout,
-                           chunks);
+                           (int)chunks);
     }
     return status();
 }
@@ -419,6 +463,14 @@ int mplbws_channels(const float* src, float* dst, int64_t pixels, int c, int cto
     if (!src || !dst || c0 < 0 || c0 + c > ctot) return MPLBWS_ERR_ARG;
     hipLaunchKernelGGL(k_channels, dim3(grid_for(pixels * c)), dim3(256), 0, 0, src, dst, pixels,
                        c, ctot, c0, direction);
+    return status();
+}
+
+int mplbws_gather(const float* src, float* dst, const int32_t* idx, int count,
+                  int64_t elems) {
+    if (!src || !dst || !idx || count < 0 || elems < 0) return MPLBWS_ERR_ARG;
+    hipLaunchKernelGGL(k_gather, dim3(grid_for((int64_t)count * elems)), dim3(256), 0, 0, src,
+                       dst, idx, count, elems);
     return status();
 }
 

@@ -1,6 +1,6 @@
 """Learned warm start for batched 2D cavities with obstacles.
 
-A U-Net (about 212k parameters) maps (obstacle mask, log Re, lid speed, lid
+A U-Net (563,688 parameters in the default widths) maps (obstacle mask, log Re, lid speed, lid
 row, node coordinates) on an n x n cavity to the steady flow (u, v and the
 density deviation); the lattice then starts from that flow lifted to
 populations (mplbhip_init_lift: feq plus the first-order Chapman-Enskog
@@ -31,6 +31,7 @@ LID = 0.05                     # lattice lid speed of the cavities
 # small part of the lift) weighs less than the velocity in the loss
 RHO_SCALE = 1.0 / (30 * LID * LID)
 IN_CH = 8                      # input channels (6 used, padded to 8)
+LOSS_PARTIALS = 1024           # partial sums of one loss evaluation
 OUT_CH = 8                     # output channels (3 used)
 
 
@@ -57,6 +58,7 @@ def load():
         "mplbws_up2_fwd": [P, P, I, I, I, I, I, I],
         "mplbws_up2_bwd": [P, P, I, I, I, I, I, I],
         "mplbws_channels": [P, P, L, I, I, I, I],
+        "mplbws_gather": [P, P, P, I, L],
         "mplbws_mse": [P, P, P, P, P, I, L, I, I, F],
         "mplbws_adam": [P, P, P, P, L, F, F, F, F, I],
         "mplbws_sync": [],
@@ -95,7 +97,11 @@ class DeviceArray:
         return d
 
     def upload(self, a):
-        a = np.ascontiguousarray(a, dtype=np.float32)
+        """Copy a host array in: FP32 values, or int32 ones stored as they
+        are (index arrays for mplbws_gather)."""
+        a = np.ascontiguousarray(a)
+        if a.dtype != np.int32:
+            a = a.astype(np.float32, copy=False)
         if a.size != self.size:
             raise WarmStartError("upload of %d values into %d" % (a.size, self.size))
         _ck(self.lib.mplbws_upload(self.p, a.ctypes.data, a.size * 4), "upload")
@@ -129,14 +135,19 @@ def _p(t):
     return t.p
 
 
+def _at(t, offset):
+    """Device pointer offset by `offset` 4-byte elements into t."""
+    return ctypes.c_void_p(t.p.value + 4 * int(offset))
+
+
 # ---- the network -----------------------------------------------------------
 # Encoder channels per level (each level: two 3x3 conv + ReLU, then 2x2
 # average pooling), a bottleneck of two conv + ReLU, and a mirrored decoder
 # (nearest 2x upsampling, concatenation with the encoder level, two conv +
 # ReLU), then a linear 3x3 output conv.  Four levels give a receptive field
 # wider than a 128^2 cavity, whose flow is global.
-ENC = (16, 32, 48, 48)
-BOTTLENECK = 48
+ENC = (24, 48, 64, 64)
+BOTTLENECK = 64
 
 
 def layer_specs(enc=ENC, bott=BOTTLENECK):
@@ -213,7 +224,9 @@ class UNet:
             wf = max(wf, 9 * ci * co)
         self.workspace = DeviceArray((int(ws),))
         self.wflip = DeviceArray((wf,))
-        self.lossbuf = DeviceArray((1024,), zero=True)
+        self.lossbuf = DeviceArray((LOSS_PARTIALS,), zero=True)
+        self._tgt = DeviceArray((batch, n, n, 3))
+        self._msk = DeviceArray((batch, n, n))
 
     # -- building blocks
     def _conv(self, name, src, dst):
@@ -236,10 +249,12 @@ class UNet:
                                             s, s, co, ci, 0), "dgrad")
 
     def forward(self, x):
-        """x: (batch, n, n, IN_CH) host array; returns the output activation
+        """x: (batch, n, n, IN_CH) host array, or None when the input
+        activation is already on the device; returns the output activation
         (device, (batch, n, n, OUT_CH), channels 0..2 used)."""
         a, L, n, B, nl = self.act, self.lib, self.n, self.batch, len(self.enc)
-        a["x"].upload(x)
+        if x is not None:
+            a["x"].upload(x)
         src = a["x"]
         for k, c in enumerate(self.enc):
             self._conv("e%da" % k, src, a["e%da" % k])
@@ -292,19 +307,22 @@ class UNet:
         weighted by RHO_WEIGHT through its target scale).  target (batch, n,
         n, 3) and mask (batch, n, n) are host arrays.  Returns (loss,
         gradient at the output, device)."""
-        pixels = self.batch * self.n * self.n
         fluid = float((np.asarray(mask) == 0).sum())
         inv = 1.0 / max(fluid * 3.0, 1.0)
-        if not hasattr(self, "_tgt"):
-            self._tgt = DeviceArray((self.batch, self.n, self.n, 3))
-            self._msk = DeviceArray((self.batch, self.n, self.n))
         self._tgt.upload(target)
         self._msk.upload(mask)
-        nb = self.lossbuf.numel()
-        _ck(self.lib.mplbws_mse(_p(self.act["out"]), _p(self._tgt), _p(self._msk), _p(self.gout),
-                                _p(self.lossbuf), nb, pixels, 3, OUT_CH, inv), "mse")
+        self.loss_grad_device(inv, _p(self.lossbuf))
         loss = float(self.lossbuf.numpy().astype(np.float64).sum()) * inv
         return loss, self.gout
+
+    def loss_grad_device(self, inv, partial):
+        """The loss of the targets and masks already in self._tgt and
+        self._msk: gradient into self.gout, LOSS_PARTIALS squared-error
+        partial sums at the device pointer `partial` (their sum times inv is
+        the loss); nothing is read back."""
+        pixels = self.batch * self.n * self.n
+        _ck(self.lib.mplbws_mse(_p(self.act["out"]), _p(self._tgt), _p(self._msk), _p(self.gout),
+                                partial, LOSS_PARTIALS, pixels, 3, OUT_CH, inv), "mse")
 
     def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
         self.step_count += 1
@@ -427,27 +445,42 @@ def predict(net, x):
 
 def train(net, x, y, epochs, lr=2e-3, seed=0, log=None):
     """Adam on the masked MSE; x, y host arrays; returns the loss history
-    (one mean per epoch).  The learning rate follows a cosine schedule."""
+    (one mean per epoch).  The learning rate follows a cosine schedule.
+    The training set stays on the device: each step gathers its mini-batch
+    there, and the losses are read back once per epoch."""
     rng = np.random.default_rng(seed)
-    B = net.batch
-    steps_per_epoch = len(x) // B
+    lib, B, n = net.lib, net.batch, net.n
+    count = len(x)
+    steps_per_epoch = count // B
     total = epochs * steps_per_epoch
+    dx = DeviceArray.from_numpy(x)
+    dy = DeviceArray.from_numpy(y)
+    dm = DeviceArray.from_numpy(x[..., 0])
+    fluid = (x[..., 0] == 0).sum(axis=(1, 2)).astype(np.float64)
+    dperm = DeviceArray((max(count, 1),))
+    parts = DeviceArray((max(steps_per_epoch, 1) * LOSS_PARTIALS,), zero=True)
     hist = []
     it = 0
     for ep in range(epochs):
-        perm = rng.permutation(len(x))
-        acc = 0.0
+        perm = rng.permutation(count).astype(np.int32)
+        dperm.upload(perm)
+        invs = []
         for s in range(steps_per_epoch):
-            idx = np.sort(perm[s * B:(s + 1) * B])
-            xb = x[idx]
-            net.forward(xb)
-            loss, g = net.loss_grad(y[idx], xb[..., 0])
-            net.backward(g)
+            idx = _at(dperm, s * B)
+            _ck(lib.mplbws_gather(_p(dx), _p(net.act["x"]), idx, B, n * n * IN_CH), "gather")
+            _ck(lib.mplbws_gather(_p(dy), _p(net._tgt), idx, B, n * n * 3), "gather")
+            _ck(lib.mplbws_gather(_p(dm), _p(net._msk), idx, B, n * n), "gather")
+            inv = 1.0 / max(fluid[perm[s * B:(s + 1) * B]].sum() * 3.0, 1.0)
+            invs.append(inv)
+            net.forward(None)
+            net.loss_grad_device(inv, _at(parts, s * LOSS_PARTIALS))
+            net.backward(net.gout)
             rate = lr * 0.5 * (1 + math.cos(math.pi * it / max(total, 1)))
             net.adam(lr=rate)
-            acc += loss
             it += 1
-        hist.append(acc / max(steps_per_epoch, 1))
+        sums = parts.numpy().astype(np.float64).reshape(-1, LOSS_PARTIALS).sum(axis=1)
+        losses = sums[:steps_per_epoch] * np.array(invs)
+        hist.append(float(losses.mean()) if steps_per_epoch else 0.0)
         if log:
             log("epoch %d loss %.4e" % (ep, hist[-1]))
     return hist
@@ -465,8 +498,9 @@ def velocity_l2(g_a, g_b, count):
     return np.array(out)
 
 
-def run(n=128, n_train=4096, n_test=512, epochs=100, criteria=1e-8, check_period=1000,
-        seed=0x4D504C42, batch=32, chunk=512, storage=hip.STORE_F32, restart=10.0, log=print):
+def run(n=128, n_train=4096, n_test=512, epochs=200, criteria=1e-8, check_period=1000,
+        seed=0x4D504C42, batch=32, chunk=512, storage=hip.STORE_F32, restart=1e3,
+        enc=ENC, bott=BOTTLENECK, diagnose=False, log=print):
     """Time-to-residual of learned versus rest initialisation.
 
     1. Training targets: n_train cavities (Re log-uniform in [100, 1000],
@@ -477,7 +511,10 @@ def run(n=128, n_train=4096, n_test=512, epochs=100, criteria=1e-8, check_period
        and the learned one includes inference.  Fallback: an instance whose
        residual rises above `restart` times its lowest restarts at rest
        inside the batch; one still unconverged when the budget ends is
-       re-run from rest and that time is added."""
+       re-run from rest and that time is added.
+    With diagnose, two more solves split the learned start's error: the
+    converged velocity with the predicted density, and the predicted
+    velocity with the converged density."""
     t_all = time.perf_counter()
     res, rects = case_params(n_train + n_test, seed, n=n)
     x_all = inputs(n, res, rects)
@@ -497,7 +534,7 @@ def run(n=128, n_train=4096, n_test=512, epochs=100, criteria=1e-8, check_period
     t_data = time.perf_counter() - t0
     # 2. training
     t0 = time.perf_counter()
-    net = UNet(n=n, batch=batch, seed=seed & 0xFFFF)
+    net = UNet(n=n, batch=batch, seed=seed & 0xFFFF, enc=enc, bott=bott)
     hist = train(net, x_all[:n_train], y_train, epochs, log=log)
     mplbws_sync = load().mplbws_sync
     mplbws_sync()
@@ -532,15 +569,33 @@ def run(n=128, n_train=4096, n_test=512, epochs=100, criteria=1e-8, check_period
             s_warm[b] = s_fb[k]
     g_warm.update_macros()
     l2 = velocity_l2(g_warm, g_rest, n_test)
-    pred_err = []
+    pred_err, rho_err = [], []
     lid = cases.CS * LID
+    exact = [np.stack([g_rest.get_macro(b, v) for b in range(n_test)]) for v in range(3)]
     for b in range(n_test):
-        u, v = g_rest.get_macro(b, 1) / lid, g_rest.get_macro(b, 2) / lid
+        u, v = exact[1][b] / lid, exact[2][b] / lid
         num = np.sqrt(((pred[b, :, :, 0] - u) ** 2 + (pred[b, :, :, 1] - v) ** 2).sum())
         pred_err.append(float(num / np.sqrt((u ** 2 + v ** 2).sum())))
+        fluid = masks[b] == 0
+        dr = exact[0][b][fluid] - exact[0][b][fluid].mean()
+        dp = init[0][b][fluid] - init[0][b][fluid].mean()
+        rho_err.append(float(np.sqrt(((dp - dr) ** 2).sum() / (dr ** 2).sum())))
+    diag = {}
+    if diagnose:
+        for name, start in (("exact_velocity_predicted_density", (init[0], exact[1], exact[2])),
+                            ("predicted_velocity_exact_density", (exact[0], init[1], init[2]))):
+            gd, sd, _, td = solve(n, tr, tq, criteria, check_period, budget, storage=storage,
+                                  init=start)
+            gd.close()
+            diag[name] = {"seconds": round(td, 3), "steps_median": float(np.median(sd)),
+                          "unconverged": int((sd < 0).sum()),
+                          "per_case_step_ratio_median":
+                              float(np.median(s_rest / np.maximum(sd, 1)))}
+            log("diagnosis %s: %s" % (name, diag[name]))
     out = {
         "n": n, "train_cases": n_train, "test_cases": n_test, "criteria": criteria,
-        "check_period": check_period, "params": param_count(), "epochs": epochs,
+        "check_period": check_period, "params": param_count(layer_specs(enc, bott)),
+        "epochs": epochs,
         "loss_first": hist[0], "loss_last": hist[-1],
         "data_seconds": round(t_data, 2), "data_steps_median": float(np.median(data_steps)),
         "train_seconds": round(t_train, 2),
@@ -554,6 +609,9 @@ def run(n=128, n_train=4096, n_test=512, epochs=100, criteria=1e-8, check_period
         "per_case_step_ratio_median": float(np.median(s_rest / np.maximum(s_warm, 1))),
         "work_ratio_sum_steps": float(s_rest.sum() / max(s_warm.sum(), 1)),
         "prediction_rel_l2_median": float(np.median(pred_err)),
+        "prediction_density_rel_l2_median": float(np.median(rho_err)),
+        "network": {"encoder": list(enc), "bottleneck": bott,
+                    "params": param_count(layer_specs(enc, bott))},
         "velocity_rel_l2_learned_vs_rest_median": float(np.median(l2)),
         "velocity_rel_l2_learned_vs_rest_max": float(l2.max()),
         "total_seconds": round(time.perf_counter() - t_all, 1),
@@ -564,6 +622,8 @@ def run(n=128, n_train=4096, n_test=512, epochs=100, criteria=1e-8, check_period
                            "prediction_rel_l2": round(pred_err[b], 4)}
                           for b in np.argsort(-s_rest)[:5]],
     }
+    if diag:
+        out["diagnosis"] = diag
     g_rest.close()
     g_warm.close()
     return out, net

@@ -121,6 +121,20 @@ def test_f64_box3d_nonuniform():
     assert compare(g, o, "box3d") == 0.0
 
 
+def test_f64_row_workgroups_bitexact(monkeypatch):
+    """The bulk kernel's row-workgroup shape (four 64-node x tiles of one row
+    per workgroup), which needs rows of a multiple of 256 nodes: a 256 x 8 x
+    12 box, split step, against the oracle."""
+    monkeypatch.setenv("MPLBHIP_FUSED", "0")
+    monkeypatch.setenv("MPLBHIP_XGROUP", "2")
+    log("box 256x8x12 F64, row workgroups")
+    g, o, _ = pair(3, lambda be: cases.cavity3d(be, 12, re=150.0, noise=True,
+                                                size=(256, 8, 12)))
+    g.step(9)
+    o.step(9)
+    assert compare(g, o, "rowwg") == 0.0
+
+
 def test_f64_lchannel_multiblock_bitexact():
     """three blocks, VirtualBoundary halos, extrapolation outlet"""
     log("lchannel n=9 F64")

@@ -61,7 +61,8 @@ def test_oracle_bf16_rounding():
 
 
 def test_network_size():
-    assert 250_000 <= ws.param_count() <= 350_000
+    # 24-48-64-64 encoder, 64-channel bottleneck: 9 ci co + co per conv
+    assert ws.param_count() == 563_688
     # every convolution input has a multiple of 8 channels (the MFMA K step)
     for name, ci, co, relu, lev in ws.LAYERS:
         assert ci % 8 == 0, name
@@ -94,7 +95,9 @@ def _empty(shape, zero=False):
 @pytest.mark.gpu
 @pytest.mark.parametrize("n,h,w,ci,co,relu", [(2, 5, 6, 8, 16, 1), (1, 8, 8, 16, 8, 0),
                                                 (3, 4, 9, 64, 64, 1), (1, 6, 6, 128, 32, 0),
-                                                (2, 16, 16, 32, 16, 1)])
+                                                (2, 16, 16, 32, 16, 1),
+                                                # several row chunks per image, a ragged row
+                                                (2, 40, 130, 16, 16, 1)])
 def test_gpu_conv_fwd_dgrad_wgrad(n, h, w, ci, co, relu):
     print("conv %s" % ((n, h, w, ci, co, relu),), flush=True)
     lib = ws.load()
@@ -152,6 +155,19 @@ def test_gpu_pool_up_channels():
     gy = _empty((2, 4, 3, 4))
     assert lib.mplbws_up2_bwd(ws._p(_t(z)), ws._p(gy), 2, 4, 3, 4, 7, 2) == 0
     assert abs(float((nno.up2(y) * z[..., 2:6]).sum()) - float((y * _np(gy)).sum())) < 1e-4
+
+
+@pytest.mark.gpu
+def test_gpu_gather_rows():
+    lib = ws.load()
+    rng = np.random.default_rng(5)
+    src = rng.standard_normal((9, 3, 5, 4)).astype(np.float32)
+    idx = np.array([7, 0, 7, 3, 8], np.int32)
+    S, I = _t(src), ws.DeviceArray((5,))
+    I.upload(idx)
+    dst = _empty((5, 3, 5, 4))
+    assert lib.mplbws_gather(ws._p(S), ws._p(dst), ws._p(I), 5, 60) == 0
+    assert np.array_equal(_np(dst), src[idx])
 
 
 def _cpu_unet(params, x, target, mask, enc, bott, rb=True):
