@@ -324,8 +324,6 @@ __device__ __forceinline__ int logical_wg(int wg, int n) {
     return (wg & 7) * (n >> 3) + (wg >> 3);
 }
 
-// block of logical workgroup t: the last block whose first workgroup
-// (field `first`) is <= t
 // workgroup -> tile such that the tx tiles of one row go to the same XCD
 // (workgroups b, b+8, b+16, ... share an XCD) while consecutive rows are
 // spread over the 8 XCDs; a bijection when the grid is a multiple of 8*tx
@@ -334,12 +332,35 @@ __device__ __forceinline__ int row_group_wg(int b, int tx) {
     return ((j / tx) * 8 + (b & 7)) * tx + j % tx;
 }
 
+// Block parameters and parities are never written while a step kernel
+// runs.  Read through the constant address space, a wave-uniform index
+// turns their loads into scalar loads: one batched round trip per wave
+// instead of a chain of dependent vector loads ahead of the population
+// loads.
+#define MPLB_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ const MPLB_CONST T* const_view(const T* p) {
+    return (const MPLB_CONST T*)(p);
+}
+
+__device__ __forceinline__ BlockParams load_block(const KParams& kp, int b) {
+    BlockParams B;
+    __builtin_memcpy(&B, (const BlockParams*)&const_view(kp.blocks)[b], sizeof B);
+    return B;
+}
+
+__device__ __forceinline__ int load_parity(const KParams& kp, int b) {
+    return const_view(kp.cur_in)[b];
+}
+
+// block of logical workgroup t: the last block whose first workgroup
+// (field `first`) is <= t
 template <int32_t BlockParams::*first>
 __device__ __forceinline__ int find_block(const KParams& kp, int t) {
     int lo = 0, hi = kp.nblocks - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (kp.blocks[mid].*first <= t) lo = mid; else hi = mid - 1;
+        if (const_view(kp.blocks)[mid].*first <= t) lo = mid; else hi = mid - 1;
     }
     return lo;
 }
@@ -492,7 +513,10 @@ __device__ __forceinline__ void finish_node(const KParams& kp, const BlockParams
                     gn += B.fforce * wq * rho * cg;
                 }
                 E* p = at_u(next, (int64_t)q * B.sq, (uint32_t)node);
-                if constexpr (ST == MPLBHIP_STORE_F16) *p = __float2half_rn(gn * kp.inv_scale[q]);
+                if constexpr (ST == MPLBHIP_STORE_F16)
+                    __builtin_nontemporal_store(
+                        __half_as_ushort(__float2half_rn(gn * kp.inv_scale[q])),
+                        reinterpret_cast<unsigned short*>(p));
                 else __builtin_nontemporal_store(gn, p);
             }
         }
@@ -517,7 +541,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
     if constexpr (SCOPE == SCOPE_ALL) {
         wg = logical_wg(blockIdx.x, gridDim.x);
         b = find_block<&BlockParams::tile_start>(kp, wg);
-        const BlockParams& B = kp.blocks[b];
+        const BlockParams B = load_block(kp, b);
         int t = wg - B.tile_start;
         const int tx = t % B.tiles_x;
         t /= B.tiles_x;
@@ -528,7 +552,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
         active = x < B.n[0] && y < B.n[1] && B.active;
         if constexpr (MODE == MODE_STEP || MODE == MODE_STEP_CHECK) {
             if (wg == B.tile_start && threadIdx.x == 0 && threadIdx.y == 0)
-                kp.cur_out[b] = B.active ? 1 - kp.cur_in[b] : kp.cur_in[b];
+                kp.cur_out[b] = B.active ? 1 - load_parity(kp, b) : load_parity(kp, b);
         }
     } else {
         wg = blockIdx.x;
@@ -536,10 +560,10 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
         int lo = 0, hi = kp.nblocks - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (kp.blocks[mid].bnd_wg_start[v] <= wg) lo = mid; else hi = mid - 1;
+            if (const_view(kp.blocks)[mid].bnd_wg_start[v] <= wg) lo = mid; else hi = mid - 1;
         }
         b = lo;
-        const BlockParams& B = kp.blocks[b];
+        const BlockParams B = load_block(kp, b);
         const int i = (wg - B.bnd_wg_start[v]) * kBndWG + threadIdx.y * kTX + threadIdx.x;
         int s = B.seg_start[v];
         const int send = B.seg_start[v] + B.seg_count[v];
@@ -555,10 +579,10 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
             z = S.org[2] + r / S.ext[1];
         }
     }
-    const BlockParams& B = kp.blocks[b];
+    const BlockParams B = load_block(kp, b);
     // buffers of this block: steps read the latest fStage and write the
     // other; MATERIALIZE re-streams from the one before the latest
-    const int cur = kp.cur_in[b];
+    const int cur = load_parity(kp, b);
     const int src = MODE == MODE_MATERIALIZE ? 1 - cur : cur;
     const void* fs_prev = kp.fs_buf[src];
     const void* per_prev = kp.per_buf[src];
@@ -691,7 +715,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
     constexpr int Q = D == 2 ? 9 : 19;
     const int wg = kp.xcd_remap ? logical_wg(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int b = find_block<&BlockParams::bulk_tile_start>(kp, wg);
-    const BlockParams& B = kp.blocks[b];
+    const BlockParams B = load_block(kp, b);
     int t = wg - B.bulk_tile_start;
     // a block whose first tile sits on XCD 0 and whose tile count is a
     // multiple of 8 rows keeps each row's tiles on one XCD
@@ -715,7 +739,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
         x = tx * kTX + threadIdx.x;
         y = ty * kTY + threadIdx.y;
     }
-    const int cur = kp.cur_in[b];
+    const int cur = load_parity(kp, b);
     if (first && threadIdx.x == 0 && threadIdx.y == 0) kp.cur_out[b] = B.active ? 1 - cur : cur;
     double acc[kNAcc];
 #pragma unroll
@@ -927,7 +951,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk_h2(KParams kp) {
     constexpr int Q = D == 2 ? 9 : 19;
     const int wg = (int)blockIdx.x;
     const int b = find_block<&BlockParams::h2_tile_start>(kp, wg);
-    const BlockParams& B = kp.blocks[b];
+    const BlockParams B = load_block(kp, b);
     int t = wg - B.h2_tile_start;
     const bool first = t == 0;
     const int tx = t % B.tiles_x2;
@@ -937,7 +961,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk_h2(KParams kp) {
     const int lane = threadIdx.x;
     const int x0 = tx * 2 * kTX + 2 * lane;
     const int y = ty * kTY + threadIdx.y;
-    const int cur = kp.cur_in[b];
+    const int cur = load_parity(kp, b);
     if (first && lane == 0 && threadIdx.y == 0) kp.cur_out[b] = B.active ? 1 - cur : cur;
     if (!B.active) return;
     const bool yin = y >= 1 && y < B.n[1] - 1;

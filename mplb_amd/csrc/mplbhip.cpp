@@ -1148,9 +1148,11 @@ static int autotune(mplbhip_ctx* c) {
             HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
             tmin[i] = std::min(tmin[i], ms / kSteps);
         }
+    // the first candidate within 1% of the fastest: near-ties go to the
+    // earlier (simpler) form, so repeated runs pick the same one
+    const float fastest = *std::min_element(tmin.begin(), tmin.end());
     size_t ipick = 0;
-    for (size_t i = 1; i < cand.size(); i++)
-        if (tmin[i] < tmin[ipick]) ipick = i;
+    while (tmin[ipick] > 1.01f * fastest) ipick++;
     const std::array<int, 3> pick = cand[ipick];
     const float best = tmin[ipick];
     if (std::getenv("MPLBHIP_VERBOSE"))
