@@ -217,6 +217,56 @@ def bench_cavity2d(args):
     }
 
 
+def bench_unet(args):
+    """Training throughput of the warm-start U-Net alone (the C5 training
+    path): mini-batches of 32 synthetic 128^2 inputs and targets resident
+    on the device, forward + backward + Adam per step; the roofline is the
+    convolutions' dense BF16 matrix work against the matrix-core peak."""
+    from mplb_amd import warmstart
+    n, B = args.ws_n, 32
+    enc = tuple(int(c) for c in args.ws_enc.split(","))
+    rng = np.random.default_rng(0)
+    # a pool of at most 32 mini-batches, cycled as epochs
+    pool = min(max(args.steps, 1), 32)
+    epochs = max(1, round(args.steps / pool))
+    steps = epochs * pool
+    count = B * pool
+    x = rng.standard_normal((count, n, n, warmstart.IN_CH)).astype(np.float32)
+    x[..., 0] = (rng.random((count, n, n)) < 0.05).astype(np.float32)
+    y = (0.1 * rng.standard_normal((count, n, n, 3))).astype(np.float32)
+    net = warmstart.UNet(n=n, batch=B, seed=1, enc=enc, bott=args.ws_bott)
+    if args.warmup > 0:
+        w = min(args.warmup, pool)
+        warmstart.train(net, x[:B * w], y[:B * w], 1)
+    sync = warmstart.load().mplbws_sync
+    sync()
+    t0 = time.perf_counter()
+    warmstart.train(net, x, y, epochs)
+    sync()
+    wall = time.perf_counter() - t0
+    ms = wall / steps * 1e3
+    layers = warmstart.layer_specs(enc, args.ws_bott)
+    # forward, data gradient (not for the first layer) and weight gradient
+    flops = sum(2.0 * 9 * ci * co * B * (n >> lev) ** 2 * (2 if name == "e0a" else 3)
+                for name, ci, co, _, lev in layers)
+    tflops = flops / (ms * 1e-3) / 1e12
+    log("bench: U-Net training %.3f ms/step, %.1f TFLOP/s of convolution work" % (ms, tflops))
+    return {
+        "metric": "U-Net training throughput", "value": round(B / (ms * 1e-3), 1),
+        "unit": "images/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 (fp32 accumulation)", "data": "synthetic",
+        "config": {"workload": "warm-start U-Net training step, 32 images of %d^2, "
+                               "encoder %s, bottleneck %d, %d parameters"
+                               % (n, "-".join(map(str, enc)), args.ws_bott,
+                                  warmstart.param_count(layers))},
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 5), "traffic": None,
+                     "kernel": "whole step (convolutions, pooling, loss, Adam)"},
+        "cpu_baseline": None,
+    }
+
+
 def bench_warmstart(args):
     """Learned warm start on 128^2 cavities with 0-3 rectangular obstacles:
     training data from the batched solver, U-Net training on the matrix
@@ -224,8 +274,10 @@ def bench_warmstart(args):
     held-out cases."""
     from mplb_amd import warmstart
     enc = tuple(int(c) for c in args.ws_enc.split(","))
+    coarse = tuple((int(k), float(w)) for k, w in
+                   (item.split(":") for item in args.ws_coarse.split(",") if item))
     res, _ = warmstart.run(n=args.ws_n, n_train=args.ws_train, n_test=args.ws_test,
-                           epochs=args.ws_epochs, enc=enc, bott=args.ws_bott,
+                           epochs=args.ws_epochs, enc=enc, bott=args.ws_bott, coarse=coarse,
                            diagnose=args.ws_diagnose, log=log)
     log("bench: warm start %s" % res)
     return {
@@ -259,10 +311,12 @@ def main():
     ap.add_argument("--ws-epochs", type=int, default=200)
     ap.add_argument("--ws-enc", default="24,48,64,64", help="encoder widths per level")
     ap.add_argument("--ws-bott", type=int, default=64, help="bottleneck width")
+    ap.add_argument("--ws-coarse", default="2:4,4:4",
+                    help="large-scale loss terms, pooling-levels:weight pairs ('' for none)")
     ap.add_argument("--ws-diagnose", action="store_true",
                     help="also solve from exact velocity / exact density splits")
     ap.add_argument("--workload", choices=("cavity3d", "cavity2d", "lchannel", "batch",
-                                           "warmstart"),
+                                           "warmstart", "unet"),
                     default="cavity3d",
                     help="cavity3d: the headline line; lchannel and batch: the "
                          "multi-block and batched workloads")
@@ -288,6 +342,9 @@ def main():
         return
     if args.workload == "warmstart":
         print(json.dumps(bench_warmstart(args)), flush=True)
+        return
+    if args.workload == "unet":
+        print(json.dumps(bench_unet(args)), flush=True)
         return
 
     n, K, W = args.n, args.steps, args.warmup

@@ -54,6 +54,10 @@ int mplbws_up2_bwd(const float* dout, float* din, int n, int h, int wd, int c, i
  * 2: dst += src[:, c0:c0+c] */
 int mplbws_channels(const float* src, float* dst, int64_t pixels, int c, int ctot, int c0,
                     int direction);
+/* dst[p][ch] = src[p][ch] where mask[p] == 0 and ch < cu, else 0 (c
+ * channels; src and dst may be the same array) */
+int mplbws_mask_channels(const float* src, const float* mask, float* dst, int64_t pixels, int c,
+                         int cu);
 /* dst[i][0 .. elems) = src[idx[i]][0 .. elems) for i < count (idx on the
  * device; the caller keeps every idx[i] inside src) */
 int mplbws_gather(const float* src, float* dst, const int32_t* idx, int count, int64_t elems);

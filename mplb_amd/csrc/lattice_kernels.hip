@@ -1081,35 +1081,61 @@ __global__ void k_halo(HaloParams hp) {
     }
 }
 
-// deterministic sums of per-workgroup partials: out[b*kNAcc + a] per block,
-// out[nblocks*kNAcc + a] over all blocks (block order).  After a split step
-// a block's bulk workgroups come first, then its boundary workgroups; after
-// a whole-lattice launch, its tiles.
-__global__ void k_reduce_partials(const double* partials, const BlockParams* blocks,
-                                  int nblocks, int ntiles, int nbulk, int nbnd, int split,
-                                  int fold, double* out) {
-    const int a = threadIdx.x;
-    if (a >= kNAcc) return;
-    double total = 0.0;
-    for (int b = 0; b < nblocks; b++) {
-        const bool last = b + 1 == nblocks;
-        double s = 0.0;
-        if (split) {
-            const int t0 = blocks[b].bulk_tile_start;
-            const int t1 = last ? nbulk : blocks[b + 1].bulk_tile_start;
-            for (int t = t0; t < t1; t++) s += partials[(int64_t)t * kNAcc + a];
-            const int w0 = blocks[b].bnd_wg_start[fold];
-            const int w1 = last ? nbnd : blocks[b + 1].bnd_wg_start[fold];
-            for (int w = w0; w < w1; w++) s += partials[(int64_t)(nbulk + w) * kNAcc + a];
-        } else {
-            const int t0 = blocks[b].tile_start;
-            const int t1 = last ? ntiles : blocks[b + 1].tile_start;
-            for (int t = t0; t < t1; t++) s += partials[(int64_t)t * kNAcc + a];
-        }
-        out[b * kNAcc + a] = s;
-        total += s;
+// Per-block sums of the per-workgroup partials, one workgroup per block:
+// the threads stride over the block's slots (its bulk tiles, then its
+// boundary workgroups; or its whole-lattice tiles), and an LDS tree adds
+// the threads' sums; both orders are fixed, so the sums are reproducible.
+__global__ __launch_bounds__(256) void k_reduce_blocks(const double* __restrict__ partials,
+                                                       const BlockParams* __restrict__ blocks,
+                                                       int nblocks, int ntiles, int nbulk,
+                                                       int nbnd, int split, int fold,
+                                                       double* __restrict__ out) {
+    const int b = blockIdx.x;
+    const bool last = b + 1 == nblocks;
+    int lo0, hi0, lo1 = 0, hi1 = 0;
+    if (split) {
+        lo0 = blocks[b].bulk_tile_start;
+        hi0 = last ? nbulk : blocks[b + 1].bulk_tile_start;
+        lo1 = nbulk + blocks[b].bnd_wg_start[fold];
+        hi1 = nbulk + (last ? nbnd : blocks[b + 1].bnd_wg_start[fold]);
+    } else {
+        lo0 = blocks[b].tile_start;
+        hi0 = last ? ntiles : blocks[b + 1].tile_start;
     }
-    out[nblocks * kNAcc + a] = total;
+    const int n0 = hi0 - lo0, count = n0 + (hi1 - lo1);
+    __shared__ double red[256];
+    for (int a = 0; a < kNAcc; a++) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < count; i += 256) {
+            const int slot = i < n0 ? lo0 + i : lo1 + (i - n0);
+            s += partials[(int64_t)slot * kNAcc + a];
+        }
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[(int64_t)b * kNAcc + a] = red[0];
+        __syncthreads();
+    }
+}
+
+// the total over the blocks, in the same fixed-order fashion
+__global__ __launch_bounds__(256) void k_reduce_total(double* __restrict__ out, int nblocks) {
+    __shared__ double red[256];
+    for (int a = 0; a < kNAcc; a++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += 256) s += out[(int64_t)b * kNAcc + a];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[(int64_t)nblocks * kNAcc + a] = red[0];
+        __syncthreads();
+    }
 }
 
 __global__ void k_fill_bytes(uint32_t* p, int64_t n, uint32_t v) {
@@ -1208,8 +1234,9 @@ hipError_t launch_halo(const HaloParams& hp, hipStream_t s) {
 hipError_t launch_reduce(const double* partials, const BlockParams* blocks, int nblocks,
                          int ntiles, int nbulk, int nbnd, int split, int fold, double* out,
                          hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, partials, blocks,
+    hipLaunchKernelGGL(k_reduce_blocks, dim3(nblocks), dim3(256), 0, s, partials, blocks,
                        nblocks, ntiles, nbulk, nbnd, split, fold, out);
+    hipLaunchKernelGGL(k_reduce_total, dim3(1), dim3(256), 0, s, out, nblocks);
     return hipGetLastError();
 }
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <array>
 #include <cmath>
 #include <cstdarg>
@@ -727,7 +728,9 @@ static int build_obstacles(mplbhip_ctx* c) {
         for (int k = 0; k < n[2]; k++)
             for (int j = 0; j < n[1]; j++)
                 for (int i = 0; i < n[0]; i++, p++) {
-                    const int vt = c->plan.node_type(b, i, j, k);
+                    // the node type is constant over a region (Plan::build)
+                    const int vt = c->plan.region_vt[(size_t)b * c->plan.nregions +
+                                                     c->plan.region_of(b, i, j, k)];
                     uint32_t mask = 0;
                     if (ext[b][eidx(b, i, j, k)]) {
                         if (vt != VT_Fluid && vt != VT_MDPeriodic)
@@ -818,7 +821,6 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     const int align = (int)(128 / c->es);
     c->bp.assign(nb, BlockParams{});
     int64_t off = 0, moff = 0;
-    int tiles = 0, bulk = 0, bulk2 = 0, bnd_v[3] = {0, 0, 0};
     c->segs.clear();
     // measured at 256^3: folding the x faces pays for 4- and 8-byte
     // populations; with 2-byte ones the extra lanes cost more than the
@@ -860,18 +862,12 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         off += (elems + 255) / 256 * 256;
         B.tiles_x = (n[0] + kTX - 1) / kTX;
         B.tiles_y = (n[1] + kTY - 1) / kTY;
-        B.tile_start = tiles;
-        tiles += B.tiles_x * B.tiles_y * n[2];
-        // the step's split: bulk tiles over z = 1 .. n-2 (3D), and the
-        // boundary regions as segments, 256 nodes per workgroup
-        B.bulk_tile_start = bulk;
-        bulk += B.tiles_x * B.tiles_y * (c->dim == 3 ? n[2] - 2 : 1);
         B.tiles_x2 = (n[0] + 2 * kTX - 1) / (2 * kTX);
-        B.h2_tile_start = bulk2;
-        bulk2 += B.tiles_x2 * B.tiles_y * (c->dim == 3 ? n[2] - 2 : 1);
-        // three segment lists: [0] every boundary region, [1] without the
-        // simple x-face interiors, [2] without every simple region of the
-        // bulk grid (the bulk kernel runs those inline at that fold level)
+        // the step's split: bulk tiles over z = 1 .. n-2 (3D), and the
+        // boundary regions as segments, 256 nodes per workgroup, in three
+        // lists: [0] every boundary region, [1] without the simple x-face
+        // interiors, [2] without every simple region of the bulk grid (the
+        // bulk kernel runs those inline at that fold level)
         for (int v = 0; v < 3; v++) {
             B.seg_start[v] = (int)c->segs.size();
             int count = 0;
@@ -898,8 +894,6 @@ int mplbhip_prepare(mplbhip_ctx* c) {
                 c->segs.push_back(S);
             }
             B.seg_count[v] = (int)c->segs.size() - B.seg_start[v];
-            B.bnd_wg_start[v] = bnd_v[v];
-            bnd_v[v] += (count + kBndWG - 1) / kBndWG;
         }
         B.prog_base = b * c->plan.nregions;
         B.mbase = moff;
@@ -909,11 +903,8 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     }
     c->total_elems = off;
     c->interior_nodes = moff;
-    c->ntiles = tiles;
-    c->nbulk = bulk;
-    for (int v = 0; v < 3; v++) c->nbnd_v[v] = bnd_v[v];
+    // workgroup ranges of every launch form
     const int grouped = assign_tiles(c, false);
-    (void)bulk2;
     c->xgroup = grouped > 0 ? 1 : 0;
     c->xgroup_ok = c->xgroup;
     c->rowwg_ok = false;
@@ -970,8 +961,9 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         HIPCHK(c, hipMemcpyAsync(c->dobst.p, c->obst.data(), c->dobst.bytes,
                                  hipMemcpyHostToDevice, c->stream));
     }
-    const size_t nslots =
-        (size_t)std::max(tiles, bulk + std::max(bnd_v[0], std::max(bnd_v[1], bnd_v[2])));
+    // partial slots of the launches over every block (compaction only shrinks them)
+    const size_t nslots = (size_t)std::max(
+        c->ntiles, c->nbulk + std::max(c->nbnd_v[0], std::max(c->nbnd_v[1], c->nbnd_v[2])));
     if (int rc = dalloc(c, c->partials, nslots * kNAcc * sizeof(double))) return rc;
     if (int rc = dalloc(c, c->dsegs, std::max<size_t>(1, c->segs.size()) * sizeof(BndSeg)))
         return rc;
@@ -1400,6 +1392,11 @@ int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
     long long done = 0;
     int remaining = nb;
     std::vector<double> acc((size_t)(nb + 1) * kNAcc);
+    // MPLBHIP_VERBOSE=2: one line per check (time step, live instances and
+    // nodes, wall milliseconds since the previous check)
+    const char* verbose = std::getenv("MPLBHIP_VERBOSE");
+    const bool trace = verbose && std::atoi(verbose) >= 2;
+    auto t_last = std::chrono::steady_clock::now();
     while (done < max_steps && remaining > 0) {
         long long to_check = check_period - (c->tstep % check_period) - 1;
         long long plain = std::min<long long>(to_check, max_steps - done);
@@ -1466,6 +1463,13 @@ int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
         if (changed || !restart.empty())
             HIPCHK(c, hipMemcpyAsync(c->blocks.p, c->bp.data(), nb * sizeof(BlockParams),
                                      hipMemcpyHostToDevice, c->stream));
+        if (trace) {
+            const auto now = std::chrono::steady_clock::now();
+            std::fprintf(stderr, "mplbhip: check t=%lld live=%d grid_nodes=%lld fused=%d %.3f ms\n",
+                         (long long)c->tstep, remaining, (long long)c->grid_nodes, (int)c->fused,
+                         std::chrono::duration<double, std::milli>(now - t_last).count());
+            t_last = now;
+        }
     }
     // freezing lasts for this call only: later steps, macros and f reads
     // see every instance again

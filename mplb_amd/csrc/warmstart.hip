@@ -114,10 +114,13 @@ __host__ __device__ inline int wgrad_rows(int h, int wd) {
 // tap*cin + ci) x 32 (co) tile of sum_m A[m][k] dout[m][co] over the
 // chunk's pixels, 8 per product (MFMA reduction dimension = pixels); the
 // waves take the chunk's rows in turn and their tiles are summed through
-// LDS in wave order.  partial[chunk][k][co].
+// LDS in wave order.  partial[chunk][k][co].  The workgroups of the first
+// k tile also sum the dout values they load (FP32, fixed order) into the
+// bias-gradient partial bpart[chunk][co] when bpart is given.
 __global__ __launch_bounds__(64 * kWaves) void k_conv_wgrad(const float* __restrict__ in,
                                                             const float* __restrict__ dout,
                                                             float* __restrict__ partial,
+                                                            float* __restrict__ bpart,
                                                             int h, int wd, int cin, int cout,
                                                             int rows) {
     const int lane = threadIdx.x & 63;
@@ -138,6 +141,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_conv_wgrad(const float* __restr
     const float* inb = in + (int64_t)pn * h * wd * cin;
     const float* db = dout + (int64_t)pn * h * wd * cout;
     f32x16 acc = {};
+    float bsum = 0.f;
     for (int py = r0 + wave; py < r1; py += kWaves) {
         const int yy = py + dy;
         const bool yin = kvalid && yy >= 0 && yy < h;
@@ -152,10 +156,21 @@ __global__ __launch_bounds__(64 * kWaves) void k_conv_wgrad(const float* __restr
                     if (yin && xx >= 0 && xx < wd) av = inb[((int64_t)yy * wd + xx) * cin + ci];
                     if (cvalid) bv = db[((int64_t)py * wd + px) * cout + co];
                 }
+                bsum += bv;
                 a[t] = to_bf16(av);
                 b[t] = to_bf16(bv);
             }
             acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        }
+    }
+    __shared__ float bred[kWaves][2][32];
+    if (bpart && blockIdx.x == 0) {
+        bred[wave][g][row] = bsum;
+        __syncthreads();
+        if (threadIdx.x < 32 && co0 + (int)threadIdx.x < cout) {
+            float s = 0.f;
+            for (int v = 0; v < kWaves; v++) s += bred[v][0][threadIdx.x] + bred[v][1][threadIdx.x];
+            bpart[(int64_t)blockIdx.z * cout + co0 + threadIdx.x] = s;
         }
     }
     // sum the waves' tiles through LDS, then write this chunk's partial
@@ -182,33 +197,6 @@ __global__ void k_sum_partials(const float* __restrict__ partial, float* __restr
         float s = 0.f;
         for (int c = 0; c < nchunks; c++) s += partial[(int64_t)c * count + i];
         dst[i] = s;
-    }
-}
-
-// bias gradient partials over the same chunks: 256 / cout groups of cout
-// threads take the chunk's pixels in turn; the groups' sums are added in
-// group order
-__global__ __launch_bounds__(256) void k_bias_partial(const float* __restrict__ dout,
-                                                      float* __restrict__ partial, int h, int wd,
-                                                      int cout, int rows) {
-    const int t = threadIdx.x;
-    const int groups = 256 / cout;
-    const int grp = t / cout, co = t % cout;
-    const int per_image = (h + rows - 1) / rows;
-    const int pn = (int)blockIdx.x / per_image;
-    const int r0 = ((int)blockIdx.x % per_image) * rows;
-    const int r1 = r0 + rows < h ? r0 + rows : h;
-    const float* db = dout + (int64_t)pn * h * wd * cout;
-    float s = 0.f;
-    if (grp < groups)
-        for (int m = r0 * wd + grp; m < r1 * wd; m += groups) s += db[(int64_t)m * cout + co];
-    __shared__ float red[256];
-    red[t] = s;
-    __syncthreads();
-    if (t < cout) {
-        float tot = 0.f;
-        for (int q = 0; q < groups; q++) tot += red[q * cout + t];
-        partial[(int64_t)blockIdx.x * cout + t] = tot;
     }
 }
 
@@ -293,6 +281,18 @@ __global__ void k_channels(const float* __restrict__ a, float* __restrict__ b, i
         if (dir == 0) b[p * ctot + c0 + ch] = a[i];
         else if (dir == 1) b[i] = a[p * ctot + c0 + ch];
         else b[i] += a[p * ctot + c0 + ch];
+    }
+}
+
+// dst[p][ch] = src[p][ch] on pixels with mask[p] == 0 and ch < cu, else 0
+__global__ void k_mask_channels(const float* src, const float* __restrict__ mask, float* dst,
+                                int64_t pixels, int c, int cu) {
+    const int64_t total = pixels * c;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % c);
+        const int64_t p = i / c;
+        dst[i] = (ch < cu && mask[p] == 0.f) ? src[i] : 0.f;
     }
 }
 
@@ -402,25 +402,22 @@ int64_t mplbws_wgrad_workspace(int n, int h, int wd, int cin, int cout) {
 int mplbws_conv3x3_wgrad(const float* in, const float* dout, float* dw, float* dbias,
                          float* workspace, int n, int h, int wd, int cin, int cout) {
     if (!in || !dout || !dw || !workspace || n <= 0 || h <= 0 || wd <= 0 || cin <= 0 ||
-        cout <= 0 || (dbias && cout > 256))
+        cout <= 0)
         return MPLBWS_ERR_ARG;
     const int rows = wgrad_rows(h, wd);
     const int64_t chunks = (int64_t)n * ((h + rows - 1) / rows);
     if (chunks > 65535) return MPLBWS_ERR_ARG;
     const int K = 9 * cin;
     const dim3 grd((unsigned)((K + 31) / 32), (unsigned)((cout + 31) / 32), (unsigned)chunks);
-    hipLaunchKernelGGL(k_conv_wgrad, grd, dim3(64 * kWaves), 0, 0, in, dout, workspace, h, wd,
-                       cin, cout, rows);
     const int64_t cnt = (int64_t)K * cout;
+    float* bp = dbias ? workspace + chunks * cnt : nullptr;
+    hipLaunchKernelGGL(k_conv_wgrad, grd, dim3(64 * kWaves), 0, 0, in, dout, workspace, bp, h,
+                       wd, cin, cout, rows);
     hipLaunchKernelGGL(k_sum_partials, dim3(grid_for(cnt)), dim3(256), 0, 0, workspace, dw, cnt,
                        (int)chunks);
-    if (dbias) {
-        float* bp = workspace + chunks * cnt;
-        hipLaunchKernelGGL(k_bias_partial, dim3((unsigned)chunks), dim3(256), 0, 0, dout, bp, h,
-                           wd, cout, rows);
+    if (dbias)
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, 0, bp, dbias, (int64_t)cout,
                            (int)chunks);
-    }
     return status();
 }
 
@@ -463,6 +460,14 @@ int mplbws_channels(const float* src, float* dst, int64_t pixels, int c, int cto
     if (!src || !dst || c0 < 0 || c0 + c > ctot) return MPLBWS_ERR_ARG;
     hipLaunchKernelGGL(k_channels, dim3(grid_for(pixels * c)), dim3(256), 0, 0, src, dst, pixels,
                        c, ctot, c0, direction);
+    return status();
+}
+
+int mplbws_mask_channels(const float* src, const float* mask, float* dst, int64_t pixels, int c,
+                         int cu) {
+    if (!src || !mask || !dst || pixels < 0 || c <= 0 || cu < 0) return MPLBWS_ERR_ARG;
+    hipLaunchKernelGGL(k_mask_channels, dim3(grid_for(pixels * c)), dim3(256), 0, 0, src, mask,
+                       dst, pixels, c, cu);
     return status();
 }
 

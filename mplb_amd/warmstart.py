@@ -59,6 +59,7 @@ def load():
         "mplbws_up2_bwd": [P, P, I, I, I, I, I, I],
         "mplbws_channels": [P, P, L, I, I, I, I],
         "mplbws_gather": [P, P, P, I, L],
+        "mplbws_mask_channels": [P, P, P, L, I, I],
         "mplbws_mse": [P, P, P, P, P, I, L, I, I, F],
         "mplbws_adam": [P, P, P, P, L, F, F, F, F, I],
         "mplbws_sync": [],
@@ -148,6 +149,9 @@ def _at(t, offset):
 # wider than a 128^2 cavity, whose flow is global.
 ENC = (24, 48, 64, 64)
 BOTTLENECK = 64
+# large-scale loss terms of the pipeline's training: (pooling levels,
+# weight), i.e. errors averaged over 4x4 and 16x16 pixel squares
+COARSE = ((2, 4.0), (4, 4.0))
 
 
 def layer_specs(enc=ENC, bott=BOTTLENECK):
@@ -179,9 +183,15 @@ class UNet:
     """U-Net on n x n images (n divisible by 2^levels), batch `batch`;
     parameters in FP32 with Adam state; activations kept for backward."""
 
-    def __init__(self, n=128, batch=32, seed=0, enc=ENC, bott=BOTTLENECK):
+    def __init__(self, n=128, batch=32, seed=0, enc=ENC, bott=BOTTLENECK, coarse=()):
+        """coarse: ((pools, weight), ...) adds, per entry, the squared error
+        of the fluid-masked prediction and the target both averaged over
+        2^pools x 2^pools pixel squares, times weight (see loss_grad_device)."""
         self.lib = load()
         self.n, self.batch = n, batch
+        self.coarse = tuple((int(k), float(w)) for k, w in coarse)
+        if any(k < 1 or n % (1 << k) for k, _ in self.coarse):
+            raise WarmStartError("coarse levels must divide the image size")
         self.enc, self.bott = tuple(enc), bott
         self.layers = layer_specs(self.enc, bott)
         nl = len(self.enc)
@@ -227,6 +237,18 @@ class UNet:
         self.lossbuf = DeviceArray((LOSS_PARTIALS,), zero=True)
         self._tgt = DeviceArray((batch, n, n, 3))
         self._msk = DeviceArray((batch, n, n))
+        # multi-scale loss: masked prediction, then per pooling level the
+        # pooled prediction, target and gradient (index 0 = full size)
+        kmax = max([k for k, _ in self.coarse], default=0)
+        self._pm = DeviceArray((batch, n, n, OUT_CH)) if kmax else None
+        self._pp = [None] + [DeviceArray((batch, n >> k, n >> k, OUT_CH)) for k in range(1, kmax + 1)]
+        self._pt = [None] + [DeviceArray((batch, n >> k, n >> k, 3)) for k in range(1, kmax + 1)]
+        self._pg = [DeviceArray((batch, n >> k, n >> k, OUT_CH)) for k in range(kmax + 1)]
+        self._pgo = [None] + [DeviceArray((batch, n >> k, n >> k, OUT_CH)) if k < kmax else None
+                              for k in range(1, kmax + 1)]
+        self._pzero = [None] + [DeviceArray((batch, n >> k, n >> k), zero=True)
+                                for k in range(1, kmax + 1)]
+        self._pparts = DeviceArray((LOSS_PARTIALS,), zero=True) if kmax else None
 
     # -- building blocks
     def _conv(self, name, src, dst):
@@ -323,6 +345,47 @@ class UNet:
         pixels = self.batch * self.n * self.n
         _ck(self.lib.mplbws_mse(_p(self.act["out"]), _p(self._tgt), _p(self._msk), _p(self.gout),
                                 partial, LOSS_PARTIALS, pixels, 3, OUT_CH, inv), "mse")
+        if self.coarse:
+            self._coarse_loss()
+
+    def _coarse_loss(self):
+        """Large-scale terms: for each (k, w), w times the mean over the
+        2^k-pooled pixels and the 3 channels of (P_k(m p) - P_k(t))^2, with
+        m the fluid mask and P_k the 2^k x 2^k average (the targets are 0
+        on obstacles).  Their gradient, m P_k^T(2 w (P_k(m p) - P_k(t)) /
+        count), is added to self.gout.  Slow, large-scale errors of the
+        initial flow cost the most solver steps."""
+        L, B, n = self.lib, self.batch, self.n
+        kmax = max(k for k, _ in self.coarse)
+        _ck(L.mplbws_mask_channels(_p(self.act["out"]), _p(self._msk), _p(self._pm),
+                                   B * n * n, OUT_CH, 3), "mask")
+        src, tsrc = self._pm, self._tgt
+        for k in range(1, kmax + 1):
+            s = n >> (k - 1)
+            _ck(L.mplbws_pool2_fwd(_p(src), _p(self._pp[k]), B, s, s, OUT_CH), "pool")
+            _ck(L.mplbws_pool2_fwd(_p(tsrc), _p(self._pt[k]), B, s, s, 3), "pool")
+            src, tsrc = self._pp[k], self._pt[k]
+        weight = dict(self.coarse)
+        # gradient with respect to level k's pooled prediction: its own term
+        # plus, carried down by the pooling adjoint, level k + 1's gradient
+        # (already in self._pg[k] for k < kmax when the loop gets there)
+        for k in range(kmax, 0, -1):
+            s = n >> k
+            pix = B * s * s
+            if k in weight:
+                own = self._pg[k] if k == kmax else self._pgo[k]
+                _ck(L.mplbws_mse(_p(self._pp[k]), _p(self._pt[k]), _p(self._pzero[k]), _p(own),
+                                 _p(self._pparts), LOSS_PARTIALS, pix, 3, OUT_CH,
+                                 weight[k] / (pix * 3.0)), "mse")
+                if k < kmax:
+                    _ck(L.mplbws_channels(_p(own), _p(self._pg[k]), pix, OUT_CH, OUT_CH, 0, 2),
+                        "add")
+            _ck(L.mplbws_pool2_bwd(_p(self._pg[k]), _p(self._pg[k - 1]), B, 2 * s, 2 * s,
+                                   OUT_CH), "pool")
+        _ck(L.mplbws_mask_channels(_p(self._pg[0]), _p(self._msk), _p(self._pg[0]),
+                                   B * n * n, OUT_CH, 3), "mask")
+        _ck(L.mplbws_channels(_p(self._pg[0]), _p(self.gout), B * n * n, OUT_CH, OUT_CH, 0, 2),
+            "add")
 
     def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
         self.step_count += 1
@@ -378,17 +441,20 @@ def inputs(n, res, rects, re_range=(100.0, 1000.0)):
 
 
 def solve(n, res, rects, criteria, check_period=1000, max_steps=2000000,
-          storage=hip.STORE_F32, init=None, device=0, restart=0.0):
+          storage=hip.STORE_F32, init=None, device=0, restart=0.0, timings=None):
     """Run the cavities (one block each) to the velocity residual criteria.
     init: None (rest) or (rho, u, v) arrays (count, n, n) in code units,
     lifted to populations.  restart > 0: an instance whose residual climbs
     above restart times its lowest restarts once at rest.  Returns (case,
     steps per case, residual per case, wall seconds from case set-up to
-    convergence)."""
+    convergence).  timings, a dict, receives the seconds of the case
+    set-up ("setup"), of the initial state ("init") and of the iteration
+    ("iterate")."""
     t0 = time.perf_counter()
     g = hip.LatticeCase(2, storage=storage, device=device)
     masks = [cases.rect_mask(n, r) for r in rects]
     cases.batched_cavities(g, [n] * len(res), res, ulid=LID, masks=masks)
+    t1 = time.perf_counter()
     if init is not None:
         rho, u, v = init
         for b in range(len(res)):
@@ -396,9 +462,13 @@ def solve(n, res, rects, criteria, check_period=1000, max_steps=2000000,
         g.init_lift()
     if restart > 0:
         g.set_batch_restart(restart)
+    t2 = time.perf_counter()
     steps, resid, _ = g.iterate_batch(criteria, check_period, max_steps)
     g.sync()
-    return g, steps, resid, time.perf_counter() - t0
+    t3 = time.perf_counter()
+    if timings is not None:
+        timings.update(setup=t1 - t0, init=t2 - t1, iterate=t3 - t2)
+    return g, steps, resid, t3 - t0
 
 
 def targets_from(g, count, n, masks):
@@ -500,7 +570,7 @@ def velocity_l2(g_a, g_b, count):
 
 def run(n=128, n_train=4096, n_test=512, epochs=200, criteria=1e-8, check_period=1000,
         seed=0x4D504C42, batch=32, chunk=512, storage=hip.STORE_F32, restart=1e3,
-        enc=ENC, bott=BOTTLENECK, diagnose=False, log=print):
+        enc=ENC, bott=BOTTLENECK, coarse=COARSE, diagnose=False, log=print):
     """Time-to-residual of learned versus rest initialisation.
 
     1. Training targets: n_train cavities (Re log-uniform in [100, 1000],
@@ -534,7 +604,7 @@ def run(n=128, n_train=4096, n_test=512, epochs=200, criteria=1e-8, check_period
     t_data = time.perf_counter() - t0
     # 2. training
     t0 = time.perf_counter()
-    net = UNet(n=n, batch=batch, seed=seed & 0xFFFF, enc=enc, bott=bott)
+    net = UNet(n=n, batch=batch, seed=seed & 0xFFFF, enc=enc, bott=bott, coarse=coarse)
     hist = train(net, x_all[:n_train], y_train, epochs, log=log)
     mplbws_sync = load().mplbws_sync
     mplbws_sync()
@@ -542,7 +612,9 @@ def run(n=128, n_train=4096, n_test=512, epochs=200, criteria=1e-8, check_period
     # 3. evaluation on the held-out cases
     tr, tq = res[n_train:], rects[n_train:]
     log("evaluation: %d held-out cases from rest" % n_test)
-    g_rest, s_rest, r_rest, t_rest = solve(n, tr, tq, criteria, check_period, storage=storage)
+    tm_rest, tm_warm = {}, {}
+    g_rest, s_rest, r_rest, t_rest = solve(n, tr, tq, criteria, check_period, storage=storage,
+                                           timings=tm_rest)
     log("rest: %.2f s, steps median %.0f max %d" % (t_rest, np.median(s_rest), s_rest.max()))
     g_rest.update_macros()
     t0 = time.perf_counter()
@@ -554,7 +626,8 @@ def run(n=128, n_train=4096, n_test=512, epochs=200, criteria=1e-8, check_period
     # failed even counting one restart
     budget = 2 * int(s_rest.max()) + check_period
     g_warm, s_warm, r_warm, t_solve = solve(n, tr, tq, criteria, check_period, budget,
-                                            storage=storage, init=init, restart=restart)
+                                            storage=storage, init=init, restart=restart,
+                                            timings=tm_warm)
     restarts = g_warm.batch_restarts()
     t_warm = t_inf + t_solve
     log("learned: %.2f s (inference %.3f s), steps median %.0f max %d"
@@ -600,6 +673,8 @@ def run(n=128, n_train=4096, n_test=512, epochs=200, criteria=1e-8, check_period
         "data_seconds": round(t_data, 2), "data_steps_median": float(np.median(data_steps)),
         "train_seconds": round(t_train, 2),
         "rest_seconds": round(t_rest, 3), "learned_seconds": round(t_warm + t_fb, 3),
+        "rest_phases_seconds": {k: round(v, 3) for k, v in tm_rest.items()},
+        "learned_phases_seconds": {k: round(v, 3) for k, v in tm_warm.items()},
         "inference_seconds": round(t_inf, 4), "fallback_cases": len(bad),
         "restart_factor": restart, "restarted_cases": int((restarts > 0).sum()),
         "speedup": round(t_rest / (t_warm + t_fb), 3),
@@ -610,7 +685,7 @@ def run(n=128, n_train=4096, n_test=512, epochs=200, criteria=1e-8, check_period
         "work_ratio_sum_steps": float(s_rest.sum() / max(s_warm.sum(), 1)),
         "prediction_rel_l2_median": float(np.median(pred_err)),
         "prediction_density_rel_l2_median": float(np.median(rho_err)),
-        "network": {"encoder": list(enc), "bottleneck": bott,
+        "network": {"encoder": list(enc), "bottleneck": bott, "coarse_loss": list(coarse),
                     "params": param_count(layer_specs(enc, bott))},
         "velocity_rel_l2_learned_vs_rest_median": float(np.median(l2)),
         "velocity_rel_l2_learned_vs_rest_max": float(l2.max()),

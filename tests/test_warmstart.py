@@ -285,6 +285,37 @@ def test_gpu_unet_gradients_match_cpu():
 
 
 @pytest.mark.gpu
+def test_gpu_multiscale_loss_gradient():
+    """The output gradient with large-scale terms, against the same loss
+    written with numpy reshapes: fine masked MSE plus, per level k, w times
+    the mean of (P_k(m p) - P_k(t))^2 over the 2^k-pooled pixels."""
+    n, B = 32, 2
+    coarse = ((1, 0.5), (3, 2.0))
+    net = ws.UNet(n=n, batch=B, seed=3, enc=(16, 32, 32), bott=32, coarse=coarse)
+    res, rects = ws.case_params(B, seed=4, n=n)
+    x = ws.inputs(n, res, rects)
+    m = x[..., 0] == 0
+    rng = np.random.default_rng(11)
+    t = (rng.standard_normal((B, n, n, 3)) * 0.1 * m[..., None]).astype(np.float32)
+    out = net.forward(x).numpy().astype(np.float64)
+    loss, g = net.loss_grad(t, x[..., 0])
+    g = g.numpy()
+    p = out[..., :3]
+    ref = np.zeros((B, n, n, ws.OUT_CH))
+    ref[..., :3] = np.where(m[..., None], 2.0 * (p - t) / (m.sum() * 3.0), 0.0)
+    assert abs(loss - ((p - t) ** 2 * m[..., None]).sum() / (m.sum() * 3.0)) <= 1e-5 * loss
+    pm = p * m[..., None]
+    for k, w in coarse:
+        f = 1 << k
+        sz = n // f
+        pool = lambda a: a.reshape(B, sz, f, sz, f, 3).mean(axis=(2, 4))
+        gc = 2.0 * w * (pool(pm) - pool(t.astype(np.float64))) / (B * sz * sz * 3.0)
+        up = np.repeat(np.repeat(gc, f, axis=1), f, axis=2) / (f * f)
+        ref[..., :3] += up * m[..., None]
+    assert np.max(np.abs(g - ref)) <= 1e-5 * np.max(np.abs(ref))
+
+
+@pytest.mark.gpu
 def test_gpu_training_reduces_loss():
     print("U-Net 32x32 training steps", flush=True)
     n, B = 32, 8
