@@ -30,7 +30,7 @@ LID = 0.05                     # lattice lid speed of the cavities
 # density channel (dominated by the lid-corner singularities, and only a
 # small part of the lift) weighs less than the velocity in the loss
 RHO_SCALE = 1.0 / (30 * LID * LID)
-IN_CH = 8                      # input channels (6 used, padded to 8)
+IN_CH = 8                      # input channels
 LOSS_PARTIALS = 1024           # partial sums of one loss evaluation
 OUT_CH = 8                     # output channels (3 used)
 
@@ -423,20 +423,37 @@ def case_params(count, seed, re_range=(100.0, 1000.0), n=128):
     return [float(r) for r in res], cases.obstacle_rects(count, n=n, seed=seed, side=side)
 
 
-def inputs(n, res, rects, re_range=(100.0, 1000.0)):
+def inputs(n, res, rects, re_range=(100.0, 1000.0), sdf=True):
     """Network inputs (count, n, n, IN_CH): mask, normalised log Re, lid
-    speed / 0.05, lid row, x and y coordinates in [0, 1]."""
+    speed / 0.05, lid row, x and y coordinates in [0, 1], and (sdf) the
+    Euclidean distance to the nearest wall or obstacle node / (n / 2) and
+    to the nearest obstacle node / n (1 without obstacles).  A stack of
+    3x3 convolutions cannot measure long distances itself; the distance
+    fields hand it the geometry at every pixel."""
     cnt = len(res)
     x = np.zeros((cnt, n, n, IN_CH), np.float32)
     lo, hi = math.log(re_range[0]), math.log(re_range[1])
     yy, xx = np.meshgrid(np.arange(n) / (n - 1), np.arange(n) / (n - 1), indexing="ij")
+    iy, ix = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    # the walls are the node frame just outside the n x n nodes
+    d_wall = np.minimum(np.minimum(ix + 1, n - ix), np.minimum(iy + 1, n - iy)).astype(float)
     for k in range(cnt):
-        x[k, :, :, 0] = cases.rect_mask(n, rects[k])
+        solid = cases.rect_mask(n, rects[k])
+        x[k, :, :, 0] = solid
         x[k, :, :, 1] = (math.log(res[k]) - lo) / (hi - lo)
         x[k, :, :, 2] = 1.0
         x[k, n - 1, :, 3] = 1.0
         x[k, :, :, 4] = xx
         x[k, :, :, 5] = yy
+        if sdf:
+            d_obs = np.full((n, n), float(n))
+            for x0, y0, w, h in rects[k]:
+                # node distance to an axis-aligned box of nodes, in closed form
+                dx = np.maximum(np.maximum(x0 - ix, ix - (x0 + w - 1)), 0)
+                dy = np.maximum(np.maximum(y0 - iy, iy - (y0 + h - 1)), 0)
+                d_obs = np.minimum(d_obs, np.sqrt(dx * dx + dy * dy))
+            x[k, :, :, 6] = np.minimum(d_wall, d_obs) / (n / 2)
+            x[k, :, :, 7] = d_obs / n
     return x
 
 
@@ -587,7 +604,7 @@ def run(n=128, n_train=4096, n_test=512, epochs=200, criteria=1e-8, check_period
     velocity with the converged density."""
     t_all = time.perf_counter()
     res, rects = case_params(n_train + n_test, seed, n=n)
-    x_all = inputs(n, res, rects)
+    x_train = inputs(n, res[:n_train], rects[:n_train])
     # 1. data
     t0 = time.perf_counter()
     ys, data_steps = [], []
@@ -605,7 +622,7 @@ def run(n=128, n_train=4096, n_test=512, epochs=200, criteria=1e-8, check_period
     # 2. training
     t0 = time.perf_counter()
     net = UNet(n=n, batch=batch, seed=seed & 0xFFFF, enc=enc, bott=bott, coarse=coarse)
-    hist = train(net, x_all[:n_train], y_train, epochs, log=log)
+    hist = train(net, x_train, y_train, epochs, log=log)
     mplbws_sync = load().mplbws_sync
     mplbws_sync()
     t_train = time.perf_counter() - t0
@@ -617,8 +634,9 @@ def run(n=128, n_train=4096, n_test=512, epochs=200, criteria=1e-8, check_period
                                            timings=tm_rest)
     log("rest: %.2f s, steps median %.0f max %d" % (t_rest, np.median(s_rest), s_rest.max()))
     g_rest.update_macros()
+    # the learned start's own cost: its inputs, inference and conversion
     t0 = time.perf_counter()
-    pred = predict(net, x_all[n_train:])
+    pred = predict(net, inputs(n, tr, tq))
     masks = [cases.rect_mask(n, r) for r in tq]
     init = prediction_to_macros(pred, masks)
     t_inf = time.perf_counter() - t0

@@ -60,6 +60,24 @@ def test_oracle_bf16_rounding():
     assert abs(r[4] - 3.0e-3) / 3.0e-3 < 2 ** -8
 
 
+def test_distance_inputs_match_a_distance_transform():
+    """The closed-form distance channels equal scipy's exact Euclidean
+    distance transform of the obstacle mask framed by the walls."""
+    from scipy import ndimage
+    from mplb_amd import cases
+    n = 64
+    res, rects = ws.case_params(24, seed=21, n=n)
+    x = ws.inputs(n, res, rects)
+    for k in range(len(res)):
+        solid = cases.rect_mask(n, rects[k]).astype(bool)
+        framed = np.pad(solid, 1, constant_values=True)
+        d = ndimage.distance_transform_edt(~framed)[1:-1, 1:-1]
+        assert np.max(np.abs(x[k, :, :, 6] - d / (n / 2))) < 1e-6
+        do = (np.minimum(ndimage.distance_transform_edt(~solid), n) / n if solid.any()
+              else np.ones((n, n)))
+        assert np.max(np.abs(x[k, :, :, 7] - do)) < 1e-6
+
+
 def test_network_size():
     # 24-48-64-64 encoder, 64-channel bottleneck: 9 ci co + co per conv
     assert ws.param_count() == 563_688
