@@ -1055,30 +1055,32 @@ hipError_t launch_rest_macros(const KParams& kp, int b, int64_t nodes, hipStream
 }
 
 // fStage halo slab copy of one block connection (Field::TransferHalos ->
-// ops_halo_transfer, field.h:94-99, slabs of field.h:215-395)
+// ops_halo_transfer, field.h:94-99, slabs of field.h:215-395).  One thread
+// per slab cell and population (grid.y = population), 32-bit cell
+// arithmetic: the slab is one layer of a face.  A target halo cell is read
+// by the target block's pulls only for the populations that stream inward
+// across the face, so only those are copied, except for cells another
+// connection reads (flag bit 1).  Across a face the slab has no contiguous
+// axis, so every copied element costs a cache line each way.
 template <typename E>
-__global__ void k_halo(HaloParams hp) {
-    const int64_t ncell = (int64_t)hp.ext[0] * hp.ext[1] * hp.ext[2];
-    const int64_t total = ncell * hp.Q;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int q = (int)(i / ncell);
-        int64_t c = i - (int64_t)q * ncell;
-        const int a = (int)(c % hp.ext[0]);
-        c /= hp.ext[0];
-        const int bb = (int)(c % hp.ext[1]);
-        const int cc = (int)(c / hp.ext[1]);
-        const int64_t cell = i - (int64_t)q * ncell;
-        const E* src = static_cast<const E*>(hp.other_flag && hp.other_flag[cell]
-                                                 ? hp.src_other
-                                                 : hp.src_cur);
-        const int64_t si = hp.src_base + (int64_t)q * hp.src_sq +
-                           (int64_t)(hp.org_src[2] + cc) * hp.src_sz +
-                           (int64_t)(hp.org_src[1] + bb) * hp.src_sx + hp.org_src[0] + a;
-        const int64_t di = hp.dst_base + (int64_t)hp.qmap[q] * hp.dst_sq + hp.dst_org +
-                           a * hp.dstep[0] + bb * hp.dstep[1] + cc * hp.dstep[2];
-        static_cast<E*>(hp.dst)[di] = src[si];
-    }
+__global__ __launch_bounds__(256) void k_halo(HaloParams hp) {
+    const int ncell = hp.ext[0] * hp.ext[1] * hp.ext[2];
+    const int cell = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    if (cell >= ncell) return;
+    const int q = (int)blockIdx.y;
+    const uint8_t fl = hp.other_flag ? hp.other_flag[cell] : 0;
+    if (!((hp.inward >> q) & 1u) && !(fl & 2)) return;
+    const int a = cell % hp.ext[0];
+    const int rest = cell / hp.ext[0];
+    const int bb = rest % hp.ext[1];
+    const int cc = rest / hp.ext[1];
+    const E* src = static_cast<const E*>((fl & 1) ? hp.src_other : hp.src_cur);
+    const int64_t si = hp.src_base + (int64_t)q * hp.src_sq +
+                       (int64_t)(hp.org_src[2] + cc) * hp.src_sz +
+                       (int64_t)(hp.org_src[1] + bb) * hp.src_sx + hp.org_src[0] + a;
+    const int64_t di = hp.dst_base + (int64_t)hp.qmap[q] * hp.dst_sq + hp.dst_org +
+                       a * hp.dstep[0] + bb * hp.dstep[1] + cc * hp.dstep[2];
+    static_cast<E*>(hp.dst)[di] = src[si];
 }
 
 // Per-block sums of the per-workgroup partials, one workgroup per block:
@@ -1218,16 +1220,16 @@ hipError_t launch_lattice(int dim, int storage, int mode, const KParams& kp, hip
 }
 
 hipError_t launch_halo(const HaloParams& hp, hipStream_t s) {
-    const int64_t total = (int64_t)hp.ext[0] * hp.ext[1] * hp.ext[2] * hp.Q;
-    int grid = (int)((total + 255) / 256);
-    if (grid > 4096) grid = 4096;
-    if (grid < 1) grid = 1;
+    const int64_t ncell = (int64_t)hp.ext[0] * hp.ext[1] * hp.ext[2];
+    if (ncell >= (int64_t)1 << 31 || hp.Q < 1 || hp.Q > kMaxQ) return hipErrorInvalidValue;
+    if (ncell == 0) return hipSuccess;
+    const dim3 grid((unsigned)((ncell + 255) / 256), (unsigned)hp.Q);
     if (hp.elem_bytes == 8)
-        hipLaunchKernelGGL(k_halo<double>, dim3(grid), dim3(256), 0, s, hp);
+        hipLaunchKernelGGL(k_halo<double>, grid, dim3(256), 0, s, hp);
     else if (hp.elem_bytes == 4)
-        hipLaunchKernelGGL(k_halo<float>, dim3(grid), dim3(256), 0, s, hp);
+        hipLaunchKernelGGL(k_halo<float>, grid, dim3(256), 0, s, hp);
     else
-        hipLaunchKernelGGL(k_halo<__half>, dim3(grid), dim3(256), 0, s, hp);
+        hipLaunchKernelGGL(k_halo<__half>, grid, dim3(256), 0, s, hp);
     return hipGetLastError();
 }
 

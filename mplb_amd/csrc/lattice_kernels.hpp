@@ -133,7 +133,12 @@ struct HaloParams {
     int64_t dst_org;
     int64_t dstep[3];
     int32_t qmap[kMaxQ];
-    const uint8_t* other_flag;  // per slab cell: 1 = read src_other
+    // per slab cell (or null = all 0): bit 0 = read src_other; bit 1 = copy
+    // every population (another connection reads the target cell)
+    const uint8_t* other_flag;
+    // populations (source index) that stream from the target halo into the
+    // target block: the only ones a cell without bit 1 needs
+    uint32_t inward;
 };
 
 }  // namespace mplbhip

@@ -620,6 +620,24 @@ static int setup_halos(mplbhip_ctx* c) {
                         anyother = true;
                     }
                 }
+        // target cells that another connection reads (as its source, from
+        // either buffer) keep every population
+        const LinkMap& mi = maps[i];
+        const BlockDef& Ti = c->plan.blocks[dst[i].b];
+        for (int zz = 0; zz < s.ext[2]; zz++)
+            for (int yy = 0; yy < s.ext[1]; yy++)
+                for (int xx = 0; xx < s.ext[0]; xx++) {
+                    const int p[3] = {s.org[0] + xx, s.org[1] + yy, s.org[2] + zz};
+                    int t[3];
+                    link_target(mi, Ti.n, p, t);
+                    if (c->dim == 2) t[2] = 0;
+                    for (size_t j = 0; j < seq.size(); j++)
+                        if (j != i && covers(src[j], dst[i].b, t[0], t[1], t[2])) {
+                            flag[((int64_t)zz * s.ext[1] + yy) * s.ext[0] + xx] |= 2;
+                            anyother = true;
+                            break;
+                        }
+                }
         HaloLink h;
         std::memset(&h.hp, 0, sizeof h.hp);
         h.from = seq[i].from;
@@ -658,6 +676,10 @@ static int setup_halos(mplbhip_ctx* c) {
         }
         h.hp.Q = c->Q;
         h.hp.elem_bytes = (int)c->es;
+        // inward across the target face: against its outward normal st e_at
+        h.hp.inward = 0;
+        for (int q = 0; q < c->Q; q++)
+            if (cv[h.hp.qmap[q]][m.at] * m.st < 0) h.hp.inward |= 1u << q;
         if (anyother && c->device >= 0) {
             HIPCHK(c, hipMalloc(&h.d_flag, ncell));
             HIPCHK(c, hipMemcpy(h.d_flag, flag.data(), ncell, hipMemcpyHostToDevice));
