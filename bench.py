@@ -370,7 +370,12 @@ def main():
     mlups = nodes * K / wall / 1e6
     q = 19
     bulk_s = ms_bulk / K / 1e3
-    bulk_bytes = (n - 2) ** 3 * 2 * q * ELEM[st]  # algorithmic, per launch
+    # algorithmic bytes per bulk launch: 2 Q s for every node it updates
+    # (the interior, plus the boundary nodes folded into it at fold level
+    # 1 or 2); the interior-only figure is kept beside it
+    fold, xgroup, h2, bulk_nodes = g.launch_shape()
+    bulk_bytes = bulk_nodes * 2 * q * ELEM[st]
+    interior_bytes = (n - 2) ** 3 * 2 * q * ELEM[st]
     achieved = bulk_bytes / bulk_s / 1e9
     log("bench: wall %.3f s, gpu span %.3f ms, bulk kernel avg %.4f ms (%.0f GB/s), "
         "boundary kernel avg %.4f ms, %.0f MLUPS"
@@ -429,6 +434,9 @@ def main():
                          "k_bulk_d3q19_%d_%s_traffic.json" % (n, st)),
                      "kernel": "k_bulk (D3Q19, %s, step)" % st,
                      "algorithmic_bytes": bulk_bytes,
+                     "bulk_nodes_per_launch": bulk_nodes,
+                     "launch_shape": {"fold": fold, "xgroup": xgroup, "h2": h2},
+                     "achieved_interior_only": round(interior_bytes / bulk_s / 1e9, 1),
                      "kernel_avg_ms": round(bulk_s * 1e3, 5),
                      "boundary_kernel_avg_ms": round(ms_bnd / K, 5),
                      "step_kernels_avg_ms": round(ms_kernel / K, 5),

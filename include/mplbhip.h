@@ -134,6 +134,13 @@ int mplbhip_step_timed(mplbhip_ctx* ctx, int nsteps, double* ms_total,
 /* Diagnostic, no reference counterpart: the bulk-kernel and boundary-kernel
  * shares of the last mplbhip_step_timed call (ms, summed over its steps). */
 int mplbhip_kernel_times(mplbhip_ctx* ctx, double* ms_bulk, double* ms_boundary);
+/* Diagnostic, no reference counterpart: the step's launch shape in force
+ * (after the first step's autotuning): fold level (0-2), bulk workgroup
+ * shape (0 plain tiles, 1 rows dealt to one XCD, 2 row workgroups), 1 if
+ * FP16 steps use the two-node kernel, and the nodes one bulk-kernel launch
+ * updates (interior nodes plus the folded boundary nodes). */
+int mplbhip_launch_shape(mplbhip_ctx* ctx, int* fold, int* xgroup, int* h2,
+                         long long* bulk_nodes);
 /* Iterate (evolution.cpp:71-135 and the templates evolution.h:78-129).
  * criteria < 0: run exactly max_steps steps, refreshing the macro variables
  * at every check_period-th step (the transient form).  criteria >= 0: step

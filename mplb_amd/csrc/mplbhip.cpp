@@ -1547,6 +1547,40 @@ int mplbhip_kernel_times(mplbhip_ctx* c, double* ms_bulk, double* ms_boundary) {
     return 0;
 }
 
+int mplbhip_launch_shape(mplbhip_ctx* c, int* fold, int* xgroup, int* h2,
+                         long long* bulk_nodes) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    if (!c->prepared) return fail(c, MPLBHIP_ERR_ARG, "call mplbhip_prepare first");
+    const int fl = c->h2 ? 0 : c->fold;
+    if (fold) *fold = fl;
+    if (xgroup) *xgroup = c->xgroup;
+    if (h2) *h2 = c->h2 ? 1 : 0;
+    if (bulk_nodes) {
+        // the bulk kernel's lanes: interior nodes of its planes, plus (fold
+        // 1) the simple x-face regions or (fold 2) every simple region of
+        // the plane, as in its lane selection
+        long long total = 0;
+        for (size_t b = 0; b < c->bp.size(); b++) {
+            const int* n = c->plan.blocks[b].n;
+            const long long planes = c->dim == 3 ? n[2] - 2 : 1;
+            long long per_plane = 0;
+            for (int ry = 0; ry < 3; ry++)
+                for (int rx = 0; rx < 3; rx++) {
+                    const int r = rx + 3 * ry;
+                    const long long cnt =
+                        (long long)(rx == 0 ? n[0] - 2 : 1) * (ry == 0 ? n[1] - 2 : 1);
+                    bool lane = r == 0;
+                    if (!lane && fl > 0 && (fl == 2 || ry == 0))
+                        lane = c->plan.progs[b * c->plan.nregions + r].simple != 0;
+                    if (lane) per_plane += cnt;
+                }
+            total += per_plane * planes;
+        }
+        *bulk_nodes = total;
+    }
+    return 0;
+}
+
 int mplbhip_update_macros(mplbhip_ctx* c) {
     if (!c) return MPLBHIP_ERR_ARG;
     if (int rc = need_device(c)) return rc;

@@ -82,6 +82,7 @@ def load():
         "mplbhip_define_block_collision": (I, [P, I, D, D]),
         "mplbhip_iterate_batch": (I, [P, D, I, ctypes.c_longlong, LLP, DP, LLP]),
         "mplbhip_set_batch_restart": (I, [P, D]),
+        "mplbhip_launch_shape": (I, [P, IP, IP, IP, LLP]),
         "mplbhip_batch_restarts": (I, [P, IP]),
     }
     for name, (res, args) in sig.items():
@@ -255,6 +256,15 @@ class LatticeCase:
             steps.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), _dp(res),
             ctypes.byref(done)))
         return steps, res, done.value
+
+    def launch_shape(self):
+        """(fold level, bulk workgroup shape, two-node FP16 kernel, nodes one
+        bulk launch updates) of the step in force."""
+        f, x, h = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        nodes = ctypes.c_longlong()
+        self._ck(self.lib.mplbhip_launch_shape(self.h, ctypes.byref(f), ctypes.byref(x),
+                                               ctypes.byref(h), ctypes.byref(nodes)))
+        return f.value, x.value, h.value, nodes.value
 
     def set_batch_restart(self, factor):
         """Restart, once, a batched instance whose residual climbs above

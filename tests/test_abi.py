@@ -120,3 +120,27 @@ def test_define_order_errors():
     p.add_connection(0, BS.Left, 0, BS.Right, VT.MDPeriodic)
     with pytest.raises(hip.MplbError, match="already defined"):
         p.add_connection(0, BS.Left, 0, BS.Right, VT.MDPeriodic)
+
+
+@pytest.mark.parametrize("fold", ["0", "1", "2"])
+def test_launch_shape_counts_bulk_nodes(monkeypatch, fold):
+    """The nodes one bulk launch updates: the interior of planes 1 .. n-2 at
+    fold level 0; plus the x-face walls (simple EQM regions) at level 1;
+    at level 2 at most whole planes."""
+    monkeypatch.setenv("MPLBHIP_XFACE_FOLD", fold)
+    g = hip.LatticeCase(3, storage=hip.STORE_F32, device=-1)
+    nx, ny, nz = 20, 16, 12
+    g.add_block(0, (nx, ny, nz))
+    g.set_collision(1.0, force_type=0)
+    for s in ("Left", "Right", "Top", "Bottom", "Front", "Back"):
+        g.add_boundary(0, cases.BS[s], BSch.EQMDiffuseRefl, (0.0, 0.0, 0.0), VT.Wall)
+    g.prepare()
+    f, _, h2, nodes = g.launch_shape()
+    assert f == int(fold) and h2 == 0
+    interior = (nx - 2) * (ny - 2) * (nz - 2)
+    if fold == "0":
+        assert nodes == interior
+    elif fold == "1":
+        assert nodes == nx * (ny - 2) * (nz - 2)
+    else:
+        assert nx * (ny - 2) * (nz - 2) <= nodes <= nx * ny * (nz - 2)
