@@ -349,6 +349,15 @@ __device__ __forceinline__ BlockParams load_block(const KParams& kp, int b) {
     return B;
 }
 
+// the pointer held by the first active lane, as a wave-uniform value
+template <typename T>
+__device__ __forceinline__ const T* first_lane_ptr(const T* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return reinterpret_cast<const T*>(((uint64_t)hi << 32) | lo);
+}
+
 __device__ __forceinline__ int load_parity(const KParams& kp, int b) {
     return const_view(kp.cur_in)[b];
 }
@@ -804,7 +813,27 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
                     if constexpr (ST != MPLBHIP_STORE_F64) f[q] += T(wgt(D, q));
                     f[q] = (pull >> q & 1u) ? f[q] : T(0);
                 }
-                for (int o = 0; o < P->nops; o++) apply_eqm<D, T>(f, P->op[o]);
+                // the folded lanes of a wave (one wall face, or one row of
+                // it) share one or a few region programs: taken one program
+                // at a time, its ops are wave-uniform, read by scalar loads,
+                // and the per-population choices of the EQM application
+                // become scalar branches instead of every alternative
+                // computed under masks.  Every pass retires at least the
+                // first pending lane.
+                bool pending = true;
+                while (pending) {
+                    const RegionProg* Pu = first_lane_ptr(P);
+                    if (P == Pu) {
+                        const int nops = const_view(Pu)->nops;
+                        for (int o = 0; o < nops; o++) {
+                            BCOp op;
+                            __builtin_memcpy(&op, (const BCOp*)&const_view(Pu)->op[o],
+                                             sizeof op);
+                            apply_eqm<D, T>(f, op);
+                        }
+                        pending = false;
+                    }
+                }
             }
             finish_node<D, ST, MODE>(kp, B, f, P != nullptr, P ? P->force != 0 : true, node,
                                      mnode, next, x, y, z, acc);
