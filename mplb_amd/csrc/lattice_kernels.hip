@@ -184,9 +184,8 @@ __device__ __forceinline__ int region_code(int x, int n) {
     return x == 0 ? 1 : (x == n - 1 ? 2 : 0);
 }
 
-template <int D>
-__device__ __forceinline__ int64_t persist_slot(const RegionProg& P, int x,
-                                                int y, int z) {
+template <int D, typename PR = RegionProg>
+__device__ __forceinline__ int64_t persist_slot(const PR& P, int x, int y, int z) {
     const int l0 = P.ext[0] > 1 ? x - 1 : 0;
     const int l1 = P.ext[1] > 1 ? y - 1 : 0;
     const int l2 = (D == 3 && P.ext[2] > 1) ? z - 1 : 0;
@@ -220,9 +219,11 @@ __device__ __forceinline__ void solid_residual(const KParams& kp, int64_t mnode,
 
 // full-value state of a node at the start of the boundary stage: streamed
 // populations from the neighbours, unstreamed ones from the persist buffer
-template <int D, int ST>
+// PR: RegionProg, or the constant-address-space RegionProg of a
+// wave-uniform program (scalar loads)
+template <int D, int ST, typename PR = RegionProg>
 __device__ void gather_masked(const KParams& kp, const BlockParams& B,
-                              const RegionProg& P, int x, int y, int z,
+                              const PR& P, int x, int y, int z,
                               typename Store<ST>::T* f, const void* fs_prev,
                               const void* per_prev, uint32_t om) {
     using T = typename Store<ST>::T;
@@ -231,7 +232,7 @@ __device__ void gather_masked(const KParams& kp, const BlockParams& B,
     const E* prev = static_cast<const E*>(fs_prev) + B.base;
     const int32_t node = z * B.sz + y * B.sx + x;
     const T* per = static_cast<const T*>(per_prev);
-    int64_t slot = P.persist >= 0 ? persist_slot<D>(P, x, y, z) : 0;
+    int64_t slot = P.persist >= 0 ? persist_slot<D, PR>(P, x, y, z) : 0;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         T v;
@@ -274,14 +275,15 @@ __device__ void neighbour_values(const KParams& kp, const BlockParams& B,
         if (op.m_ext >> q & 1u) f[q] = g[q];
 }
 
-template <int D, int ST>
+template <int D, int ST, typename PR = RegionProg>
 __device__ void boundary_program(const KParams& kp, const BlockParams& B,
-                                 const RegionProg& P, int x, int y, int z,
+                                 const PR& P, int x, int y, int z,
                                  typename Store<ST>::T* f, const void* fs_prev,
                                  const void* per_prev) {
     using T = typename Store<ST>::T;
     for (int o = 0; o < P.nops; o++) {
-        const BCOp& op = P.op[o];
+        BCOp op;
+        __builtin_memcpy(&op, (const BCOp*)&P.op[o], sizeof op);
         if (op.type == OP_EQM) {
             apply_eqm<D, T>(f, op);
         } else if (op.type == OP_EXTRAPOL) {
@@ -574,10 +576,12 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
         b = lo;
         const BlockParams B = load_block(kp, b);
         const int i = (wg - B.bnd_wg_start[v]) * kBndWG + threadIdx.y * kTX + threadIdx.x;
+        // segments start on workgroup boundaries: a lane past its
+        // segment's end (padding) finds the next segment starting after it
         int s = B.seg_start[v];
         const int send = B.seg_start[v] + B.seg_count[v];
         while (s < send && i >= kp.segs[s].start + kp.segs[s].count) s++;
-        active = s < send && B.active;
+        active = s < send && i >= kp.segs[s].start && B.active;
         x = y = z = 0;
         if (active) {
             const BndSeg& S = kp.segs[s];
@@ -687,9 +691,21 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
                 else f[q] = *p;
             }
         } else {
-            gather_masked<D, ST>(kp, B, *P, x, y, z, f, fs_prev, per_prev, om);
+            // in the boundary kernel every active lane of a workgroup holds
+            // the same region program (segments start on workgroup
+            // boundaries): as a wave-uniform pointer its masks and ops are
+            // scalar values and its per-population choices scalar branches
+            if constexpr (SCOPE == SCOPE_BND) {
+                const MPLB_CONST RegionProg& Pg = *const_view(first_lane_ptr(P));
+                gather_masked<D, ST, MPLB_CONST RegionProg>(kp, B, Pg, x, y, z, f, fs_prev,
+                                                            per_prev, om);
+                boundary_program<D, ST, MPLB_CONST RegionProg>(kp, B, Pg, x, y, z, f,
+                                                               fs_prev, per_prev);
+            } else {
+                gather_masked<D, ST>(kp, B, *P, x, y, z, f, fs_prev, per_prev, om);
+                boundary_program<D, ST>(kp, B, *P, x, y, z, f, fs_prev, per_prev);
+            }
             full = true;
-            boundary_program<D, ST>(kp, B, *P, x, y, z, f, fs_prev, per_prev);
             if constexpr (MODE != MODE_MATERIALIZE) {
                 if (P->persist >= 0) {
                     T* per = static_cast<T*>(per_next);

@@ -803,8 +803,8 @@ static int assign_tiles(mplbhip_ctx* c, bool only_active) {
             if (!on) continue;
             int count = 0;
             for (int k = B.seg_start[v]; k < B.seg_start[v] + B.seg_count[v]; k++)
-                count += c->segs[(size_t)k].count;
-            bnd_v[v] += (count + kBndWG - 1) / kBndWG;
+                count += (c->segs[(size_t)k].count + kBndWG - 1) / kBndWG;
+            bnd_v[v] += count;
         }
     }
     c->ntiles = tiles;
@@ -909,10 +909,12 @@ int mplbhip_prepare(mplbhip_ctx* c) {
                         S.ext[a] = code[a] == 0 ? n[a] - 2 : 1;
                     }
                 }
+                // every segment starts on a workgroup boundary, so a
+                // workgroup's lanes share one region program
                 S.start = count;
                 S.count = S.ext[0] * S.ext[1] * S.ext[2];
                 if (S.count <= 0) continue;
-                count += S.count;
+                count += (S.count + kBndWG - 1) / kBndWG * kBndWG;
                 c->segs.push_back(S);
             }
             B.seg_count[v] = (int)c->segs.size() - B.seg_start[v];
