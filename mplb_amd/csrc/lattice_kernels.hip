@@ -505,8 +505,23 @@ __device__ __forceinline__ void finish_node(const KParams& kp, const BlockParams
         } else {
             const float om = B.fomega, om1 = B.fone_minus_omega;
             const bool addf = force_node && kp.force_type == 1;
+            auto store = [&](int q, float gn) {
+                E* p = at_u(next, (int64_t)q * B.sq, (uint32_t)node);
+                if constexpr (ST == MPLBHIP_STORE_F16)
+                    __builtin_nontemporal_store(
+                        __half_as_ushort(__float2half_rn(gn * kp.inv_scale[q])),
+                        reinterpret_cast<unsigned short*>(p));
+                else __builtin_nontemporal_store(gn, p);
+            };
+            // opposite velocities share the even part of the equilibrium:
+            // geq(q), geq(opp q) = A +- Bq with A = w (drho + rho (4.5 cu^2 -
+            // 1.5 u^2)), Bq = 3 w rho cu, and the force term changes sign
+            const float c0 = drho + rho * (-1.5f * usq);
+            store(0, om1 * f[0] + om * (float(wgt(D, 0)) * c0));
 #pragma unroll
-            for (int q = 0; q < Q; q++) {
+            for (int q = 1; q < Q; q++) {
+                const int p = opp(D, q);
+                if (p < q) continue;
                 float cu = 0.f;
 #pragma unroll
                 for (int d = 0; d < D; d++) {
@@ -514,21 +529,20 @@ __device__ __forceinline__ void finish_node(const KParams& kp, const BlockParams
                     if (cvel(D, q, d) < 0) cu -= ul[d];
                 }
                 const float wq = float(wgt(D, q));
-                const float geq =
-                    wq * (drho + rho * (3.f * cu + 4.5f * cu * cu - 1.5f * usq));
-                float gn = om1 * f[q] + om * geq;
+                const float a = wq * (c0 + rho * (4.5f * cu * cu));
+                const float bq = wq * rho * (3.f * cu);
+                float gq = om1 * f[q] + om * (a + bq);
+                float gp = om1 * f[p] + om * (a - bq);
                 if (addf) {
                     float cg = 0.f;
 #pragma unroll
                     for (int d = 0; d < D; d++) cg += float(cvel(D, q, d)) * float(B.g[d]);
-                    gn += B.fforce * wq * rho * cg;
+                    const float fq = B.fforce * wq * rho * cg;
+                    gq += fq;
+                    gp -= fq;
                 }
-                E* p = at_u(next, (int64_t)q * B.sq, (uint32_t)node);
-                if constexpr (ST == MPLBHIP_STORE_F16)
-                    __builtin_nontemporal_store(
-                        __half_as_ushort(__float2half_rn(gn * kp.inv_scale[q])),
-                        reinterpret_cast<unsigned short*>(p));
-                else __builtin_nontemporal_store(gn, p);
+                store(q, gq);
+                store(p, gp);
             }
         }
     }
