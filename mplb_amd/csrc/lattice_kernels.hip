@@ -60,10 +60,16 @@ template <> struct Store<MPLBHIP_STORE_F64> { using E = double; using T = double
 template <> struct Store<MPLBHIP_STORE_F32> { using E = float; using T = float; };
 template <> struct Store<MPLBHIP_STORE_F16> { using E = __half; using T = float; };
 
+// a stored FP16 value times its population scale: one mixed-precision
+// multiply-add (v_fma_mix) instead of a conversion and a multiply
+__device__ __forceinline__ float h2f_scaled(__half h, float sc) {
+    return __builtin_fmaf(__half2float(h), sc, 0.0f);
+}
+
 template <int ST>
 __device__ __forceinline__ typename Store<ST>::T ld(const typename Store<ST>::E* p,
                                                     int64_t i, float sc) {
-    if constexpr (ST == MPLBHIP_STORE_F16) return __half2float(p[i]) * sc;
+    if constexpr (ST == MPLBHIP_STORE_F16) return h2f_scaled(p[i], sc);
     else return p[i];
 }
 template <int ST>
@@ -701,7 +707,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
                                  ? at_u(prev, (int64_t)opp(D, q) * B.sq, (uint32_t)node)
                                  : at_u(prev, (int64_t)q * B.sq - pull_delta<D>(B, q),
                                         (uint32_t)node);
-                if constexpr (ST == MPLBHIP_STORE_F16) f[q] = __half2float(*p) * kp.scale[q];
+                if constexpr (ST == MPLBHIP_STORE_F16) f[q] = h2f_scaled(*p, kp.scale[q]);
                 else f[q] = *p;
             }
         } else {
@@ -832,7 +838,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
                 }
                 const E* p = bounce ? at_u(prev, (int64_t)opp(D, q) * B.sq, (uint32_t)node)
                                     : at_u(prev, (int64_t)q * B.sq - pull_delta<D>(B, q), nd);
-                if constexpr (ST == MPLBHIP_STORE_F16) f[q] = __half2float(*p) * kp.scale[q];
+                if constexpr (ST == MPLBHIP_STORE_F16) f[q] = h2f_scaled(*p, kp.scale[q]);
                 else f[q] = *p;
             }
             if (FOLD && P) {
