@@ -335,9 +335,15 @@ __device__ __forceinline__ int logical_wg(int wg, int n) {
 // workgroup -> tile such that the tx tiles of one row go to the same XCD
 // (workgroups b, b+8, b+16, ... share an XCD) while consecutive rows are
 // spread over the 8 XCDs; a bijection when the grid is a multiple of 8*tx
-__device__ __forceinline__ int row_group_wg(int b, int tx) {
+__device__ __forceinline__ int fast_div(int n, uint32_t m, int sh) {
+    return (int)(((uint64_t)(uint32_t)n * m) >> sh);
+}
+
+// (the division by tx as the block's multiply-high-and-shift pair m, sh)
+__device__ __forceinline__ int row_group_wg(int b, int tx, uint32_t m, int sh) {
     const int j = b >> 3;
-    return ((j / tx) * 8 + (b & 7)) * tx + j % tx;
+    const int jq = fast_div(j, m, sh);
+    return (jq * 8 + (b & 7)) * tx + (j - jq * tx);
 }
 
 // Block parameters and parities are never written while a step kernel
@@ -474,7 +480,7 @@ __device__ __forceinline__ void finish_node(const KParams& kp, const BlockParams
             }
         }
         const float rho = 1.f + drho;
-        const float inv = 1.f / rho;
+        const float inv = __builtin_amdgcn_rcpf(rho);  // v_rcp_f32, 1 ulp
         float ul[3] = {0.f, 0.f, 0.f};
 #pragma unroll
         for (int d = 0; d < D; d++) {
@@ -573,11 +579,11 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
         wg = logical_wg(blockIdx.x, gridDim.x);
         b = find_block<&BlockParams::tile_start>(kp, wg);
         const BlockParams B = load_block(kp, b);
-        int t = wg - B.tile_start;
-        const int tx = t % B.tiles_x;
-        t /= B.tiles_x;
-        const int ty = t % B.tiles_y;
-        z = t / B.tiles_y;
+        const int t = wg - B.tile_start;
+        const int t1 = fast_div(t, B.div_m[0], B.div_sh[0]);
+        const int tx = t - t1 * B.tiles_x;
+        z = fast_div(t1, B.div_m[1], B.div_sh[1]);
+        const int ty = t1 - z * B.tiles_y;
         x = tx * kTX + threadIdx.x;
         y = ty * kTY + threadIdx.y;
         active = x < B.n[0] && y < B.n[1] && B.active;
@@ -764,23 +770,26 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
     int t = wg - B.bulk_tile_start;
     // a block whose first tile sits on XCD 0 and whose tile count is a
     // multiple of 8 rows keeps each row's tiles on one XCD
-    if (kp.xgroup == 1 && B.xgroup) t = row_group_wg(t, B.xgroup);
+    // (xgroup equals tiles_x: its divisor pair is tiles_x's)
+    if (kp.xgroup == 1 && B.xgroup) t = row_group_wg(t, B.xgroup, B.div_m[0], B.div_sh[0]);
     const bool first = t == 0;
     int x, y, z;
     if (kp.xgroup == 2 && B.rowwg) {
         // kTY consecutive x tiles of one row: the x-shifted pulls' straddled
         // lines are shared by waves of the same workgroup
         const int per_row = B.tiles_x / kTY;
-        const int tr = t % per_row;
-        t /= per_row;
-        y = t % B.n[1];
-        z = D == 3 ? 1 + t / B.n[1] : 0;
+        const int t1 = fast_div(t, B.div_m[3], B.div_sh[3]);
+        const int tr = t - t1 * per_row;
+        const int t2 = fast_div(t1, B.div_m[2], B.div_sh[2]);
+        y = t1 - t2 * B.n[1];
+        z = D == 3 ? 1 + t2 : 0;
         x = (tr * kTY + (int)threadIdx.y) * kTX + threadIdx.x;
     } else {
-        const int tx = t % B.tiles_x;
-        t /= B.tiles_x;
-        const int ty = t % B.tiles_y;
-        z = D == 3 ? 1 + t / B.tiles_y : 0;
+        const int t1 = fast_div(t, B.div_m[0], B.div_sh[0]);
+        const int tx = t - t1 * B.tiles_x;
+        const int t2 = fast_div(t1, B.div_m[1], B.div_sh[1]);
+        const int ty = t1 - t2 * B.tiles_y;
+        z = D == 3 ? 1 + t2 : 0;
         x = tx * kTX + threadIdx.x;
         y = ty * kTY + threadIdx.y;
     }
@@ -964,7 +973,7 @@ __device__ __forceinline__ void collide_dev(const KParams& kp, const BlockParams
         }
     }
     const float rho = 1.f + drho;
-    const float inv = 1.f / rho;
+    const float inv = __builtin_amdgcn_rcpf(rho);  // v_rcp_f32, 1 ulp
     float ul[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int d = 0; d < D; d++) {

@@ -48,7 +48,24 @@ struct BlockParams {
     int32_t rowwg;            // 1: tiles_x is a multiple of kTY and n[1] of kTY,
                               // so a bulk workgroup can take kTY tiles of one row
     int32_t has_obst;         // the block has obstacle link masks
+    // divisors of the bulk tile index, each as a multiply-high-and-shift
+    // pair (fast_div): [0] tiles_x, [1] tiles_y, [2] n[1], [3] tiles_x / kTY
+    uint32_t div_m[4];
+    int32_t div_sh[4];
 };
+
+// n / d for 0 <= n < 2^31 as floor(n * m / 2^sh) (Granlund and Montgomery,
+// "Division by invariant integers using multiplication", 1994): with
+// l = ceil(log2 d), sh = 31 + l and m = ceil(2^sh / d) < 2^32, the excess
+// m d - 2^sh lies in [0, d) and so below 2^l, which keeps the quotient exact
+// for every n < 2^31.
+inline void fast_div_params(int32_t d, uint32_t* m, int32_t* sh) {
+    if (d < 1) d = 1;
+    int l = 0;
+    while ((1LL << l) < d) l++;
+    *m = (uint32_t)(((1ULL << (31 + l)) + (uint64_t)d - 1) / (uint64_t)d);
+    *sh = 31 + l;
+}
 
 // Obstacle link mask of an interior node (extension: in-domain obstacles
 // with half-way bounce-back): bit q set = the pull source x - c_q is an

@@ -885,6 +885,8 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         B.tiles_x = (n[0] + kTX - 1) / kTX;
         B.tiles_y = (n[1] + kTY - 1) / kTY;
         B.tiles_x2 = (n[0] + 2 * kTX - 1) / (2 * kTX);
+        const int32_t divisors[4] = {B.tiles_x, B.tiles_y, (int32_t)n[1], B.tiles_x / kTY};
+        for (int k = 0; k < 4; k++) fast_div_params(divisors[k], &B.div_m[k], &B.div_sh[k]);
         // the step's split: bulk tiles over z = 1 .. n-2 (3D), and the
         // boundary regions as segments, 256 nodes per workgroup, in three
         // lists: [0] every boundary region, [1] without the simple x-face
