@@ -52,6 +52,15 @@ struct BlockParams {
     // pair (fast_div): [0] tiles_x, [1] tiles_y, [2] n[1], [3] tiles_x / kTY
     uint32_t div_m[4];
     int32_t div_sh[4];
+    // the bulk kernel's population accesses as raw buffer accesses: one
+    // resource per array, the lane's node offset, and a scalar byte offset
+    // per population.  Loads use the resource base prev - pull_shift
+    // elements, which keeps every offset non-negative: pull_off[q] =
+    // (q sq - delta_q + pull_shift) * elem_bytes with delta_q = c_qx +
+    // c_qy sx + c_qz sz; stores use next itself, put_off[q] = q sq * elem_bytes.
+    int32_t pull_shift;
+    uint32_t pull_off[19];
+    uint32_t put_off[19];
 };
 
 // n / d for 0 <= n < 2^31 as floor(n * m / 2^sh) (Granlund and Montgomery,

@@ -882,6 +882,24 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         B.sq = (int32_t)sq;
         B.base = off + hz * sz + sx + ox;
         off += (elems + 255) / 256 * 256;
+        {
+            const int (*cv)[3] = c->dim == 3 ? kD3Q19 : kD2Q9;
+            const int64_t shift = (c->dim == 3 ? sz : 0) + sx + 1;
+            B.pull_shift = (int32_t)shift;
+            for (int q = 0; q < c->Q; q++) {
+                const int64_t delta = cv[q][0] + cv[q][1] * sx + cv[q][2] * sz;
+                const int64_t pull = (q * sq - delta + shift) * (int64_t)c->es;
+                const int64_t put = q * sq * (int64_t)c->es;
+                // the lane offset (below elems * es) plus the scalar one stays
+                // below 4 GiB, so no 32-bit offset sum can wrap
+                if (pull < 0 || pull + elems * (int64_t)c->es >= (1LL << 32) ||
+                    put + elems * (int64_t)c->es >= (1LL << 32))
+                    return fail(c, MPLBHIP_ERR_UNSUPPORTED,
+                                "block %d too large for 32-bit population offsets", b);
+                B.pull_off[q] = (uint32_t)pull;
+                B.put_off[q] = (uint32_t)put;
+            }
+        }
         B.tiles_x = (n[0] + kTX - 1) / kTX;
         B.tiles_y = (n[1] + kTY - 1) / kTY;
         B.tiles_x2 = (n[0] + 2 * kTX - 1) / (2 * kTX);
