@@ -870,11 +870,9 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
             // lane's unpulled source is moved back inside the block along
             // the axes where it left it: that row is one the wave (or its
             // workgroup) loads anyway, so no halo line is fetched for it.
-            // FP32 without obstacles: the populations as buffer loads from
-            // prev - pull_shift, every pull offset a scalar.  (FP16 keeps
-            // global loads: its buffer form ran 3% slower, 49.6k against
-            // 51.1k MLUPS at 256^3, though it issues fewer instructions.)
-            constexpr bool kBufPull = ST == MPLBHIP_STORE_F32 && !OBST;
+            // FP32 and FP16 without obstacles: the populations as buffer
+            // loads from prev - pull_shift, every pull offset a scalar
+            constexpr bool kBufPull = ST != MPLBHIP_STORE_F64 && !OBST;
             const __amdgpu_buffer_rsrc_t rpull =
                 buf_rsrc(reinterpret_cast<uint64_t>(prev) - (uint64_t)B.pull_shift * sizeof(E));
             int32_t adj[3] = {0, 0, 0};
@@ -933,6 +931,8 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
                     }
                 }
             }
+            // buffer stores for FP32 only: 2-byte buffer stores ran 7%
+            // slower than global ones (FP16 47.7k against 51.3k MLUPS)
             finish_node<D, ST, MODE, ST == MPLBHIP_STORE_F32>(
                 kp, B, f, P != nullptr, P ? P->force != 0 : true, node, mnode, next, x, y, z,
                 acc);
