@@ -1307,8 +1307,14 @@ static int run_steps(mplbhip_ctx* c, int nsteps, bool timed, double* ms_total,
         if (ms_kernel) *ms_kernel = ktot;
         return 0;
     }
-    // untimed: replay captured graphs of G steps, the remainder one by one
-    const int G = 32;
+    // untimed: replay captured graphs of G steps (even, so the parity after
+    // a graph is unchanged; MPLBHIP_GRAPH_STEPS overrides 32), the
+    // remainder one by one
+    int G = 32;
+    if (const char* e = std::getenv("MPLBHIP_GRAPH_STEPS")) {
+        const int g = std::atoi(e);
+        if (g >= 2 && g <= 1024 && g % 2 == 0) G = g;
+    }
     int left = nsteps;
     if (left >= G) {
         if (c->graph_steps != G)
