@@ -1076,6 +1076,12 @@ static void boundaries(orc_case* c) {
     }
 }
 
+int orc_apply_boundaries(orc_case* c) {
+    if (!c->prepared) return fail(c, "call orc_prepare first");
+    boundaries(c);
+    return 0;
+}
+
 int orc_step(orc_case* c, int nsteps) {
     if (!c->prepared) return fail(c, "call orc_prepare first");
     for (int s = 0; s < nsteps; s++) {

@@ -62,6 +62,9 @@ void orc_set_dt(orc_case* c, double dt);
 int orc_set_accel(orc_case* c, int block, const double* g);
 /* StreamCollision x n (evolution.cpp:137-193) */
 int orc_step(orc_case* c, int nsteps);
+/* test hook: the boundary stage of a step alone (ImplementBoundary on the
+ * current f, every BlockBoundary in definition order) */
+int orc_apply_boundaries(orc_case* c);
 /* UpdateMacroVars[3D] (model_wrapper.cpp:112-275, 468-588) */
 void orc_update_macros(orc_case* c);
 /* distribution function of one block, interior only, layout [q][k][j][i] */

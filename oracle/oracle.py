@@ -47,6 +47,7 @@ def load():
         "orc_set_dt": (None, [P, D]),
         "orc_set_accel": (I, [P, I, DP]),
         "orc_step": (I, [P, I]),
+        "orc_apply_boundaries": (I, [P]),
         "orc_update_macros": (None, [P]),
         "orc_get_f": (I, [P, I, DP]),
         "orc_set_f": (I, [P, I, DP]),
@@ -155,6 +156,10 @@ class OracleCase:
 
     def update_macros(self):
         self.lib.orc_update_macros(self.h)
+
+    def apply_boundaries(self):
+        """The boundary stage of a step alone, on the current f."""
+        self._ck(self.lib.orc_apply_boundaries(self.h))
 
     def get_f(self, block):
         out = np.empty((self.Q,) + self._shape(block), dtype=np.float64)

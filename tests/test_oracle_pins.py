@@ -260,3 +260,394 @@ def test_extrapolation_rescale_skips_last_population():
         assert abs(ratio - 1.0) > 1e-9
         assert abs(node[:-1].sum() + ratio * node[-1] - 1.0) < 1e-14
         assert abs(node.sum() - 1.0) > 1e-12
+
+
+# ---- the residual (CalcResidualError + GetMaximumResidual) in closed form -------
+def _uniform_forced_box(be, dim, size, tau_l, dt, u0, g):
+    """A fully periodic box in a uniform state f = feq(1, u0) under
+    BodyForce_1st with the *_Force velocity macros (u = sum CS xi f / rho +
+    dt g / 2).  Streaming a uniform state is the identity, and the collision
+    adds exactly rho g dt of momentum per step: with omega = dt/(tau + dt/2)
+    the equilibrium shift contributes omega rho dt g / 2 and the source
+    tau omega rho g, together rho g dt."""
+    be.add_block(0, size)
+    pairs = [(BS.Left, BS.Right), (BS.Right, BS.Left), (BS.Top, BS.Bottom),
+             (BS.Bottom, BS.Top)]
+    if dim == 3:
+        pairs += [(BS.Front, BS.Back), (BS.Back, BS.Front)]
+    for a, b in pairs:
+        be.add_connection(0, a, 0, b, VT.MDPeriodic)
+    be.set_collision(cases.tau_ref_for(tau_l, dt), force_type=1, force_macros=True)
+    for a, _ in pairs:
+        be.add_boundary(0, a, BSch.None_, (), VT.MDPeriodic)
+    be.prepare()
+    be.set_accel(0, g)
+    shp = tuple(reversed(size))
+    be.set_macros(0, np.ones(shp), *[np.full(shp, x) for x in u0])
+    be.init_feq()
+    be.set_dt(dt)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_residual_closed_form(dim):
+    """Src/flowfield_wrapper.cpp:31-89 and Src/flowfield.cpp:148-158 on a flow
+    whose every macro changes by a known amount.  After a refresh of the copy
+    at t = 0 (u = u0 + dt g/2) and P further steps, u_d = u0_d + (P + 1/2) dt g_d
+    at every node and rho = 1, so the stored ratios are
+        sum (m - m_copy)^2 / sum m^2 = (P dt g_d)^2 / (u0_d + (P + 1/2) dt g_d)^2
+    (0 for rho), and GetMaximumResidual divides the largest by P dt.  A
+    residual that skipped the copy refresh, divided by P alone, summed the
+    copy's squares or kept the reduction handle's sum across the two
+    reductions (B7: the second result would be diff + sum) fails it."""
+    size = (6, 5) if dim == 2 else (6, 5, 4)
+    u0 = (0.02, -0.013, 0.011)[:dim]
+    g = (1e-4, 2e-4, -3e-4)[:dim]
+    dt, tau_l, P = 0.37, 0.8, 7
+    o = OracleCase(dim)
+    _uniform_forced_box(o, dim, size, tau_l, dt, u0, g)
+    o.update_macros()
+    o.residual(P)          # refresh MacroVarsCopy at u0 + dt g / 2
+    o.step(P)
+    o.update_macros()
+    for d in range(dim):   # the flow is uniform and moved by P dt g exactly
+        u = o.get_macro(0, 1 + d)
+        assert np.allclose(u, u0[d] + (P + 0.5) * dt * g[d], rtol=1e-12, atol=0)
+    res, per = o.residual(P)
+    exp = [0.0] + [(P * dt * g[d]) ** 2 / (u0[d] + (P + 0.5) * dt * g[d]) ** 2
+                   for d in range(dim)]
+    assert per[0] < 1e-28
+    assert np.allclose(per[1:], exp[1:], rtol=1e-9, atol=0), (per, exp)
+    assert math.isclose(res, max(exp) / (P * dt), rel_tol=1e-9)
+    # B7: a handle that kept its first sum would give diff / (diff + sum) and a
+    # maximum of order 1 / (P dt) from the second check on; the closed form is
+    # far below it, as every App's convergence loop needs
+    assert res < 1e-2 / (P * dt)
+
+
+# ---- multi-block junction halos (Field::CreateHalos, field.h:215-395) ----------
+_LOW = {0: BS.Left, 1: BS.Bottom, 2: BS.Back}
+_HIGH = {0: BS.Right, 1: BS.Top, 2: BS.Front}
+_ORDER = [BS.Left, BS.Right, BS.Top, BS.Bottom, BS.Front, BS.Back]
+
+
+def _split_box_links(dim, axis):
+    """The links of a periodic box split in two along `axis`: the two
+    junctions (A high <-> B low, and the wrap B high <-> A low) are
+    VirtualBoundary connections, the other axes self-linked periodic."""
+    links = [(0, _HIGH[axis], 1, _LOW[axis], VT.VirtualBoundary),
+             (1, _LOW[axis], 0, _HIGH[axis], VT.VirtualBoundary),
+             (1, _HIGH[axis], 0, _LOW[axis], VT.VirtualBoundary),
+             (0, _LOW[axis], 1, _HIGH[axis], VT.VirtualBoundary)]
+    for b in (0, 1):
+        for a in range(dim):
+            if a != axis:
+                links.append((b, _LOW[a], b, _HIGH[a], VT.MDPeriodic))
+                links.append((b, _HIGH[a], b, _LOW[a], VT.MDPeriodic))
+    return links
+
+
+def _build_split_box(be, dim, sizes, axis, tau_l, f_glob):
+    links = _split_box_links(dim, axis)
+    for b, s in enumerate(sizes):
+        be.add_block(b, s)
+    for frm, fs, to, ts, ty in links:
+        be.add_connection(frm, fs, to, ts, ty)
+    be.set_collision(cases.tau_ref_for(tau_l, 1.0))
+    for frm, fs, _, _, ty in links:
+        be.add_boundary(frm, fs, BSch.None_, (), ty)
+    be.prepare()
+    n0 = sizes[0][axis]
+    for b, s in enumerate(sizes):
+        shp = tuple(reversed(s))
+        z = np.zeros(shp)
+        be.set_macros(b, np.ones(shp), z, z, z if dim == 3 else None)
+    be.init_feq()
+    be.set_dt(1.0)
+    # the global off-equilibrium state, cut into the two blocks
+    sl = [slice(None)] * (dim + 1)
+    sl[dim - axis] = slice(0, n0)
+    be.set_f(0, f_glob[tuple(sl)])
+    sl[dim - axis] = slice(n0, None)
+    be.set_f(1, f_glob[tuple(sl)])
+
+
+def _halo_labels(dim, sizes, axis):
+    """Which block node's fresh fStage each halo cell holds after one
+    exchange, following OPS's halo group order (blocks by id, neighbours in
+    BoundarySurface order Left, Right, Top, Bottom, Front, Back) and the slab
+    geometry of Field::CreateHalos: the from-face plane, one node deep,
+    spanning the transverse halos, lands in the target's halo beyond the
+    opposite face.  Cells no copy has written yet hold the zero of a new
+    dat.  Labels: (block, i, j, k) of an interior node, or None."""
+    links = _split_box_links(dim, axis)
+    lab = []
+    for s in sizes:
+        n = list(s) + [1] * (3 - dim)
+        ext = [x + 2 for x in n[:dim]] + [1] * (3 - dim)
+        arr = np.empty(ext[::-1], dtype=object)
+        for idx in np.ndindex(*arr.shape):
+            p = [idx[2] - 1, idx[1] - 1, idx[0] - (1 if dim == 3 else 0)]
+            inside = all(0 <= p[a] < n[a] for a in range(3))
+            arr[idx] = (len(lab), p[0], p[1], p[2]) if inside else None
+        lab.append(arr)
+    hz = 1 if dim == 3 else 0
+
+    def at(b, p):
+        return (p[2] + hz, p[1] + 1, p[0] + 1)
+
+    for blk in range(2):
+        for surf in _ORDER:
+            for frm, fs, to, ts, _ in links:
+                if frm != blk or fs != surf:
+                    continue
+                a = [k for k, v in _LOW.items() if v == fs] + \
+                    [k for k, v in _HIGH.items() if v == fs]
+                a = a[0]
+                high = fs == _HIGH[a]
+                nf = list(sizes[frm]) + [1] * (3 - dim)
+                nt = list(sizes[to]) + [1] * (3 - dim)
+                src_plane = nf[a] - 1 if high else 0
+                dst_plane = -1 if high else nt[a]
+                rng = [range(-1, nf[d] + 1) if d < dim else range(0, 1) for d in range(3)]
+                rng[a] = [src_plane]
+                new = {}
+                for x in rng[0]:
+                    for y in rng[1]:
+                        for z in rng[2]:
+                            p = [x, y, z]
+                            t = list(p)
+                            t[a] = dst_plane
+                            new[tuple(t)] = lab[frm][at(frm, p)]
+                for t, v in new.items():
+                    lab[to][at(to, list(t))] = v
+    return lab
+
+
+@pytest.mark.parametrize("dim,axis,sizes", [
+    (2, 0, [(4, 5), (3, 5)]),
+    (2, 1, [(6, 3), (6, 4)]),
+    (3, 0, [(3, 4, 5), (4, 4, 5)]),
+    (3, 1, [(5, 3, 4), (5, 2, 4)]),
+    (3, 2, [(4, 5, 3), (4, 5, 3)]),
+])
+def test_split_box_junction_halos(dim, axis, sizes):
+    """Field::CreateHalos (field.h:215-395): a periodic box cut into two
+    blocks joined by VirtualBoundary connections takes one step from an
+    off-equilibrium state.  Every population equals the uncut box's, except
+    the ones pulled from halo cells that OPS's group order has not written
+    yet in the first exchange, which hold exactly the zero of a new dat; and
+    every halo cell that was written holds the periodic image of its
+    position (the slab offsets and extents).  A slab that skipped the
+    transverse halos, landed one plane off, or was copied in another order
+    fails it."""
+    rng = np.random.default_rng(11 + 3 * dim + axis)
+    glob = list(sizes[0])
+    glob[axis] += sizes[1][axis]
+    glob = tuple(glob)
+    Q = 9 if dim == 2 else 19
+    shp = tuple(reversed(glob))
+    c, w = lattice.lattice(dim)
+    f_glob = w.reshape((Q,) + (1,) * dim) * (1 + 0.2 * rng.random((Q,) + shp))
+    whole = OracleCase(dim)
+    cases.periodic_box(whole, glob, 0.8, np.ones(shp), *[np.zeros(shp)] * dim)
+    whole.set_f(0, f_glob)
+    split = OracleCase(dim)
+    _build_split_box(split, dim, sizes, axis, 0.8, f_glob)
+    whole.step(1)
+    split.step(1)
+    fw = whole.get_f(0)
+    lab = _halo_labels(dim, sizes, axis)
+    n0 = sizes[0][axis]
+    hz = 1 if dim == 3 else 0
+    stale = written = 0
+    for b in (0, 1):
+        fb = split.get_f(b)
+        n = list(sizes[b]) + [1] * (3 - dim)
+        for k in range(n[2]):
+            for j in range(n[1]):
+                for i in range(n[0]):
+                    g = [i, j, k]
+                    if b == 1:
+                        g[axis] += n0
+                    gidx = tuple(reversed(g[:dim]))
+                    for q in range(Q):
+                        cq = list(c[q]) + [0] * (3 - dim)
+                        s = [i - cq[0], j - cq[1], k - cq[2]]
+                        v = fb[(q,) + tuple(reversed([i, j, k][:dim]))]
+                        label = lab[b][(s[2] + hz, s[1] + 1, s[0] + 1)]
+                        if label is None:
+                            stale += 1
+                            assert v == 0.0, (b, i, j, k, q)
+                            continue
+                        # the label's global position is the periodic image
+                        # of the pulled source position
+                        lb, li, lj, lk = label
+                        lg = [li, lj, lk]
+                        if lb == 1:
+                            lg[axis] += n0
+                        sg = [g[0] - cq[0], g[1] - cq[1], g[2] - cq[2]]
+                        for a in range(dim):
+                            assert lg[a] == sg[a] % glob[a], (b, i, j, k, q, label)
+                        written += 1
+                        assert v == fw[(q,) + gidx], (b, i, j, k, q)
+    # the stale set is the junction rim, a small part; it is empty when every
+    # transverse axis comes before the cut axis in the group order (their
+    # self-links have then refreshed the transverse halos the slab spans)
+    assert (stale > 0) == (axis < dim - 1)
+    assert stale < 0.1 * written
+
+
+# ---- ExtrapolPressure1ST at edges and corners (boundary_kernel.inc) -----------
+# The copy rule as the reference kernels list it, case by case
+# (KerCutCellExtrapolPressure1ST, boundary_kernel.inc:404-488, and
+# KerCutCellExtrapolPressure1ST3D, :894-1182): the face part of a geometry
+# name, the surface whose definition selects it, and the neighbour it copies
+# from.  VG_IP is the face whose normal points along +x (the Left face) ...
+_FACE = {"IP": ("Left", (1, 0, 0)), "IM": ("Right", (-1, 0, 0)),
+         "JP": ("Bottom", (0, 1, 0)), "JM": ("Top", (0, -1, 0)),
+         "KP": ("Back", (0, 0, 1)), "KM": ("Front", (0, 0, -1))}
+_VG = {"IP": 10, "IM": 11, "JP": 20, "JM": 21, "KP": 30, "KM": 31}
+
+
+def _unknown(names, cv):
+    """The population test of the listed case: a face copies c . n > 0; an
+    edge '(a >= 0 && b > 0) || (a > 0 && b == 0)'; a corner '(a >= 0 && b >= 0
+    && c >= 0) && (a != 0 || b != 0 || c != 0)', with a, b, c the velocity
+    components along the inward normals of its faces."""
+    comp = [sum(_FACE[n][1][d] * cv[d] for d in range(3)) for n in names]
+    if len(comp) == 1:
+        return comp[0] > 0
+    if len(comp) == 2:
+        a, b = comp
+        return (a >= 0 and b > 0) or (a > 0 and b == 0)
+    a, b, c = comp
+    return (a >= 0 and b >= 0 and c >= 0) and (a != 0 or b != 0 or c != 0)
+
+
+def _expected_extrapolation(f, names, surface, pos, rho_given, cvs):
+    Q = f.shape[0]
+    node = [f[(q,) + pos] for q in range(Q)]
+    for q in range(Q):
+        if not _unknown(names, cvs[q]):
+            continue
+        if len(names) == 1:
+            off = _FACE[names[0]][1]           # a face copies whatever the surface
+        else:
+            offs = [_FACE[n][1] for n in names if _FACE[n][0] == surface]
+            if not offs:
+                continue
+            off = offs[0]
+        nb = tuple(p + o for p, o in zip(pos, reversed(off[:len(pos)])))
+        node[q] = f[(q,) + nb]
+    rho = 0.0
+    for q in range(Q):
+        rho += node[q]
+    ratio = rho_given / rho
+    return [node[q] * ratio if q < Q - 1 else node[q] for q in range(Q)]
+
+
+def _vg_names(dim, n, i, j, k):
+    names = []
+    if i == 0:
+        names.append("IP")
+    if i == n[0] - 1:
+        names.append("IM")
+    if j == 0:
+        names.append("JP")
+    if j == n[1] - 1:
+        names.append("JM")
+    if dim == 3 and k == 0:
+        names.append("KP")
+    if dim == 3 and k == n[2] - 1:
+        names.append("KM")
+    return names
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_extrapolation_edges_and_corners(dim):
+    """The outlet on each face in turn of a box whose other faces are Wall
+    nodes without a scheme (so they only type the edges), applied once to a
+    random f: every node of the face, its edges and corners included, equals
+    the reference's case list above, then the rescale of all but the last
+    population.  Also checks the geometry codes of those nodes."""
+    rng = np.random.default_rng(21 + dim)
+    n = (5, 6) if dim == 2 else (5, 6, 7)
+    surfaces = ["Left", "Right", "Top", "Bottom"] + (["Front", "Back"] if dim == 3 else [])
+    c, _ = lattice.lattice(dim)
+    cvs = [list(c[q]) + [0] * (3 - dim) for q in range(c.shape[0])]
+    rho_given = 1.03
+    for outlet in surfaces:
+        o = OracleCase(dim)
+        o.add_block(0, n)
+        o.set_collision(0.1)
+        for s in surfaces:
+            if s == outlet:
+                continue
+            o.add_boundary(0, BS[s], BSch.None_, (), VT.Wall)
+        o.add_boundary(0, BS[outlet], BSch.ExtrapolPressure1ST, (rho_given,), VT.OutLet)
+        o.prepare()
+        shp = tuple(reversed(n))
+        o.set_macros(0, np.ones(shp), *[np.zeros(shp)] * dim)
+        o.init_feq()
+        f = 0.02 + rng.random((o.Q,) + shp)
+        o.set_f(0, f)
+        vg = o.get_int(0, 1)
+        o.apply_boundaries()
+        out = o.get_f(0)
+        nn = list(n) + [1] * (3 - dim)
+        checked_edges = 0
+        for k in range(nn[2]):
+            for j in range(nn[1]):
+                for i in range(nn[0]):
+                    names = _vg_names(dim, nn, i, j, k)
+                    pos = (j, i) if dim == 2 else (k, j, i)
+                    if outlet not in [_FACE[x][0] for x in names]:
+                        assert np.array_equal(out[(slice(None),) + pos], f[(slice(None),) + pos])
+                        continue
+                    code = int("".join(str(_VG[x]) for x in names) + ("0" if len(names) > 1 else ""))
+                    assert vg[pos] == code, (outlet, pos, vg[pos], code)
+                    exp = _expected_extrapolation(f, names, outlet, pos, rho_given, cvs)
+                    got = out[(slice(None),) + pos]
+                    assert np.allclose(got, exp, rtol=2e-15, atol=0), (outlet, pos, names)
+                    checked_edges += len(names) > 1
+        # 2D: the face's two corners; 3D: its four edges and four corners
+        face_n = [nn[a] for a in range(dim) if BS[outlet] not in (_LOW[a], _HIGH[a])]
+        assert checked_edges == (2 if dim == 2 else 2 * sum(x - 2 for x in face_n) + 4)
+
+
+# the lattice tables as the reference lists them (Src/model.cpp:101-105 for
+# D2Q9, :116-123 for D3Q19), independent of mplb_amd.lattice
+_REF_D2Q9 = ([(0, 0), (-1, 1), (-1, 0), (-1, -1), (0, -1), (1, -1), (1, 0), (1, 1), (0, 1)],
+             [4 / 9, 1 / 36, 1 / 9, 1 / 36, 1 / 9, 1 / 36, 1 / 9, 1 / 36, 1 / 9])
+_REF_D3Q19 = (list(zip([0, -1, 0, 0, -1, -1, -1, -1, 0, 0, 1, 0, 0, 1, 1, 1, 1, 0, 0],
+                       [0, 0, -1, 0, -1, 1, 0, 0, -1, -1, 0, 1, 0, 1, -1, 0, 0, 1, 1],
+                       [0, 0, 0, -1, 0, 0, -1, 1, -1, 1, 0, 0, 1, 0, 0, 1, -1, 1, -1])),
+              [1 / 3] + [1 / 18] * 3 + [1 / 36] * 6 + [1 / 18] * 3 + [1 / 36] * 6)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_oracle_lattice_tables(dim):
+    """The oracle's own velocity and weight tables, read back through its
+    equilibrium: f(rho = 1, u = 0) is the weight list, and the odd part
+    (f(s e_a) - f(-s e_a)) / (2 CS s w_q) is the velocity component c_qa,
+    both in the reference's order."""
+    cref, wref = _REF_D2Q9 if dim == 2 else _REF_D3Q19
+    s = 1e-3
+    size = (2 * dim + 1, 3) if dim == 2 else (2 * dim + 1, 3, 3)
+    shp = tuple(reversed(size))
+    o = OracleCase(dim)
+    o.add_block(0, size)
+    o.set_collision(0.1)
+    o.prepare()
+    u = [np.zeros(shp) for _ in range(dim)]
+    for a in range(dim):   # column 2a+1: +s e_a, column 2a+2: -s e_a
+        u[a][..., 2 * a + 1] = s
+        u[a][..., 2 * a + 2] = -s
+    o.set_macros(0, np.ones(shp), *u)
+    o.init_feq()
+    f = o.get_f(0)
+    at = (lambda col: (slice(None), 1, col)) if dim == 2 else (lambda col: (slice(None), 1, 1, col))
+    assert np.array_equal(f[at(0)], np.array(wref))
+    for a in range(dim):
+        odd = (f[at(2 * a + 1)] - f[at(2 * a + 2)]) / (2 * CS * s * np.array(wref))
+        assert np.allclose(odd, [cq[a] for cq in cref], atol=1e-9), a
