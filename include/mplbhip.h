@@ -135,7 +135,9 @@ int mplbhip_step_timed(mplbhip_ctx* ctx, int nsteps, double* ms_total,
  * shares of the last mplbhip_step_timed call (ms, summed over its steps). */
 int mplbhip_kernel_times(mplbhip_ctx* ctx, double* ms_bulk, double* ms_boundary);
 /* Diagnostic, no reference counterpart: the step's launch shape in force
- * (after the first step's autotuning): fold level (0-2), bulk workgroup
+ * (after the first step's autotuning): fold level (0 none, 1 simple x faces,
+ * 2 every simple region of the planes z = 1 .. n-2, 3 every simple region
+ * of every plane, 3D only; MPLBHIP_XFACE_FOLD fixes it), bulk workgroup
  * shape (0 plain tiles, 1 rows dealt to one XCD, 2 row workgroups), 1 if
  * FP16 steps use the two-node kernel, and the nodes one bulk-kernel launch
  * updates (interior nodes plus the folded boundary nodes). */

@@ -641,14 +641,18 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
             if (const_view(kp.blocks)[mid].bnd_wg_start[v] <= wg) lo = mid; else hi = mid - 1;
         }
         b = lo;
-        const BlockParams B = load_block(kp, b);
-        const int i = (wg - B.bnd_wg_start[v]) * kBndWG + threadIdx.y * kTX + threadIdx.x;
+        // the fold level indexes the block's arrays at run time: read the
+        // three fields through scalar loads, not from a private copy of the
+        // block (a dynamically indexed struct lives in scratch memory, and
+        // every lane would write its 416-byte copy there)
+        const MPLB_CONST BlockParams& Bc = const_view(kp.blocks)[b];
+        const int i = (wg - Bc.bnd_wg_start[v]) * kBndWG + threadIdx.y * kTX + threadIdx.x;
         // segments start on workgroup boundaries: a lane past its
         // segment's end (padding) finds the next segment starting after it
-        int s = B.seg_start[v];
-        const int send = B.seg_start[v] + B.seg_count[v];
+        int s = Bc.seg_start[v];
+        const int send = s + Bc.seg_count[v];
         while (s < send && i >= kp.segs[s].start + kp.segs[s].count) s++;
-        active = s < send && i >= kp.segs[s].start && B.active;
+        active = s < send && i >= kp.segs[s].start && Bc.active;
         x = y = z = 0;
         if (active) {
             const BndSeg& S = kp.segs[s];
@@ -713,9 +717,9 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
             for (int q = 0; q < Q; q++) {
                 double fv;
                 if constexpr (MODE == MODE_INIT_FEQ) {
-                    fv = feq_ref<D, double>(q, kp.macro[0][mnode], kp.macro[1][mnode],
-                                            kp.macro[2][mnode],
-                                            D == 3 ? kp.macro[3][mnode] : 0.0);
+                    const double* const* m0 = kp.macro0[0] ? kp.macro0 : kp.macro;
+                    fv = feq_ref<D, double>(q, m0[0][mnode], m0[1][mnode], m0[2][mnode],
+                                            D == 3 ? m0[3][mnode] : 0.0);
                 } else {
                     const int64_t nn = (int64_t)B.n[0] * B.n[1] * B.n[2];
                     fv = kp.fhost[B.fbase + q * nn +
@@ -797,22 +801,27 @@ reduce:
 // regions (masked pull + EQM, nothing persisted) of the planes it covers,
 // whose rows share cache lines with the interior; the other boundary
 // regions stay in the boundary kernel, which keeps this one's register
-// budget small.  Tiles of kTX x kTY nodes, z planes 1 .. n-2 (3D).  The
-// block's first bulk workgroup also records the block's parity after the
-// step.
+// budget small.  Tiles of kTX x kTY nodes, z planes 1 .. n-2 (3D), or every
+// z plane at fold level 3 (whose z-face planes then run their simple
+// regions here too).  The block's first bulk workgroup also records the
+// block's parity after the step.
 template <int D, int ST, int MODE, bool OBST, int FOLD>
 __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
     using T = typename Store<ST>::T;
     using E = typename Store<ST>::E;
     constexpr int Q = D == 2 ? 9 : 19;
+    // fold level 3 spans every z plane (3D): its grid has its own tile starts
+    constexpr bool ZALL = FOLD == 3 && D == 3;
     const int wg = kp.xcd_remap ? logical_wg(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    const int b = find_block<&BlockParams::bulk_tile_start>(kp, wg);
+    const int b = ZALL ? find_block<&BlockParams::bulk_tile_start3>(kp, wg)
+                       : find_block<&BlockParams::bulk_tile_start>(kp, wg);
     const BlockParams B = load_block(kp, b);
-    int t = wg - B.bulk_tile_start;
+    int t = wg - (ZALL ? B.bulk_tile_start3 : B.bulk_tile_start);
     // a block whose first tile sits on XCD 0 and whose tile count is a
     // multiple of 8 rows keeps each row's tiles on one XCD
     // (xgroup equals tiles_x: its divisor pair is tiles_x's)
-    if (kp.xgroup == 1 && B.xgroup) t = row_group_wg(t, B.xgroup, B.div_m[0], B.div_sh[0]);
+    const int xg = ZALL ? B.xgroup3 : B.xgroup;
+    if (kp.xgroup == 1 && xg) t = row_group_wg(t, xg, B.div_m[0], B.div_sh[0]);
     const bool first = t == 0;
     int x, y, z;
     if (kp.xgroup == 2 && B.rowwg) {
@@ -823,14 +832,14 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
         const int tr = t - t1 * per_row;
         const int t2 = fast_div(t1, B.div_m[2], B.div_sh[2]);
         y = t1 - t2 * B.n[1];
-        z = D == 3 ? 1 + t2 : 0;
+        z = D == 3 ? (ZALL ? t2 : 1 + t2) : 0;
         x = (tr * kTY + (int)threadIdx.y) * kTX + threadIdx.x;
     } else {
         const int t1 = fast_div(t, B.div_m[0], B.div_sh[0]);
         const int tx = t - t1 * B.tiles_x;
         const int t2 = fast_div(t1, B.div_m[1], B.div_sh[1]);
         const int ty = t1 - t2 * B.tiles_y;
-        z = D == 3 ? 1 + t2 : 0;
+        z = D == 3 ? (ZALL ? t2 : 1 + t2) : 0;
         x = tx * kTX + threadIdx.x;
         y = ty * kTY + threadIdx.y;
     }
@@ -842,12 +851,14 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
     // lanes: interior nodes, and (FOLD) the face and edge nodes of this
     // plane whose region program is simple (masked pull + EQM): their loads
     // share the wave's cache lines, which a separate kernel would fetch again
-    const bool interior = x >= 1 && x < B.n[0] - 1 && y >= 1 && y < B.n[1] - 1;
+    const bool interior = x >= 1 && x < B.n[0] - 1 && y >= 1 && y < B.n[1] - 1 &&
+                          (!ZALL || (z >= 1 && z < B.n[2] - 1));
     const RegionProg* P = nullptr;
     if (FOLD && !interior && x < B.n[0] && y < B.n[1]) {
         const bool yin = y >= 1 && y < B.n[1] - 1;
-        if (FOLD == 2 || yin) {
-            P = &kp.progs[B.prog_base + region_code(x, B.n[0]) + 3 * region_code(y, B.n[1])];
+        if (FOLD >= 2 || yin) {
+            P = &kp.progs[B.prog_base + region_code(x, B.n[0]) + 3 * region_code(y, B.n[1]) +
+                          (ZALL ? 9 * region_code(z, B.n[2]) : 0)];
             if (!P->simple) P = nullptr;
         }
     }
@@ -879,6 +890,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
             if (FOLD && P) {
                 adj[0] = x == 0 ? 1 : (x == B.n[0] - 1 ? -1 : 0);
                 adj[1] = y == 0 ? B.sx : (y == B.n[1] - 1 ? -B.sx : 0);
+                if (ZALL) adj[2] = z == 0 ? B.sz : (z == B.n[2] - 1 ? -B.sz : 0);
             }
 #pragma unroll
             for (int q = 0; q < Q; q++) {
@@ -890,6 +902,8 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
                     if (cvel(D, q, 0) < 0 && adj[0] < 0) nd -= 1;
                     if (cvel(D, q, 1) > 0 && adj[1] > 0) nd += (uint32_t)B.sx;
                     if (cvel(D, q, 1) < 0 && adj[1] < 0) nd -= (uint32_t)B.sx;
+                    if (ZALL && cvel(D, q, 2) > 0 && adj[2] > 0) nd += (uint32_t)B.sz;
+                    if (ZALL && cvel(D, q, 2) < 0 && adj[2] < 0) nd -= (uint32_t)B.sz;
                 }
                 E v;
                 if constexpr (kBufPull) {
@@ -953,6 +967,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
 template <int D>
 __global__ void k_lift(KParams kp, int64_t nodes) {
     constexpr int Q = D == 2 ? 9 : 19;
+    const double* const* m0 = kp.macro0[0] ? kp.macro0 : kp.macro;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nodes;
          i += (int64_t)gridDim.x * blockDim.x) {
         int lo = 0, hi = kp.nblocks - 1;
@@ -977,7 +992,7 @@ __global__ void k_lift(KParams kp, int64_t nodes) {
                             !(obst_mask(kp, B, i + stride[a]) & kObstSolid);
             const bool hm = c[a] > 0 && !(obst_mask(kp, B, i - stride[a]) & kObstSolid);
             for (int b = 0; b < D; b++) {
-                const double* u = kp.macro[1 + b];
+                const double* u = m0[1 + b];
                 const double u0 = u[i];
                 double d = 0.0;
                 if (hp && hm) d = 0.5 * (u[i + stride[a]] - u[i - stride[a]]);
@@ -986,7 +1001,7 @@ __global__ void k_lift(KParams kp, int64_t nodes) {
                 g[a][b] = d / kCS;
             }
         }
-        const double rho = kp.macro[0][i];
+        const double rho = m0[0][i];
         const double tl = B.tau / B.dt + 0.5;
         const double div = g[0][0] + g[1][1] + g[2][2];
         for (int q = 0; q < Q; q++) {
@@ -994,8 +1009,8 @@ __global__ void k_lift(KParams kp, int64_t nodes) {
             for (int a = 0; a < D; a++)
                 for (int b = 0; b < D; b++) s += cvel(D, q, a) * cvel(D, q, b) * g[a][b];
             const double f1 = -3.0 * tl * wgt(D, q) * rho * (s - div / 3.0);
-            out[q * nn] = feq_ref<D, double>(q, rho, kp.macro[1][i], kp.macro[2][i],
-                                             D == 3 ? kp.macro[3][i] : 0.0) + f1;
+            out[q * nn] = feq_ref<D, double>(q, rho, m0[1][i], m0[2][i],
+                                             D == 3 ? m0[3][i] : 0.0) + f1;
         }
     }
 }
@@ -1224,8 +1239,10 @@ __global__ __launch_bounds__(256) void k_reduce_blocks(const double* __restrict_
     const bool last = b + 1 == nblocks;
     int lo0, hi0, lo1 = 0, hi1 = 0;
     if (split) {
-        lo0 = blocks[b].bulk_tile_start;
-        hi0 = last ? nbulk : blocks[b + 1].bulk_tile_start;
+        // the fold-3 bulk grid (every z plane) has its own tile starts
+        const bool zall = fold == 3;
+        lo0 = zall ? blocks[b].bulk_tile_start3 : blocks[b].bulk_tile_start;
+        hi0 = last ? nbulk : (zall ? blocks[b + 1].bulk_tile_start3 : blocks[b + 1].bulk_tile_start);
         lo1 = nbulk + blocks[b].bnd_wg_start[fold];
         hi1 = nbulk + (last ? nbnd : blocks[b + 1].bnd_wg_start[fold]);
     } else {
@@ -1295,12 +1312,18 @@ static void launch_step(const KParams& kp, int part, hipStream_t s) {
     if (part != 2 && kp.nbulk > 0) {
         const dim3 g(kp.nbulk);
         if (kp.obst) {
-            if (kp.fold == 2) hipLaunchKernelGGL((k_bulk<D, ST, MODE, true, 2>), g, blk, 0, s, kp);
+            if (kp.fold == 3 && D == 3)
+                hipLaunchKernelGGL((k_bulk<D, ST, MODE, true, 3>), g, blk, 0, s, kp);
+            else if (kp.fold >= 2)
+                hipLaunchKernelGGL((k_bulk<D, ST, MODE, true, 2>), g, blk, 0, s, kp);
             else if (kp.fold == 1)
                 hipLaunchKernelGGL((k_bulk<D, ST, MODE, true, 1>), g, blk, 0, s, kp);
             else hipLaunchKernelGGL((k_bulk<D, ST, MODE, true, 0>), g, blk, 0, s, kp);
         } else {
-            if (kp.fold == 2) hipLaunchKernelGGL((k_bulk<D, ST, MODE, false, 2>), g, blk, 0, s, kp);
+            if (kp.fold == 3 && D == 3)
+                hipLaunchKernelGGL((k_bulk<D, ST, MODE, false, 3>), g, blk, 0, s, kp);
+            else if (kp.fold >= 2)
+                hipLaunchKernelGGL((k_bulk<D, ST, MODE, false, 2>), g, blk, 0, s, kp);
             else if (kp.fold == 1)
                 hipLaunchKernelGGL((k_bulk<D, ST, MODE, false, 1>), g, blk, 0, s, kp);
             else hipLaunchKernelGGL((k_bulk<D, ST, MODE, false, 0>), g, blk, 0, s, kp);

@@ -36,13 +36,18 @@ struct BlockParams {
     // n-2 in 3D) in tiles_x * tiles_y tiles per plane; the boundary kernel
     // walks the block's boundary segments, 256 nodes per workgroup
     int32_t bulk_tile_start;  // first bulk workgroup of this block
+    // fold level 3: the bulk grid spans every z plane (3D), so the z-face
+    // planes' simple regions run in it too; its first workgroup
+    int32_t bulk_tile_start3;
     // boundary workgroups and segments per fold level: [0] every boundary
     // region, [1] without the simple x-face interiors, [2] without every
-    // simple region of the bulk grid (those run inside the bulk kernel)
-    int32_t bnd_wg_start[3];
-    int32_t seg_start[3], seg_count[3];
+    // simple region of the planes z = 1 .. n-2, [3] without every simple
+    // region (those run inside the bulk kernel)
+    int32_t bnd_wg_start[4];
+    int32_t seg_start[4], seg_count[4];
     int32_t xgroup;           // tiles_x when this block's bulk tiles can be
                               // dealt a row per XCD, else 0
+    int32_t xgroup3;          // the same for the fold-3 bulk grid
     int32_t tiles_x2;         // FP16 two-node bulk kernel: 128-node tiles per row
     int32_t h2_tile_start;    // its first workgroup of this block
     int32_t rowwg;            // 1: tiles_x is a multiple of kTY and n[1] of kTY,
@@ -118,6 +123,9 @@ struct KParams {
     int32_t* cur_out;       // per-block index after this step (step modes)
     double* macro[4];       // MacroVars (current values, host layout)
     double* macro_copy[4];  // MacroVarsCopy (residual reference)
+    // the macros the initial condition was formed from (SetInitialMacrosVars
+    // as uploaded when PreDefinedInitialCondition was called); null: macro
+    const double* macro0[4];
     double* fhost;      // materialize output / init input, host layout
     double* partials;   // per logical workgroup: kNAcc doubles
     float scale[kMaxQ];     // FP16 storage: g = h * scale[q]
@@ -129,7 +137,9 @@ struct KParams {
     const uint32_t* obst;   // obstacle link masks per interior node, or null
     const BndSeg* segs;     // boundary segments, blocks in order
     int32_t fold;           // 0 none, 1 simple x faces, 2 every simple region of
-                            // the bulk grid run inside the bulk kernel
+                            // the planes z = 1 .. n-2, 3 every simple region
+                            // (the bulk grid then spans every z plane) run
+                            // inside the bulk kernel
     int32_t xcd_remap;      // bulk tiles dealt to XCDs in contiguous runs
     int32_t xgroup;         // bulk workgroup shape: 0 kTX x kTY tiles in order,
                             // 1 the same with each block's rows of tiles dealt

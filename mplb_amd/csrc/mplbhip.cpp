@@ -72,8 +72,9 @@ struct mplbhip_ctx {
     int64_t total_elems = 0;
     int64_t interior_nodes = 0;
     int ntiles = 0;            // whole-lattice tiles
-    int nbulk = 0;             // bulk tiles of a step
-    int nbnd_v[3] = {0, 0, 0}; // boundary workgroups of a step, [fold]
+    int nbulk = 0;             // bulk tiles of a step (z planes 1 .. n-2)
+    int nbulk3 = 0;            // bulk tiles of a fold-3 step (every z plane)
+    int nbnd_v[4] = {0, 0, 0, 0};  // boundary workgroups of a step, [fold]
     bool tuned = false;        // launch configuration chosen (autotune)
     bool fused = false;        // small case: one whole-lattice launch per step
     bool h2 = false;           // FP16: two-node-per-lane bulk kernel (fold level 0)
@@ -86,16 +87,20 @@ struct mplbhip_ctx {
     int64_t grid_nodes = 0;    // nodes of the blocks the step launches cover
     bool batch_compact = true;  // iterate_batch shrinks the launches to live blocks
     int fold = 1;              // simple regions the bulk kernel runs inline:
-                               // 0 none, 1 x faces, 2 every one in its grid
+                               // 0 none, 1 x faces, 2 every one of the planes
+                               // z = 1 .. n-2, 3 every one (3D: the bulk grid
+                               // then spans every z plane)
     bool xcd_remap = false;    // bulk tiles dealt to XCDs in contiguous runs
     int xgroup = 0;            // 1: rows of bulk tiles dealt to one XCD each
     int xgroup_ok = 0;         // 1 when some block allows that grouping
     bool rowwg_ok = false;     // some block allows row workgroups (xgroup 2)
     // split steps: the boundary kernel on a side stream beside the bulk
     // kernel.  On by default for cases with halo links (three-block
-    // L-channel: 39.3k -> 40.1k MLUPS); a single 256^3 block gains nothing
-    // (FP32 +0.2%, FP16 -0.7%).  MPLBHIP_OVERLAP=0/1 overrides.
+    // L-channel: 39.3k -> 40.1k MLUPS) and wherever the boundary launch is
+    // small (the eight cavity corners left at fold level 3: its launch then
+    // hides behind the bulk kernel).  MPLBHIP_OVERLAP=0/1 overrides.
     bool overlap = false;
+    bool overlap_env = false;
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool fold_env = false, xgroup_env = false;  // fixed by the environment
@@ -108,6 +113,10 @@ struct mplbhip_ctx {
     hipStream_t stream = nullptr;
     DevBuf fs[2], per[2], macro[4], macro_copy[4], progs, blocks, partials,
         red, fhost, parity[2], dsegs, dobst;
+    // the macros at PreDefinedInitialCondition: f^0 = feq of these, formed
+    // at the first step (the collision needs the time step, which the Apps
+    // set afterwards); a macro refresh at t = 0 must not change it
+    DevBuf macro0[4];
     std::vector<HaloLink> halos;
     std::vector<double> host_init_f;  // f given by mplbhip_set_f, per block
     std::vector<float> scale, inv_scale;
@@ -206,15 +215,22 @@ static void drop_graphs(mplbhip_ctx* c) {
     c->graph_steps = 0;
 }
 
+// fold level of the step in force: the two-node FP16 kernel covers
+// interior nodes only (level 0)
+static int step_fold(const mplbhip_ctx* c) { return c->h2 ? 0 : c->fold; }
+// bulk workgroups of the step in force
+static int step_nbulk(const mplbhip_ctx* c) {
+    return step_fold(c) == 3 ? c->nbulk3 : c->nbulk;
+}
+
 static KParams base_params(mplbhip_ctx* c) {
     KParams kp;
     std::memset(&kp, 0, sizeof kp);
     kp.blocks = static_cast<const BlockParams*>(c->blocks.p);
     kp.nblocks = (int)c->bp.size();
     kp.ntiles = c->ntiles;
-    kp.nbulk = c->nbulk;
-    // the two-node FP16 kernel covers interior nodes only: fold level 0
-    const int fold = c->h2 ? 0 : c->fold;
+    kp.nbulk = step_nbulk(c);
+    const int fold = step_fold(c);
     kp.nbnd = c->nbnd_v[fold];
     kp.fold = fold;
     kp.h2 = c->h2 ? 1 : 0;
@@ -228,6 +244,7 @@ static KParams base_params(mplbhip_ctx* c) {
     for (int v = 0; v < 4; v++) {
         kp.macro[v] = static_cast<double*>(c->macro[v].p);
         kp.macro_copy[v] = static_cast<double*>(c->macro_copy[v].p);
+        kp.macro0[v] = static_cast<const double*>(c->macro0[v].p);
     }
     kp.partials = static_cast<double*>(c->partials.p);
     for (int q = 0; q < kMaxQ; q++) {
@@ -258,26 +275,54 @@ static int exchange(mplbhip_ctx* c, int into) {
     return 0;
 }
 
-// one step: fused kernel from buffer cur into 1-cur, then halos
-// one step from parity c->cur; leaves c->cur flipped
+// the boundary kernel of this step runs on the side stream
+static bool use_side(const mplbhip_ctx* c, const KParams& kp) {
+    if (kp.fused || kp.nbulk <= 0 || kp.nbnd <= 0) return false;
+    if (c->overlap_env) return c->overlap;
+    return !c->halos.empty() || kp.nbnd <= 256;
+}
+
+// The step kernels of one step (no halo exchange).  The bulk and boundary
+// kernels touch disjoint nodes and read only the previous fStage: with the
+// side stream the boundary kernel runs beside the bulk kernel (a fork and
+// join, also in a captured graph).  With ev (four events: bulk start/end on
+// the context stream, boundary start/end on the stream it runs on) each
+// kernel is bracketed for timing.
+static int launch_step_kernels(mplbhip_ctx* c, const KParams& kp, int mode,
+                               hipEvent_t* ev = nullptr) {
+    if (!use_side(c, kp)) {
+        if (!ev) {
+            HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->stream));
+            return 0;
+        }
+        HIPCHK(c, hipEventRecord(ev[0], c->stream));
+        HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->stream, 1));
+        HIPCHK(c, hipEventRecord(ev[1], c->stream));
+        HIPCHK(c, hipEventRecord(ev[2], c->stream));
+        HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->stream, 2));
+        HIPCHK(c, hipEventRecord(ev[3], c->stream));
+        return 0;
+    }
+    HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    if (ev) HIPCHK(c, hipEventRecord(ev[2], c->side));
+    HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->side, 2));
+    if (ev) HIPCHK(c, hipEventRecord(ev[3], c->side));
+    if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->stream, 1));
+    if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    HIPCHK(c, hipEventRecord(c->ev_join, c->side));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    return 0;
+}
+
+// one step from parity c->cur (step kernels, then the halo copies, which
+// wait for both kernels); leaves c->cur flipped
 static int enqueue_step(mplbhip_ctx* c, bool check, bool residual) {
     KParams kp = base_params(c);
     kp.want_residual = residual ? 1 : 0;
     const int mode = check ? MODE_STEP_CHECK : MODE_STEP;
-    if (c->overlap && !kp.fused && kp.nbulk > 0 && kp.nbnd > 0) {
-        // the bulk and boundary kernels of a step touch disjoint nodes and
-        // read only the previous fStage: the boundary kernel runs on a side
-        // stream beside the bulk kernel (a fork and join, also in a
-        // captured graph), and the halo copies wait for both
-        HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
-        HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
-        HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->side, 2));
-        HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->stream, 1));
-        HIPCHK(c, hipEventRecord(c->ev_join, c->side));
-        HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
-    } else {
-        HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->stream));
-    }
+    if (int rc = launch_step_kernels(c, kp, mode)) return rc;
     if (int rc = exchange(c, 1 - c->cur)) return rc;
     c->cur ^= 1;
     return 0;
@@ -428,6 +473,7 @@ void mplbhip_destroy(mplbhip_ctx* c) {
         for (int v = 0; v < 4; v++) {
             dfree(c->macro[v]);
             dfree(c->macro_copy[v]);
+            dfree(c->macro0[v]);
         }
         dfree(c->progs);
         dfree(c->blocks);
@@ -810,21 +856,24 @@ static int build_obstacles(mplbhip_ctx* c) {
 // so the launches' block search (the last block whose start <= the
 // workgroup) never lands on them.  Returns the number of row-grouped blocks.
 static int assign_tiles(mplbhip_ctx* c, bool only_active) {
-    int tiles = 0, bulk = 0, bulk2 = 0, bnd_v[3] = {0, 0, 0};
+    int tiles = 0, bulk = 0, bulk3 = 0, bulk2 = 0, bnd_v[4] = {0, 0, 0, 0};
     int64_t nodes = 0;
     for (BlockParams& B : c->bp) {
         const bool on = !only_active || B.active;
         const int zb = c->dim == 3 ? B.n[2] - 2 : 1;
+        const int zb3 = c->dim == 3 ? B.n[2] : 1;
         B.tile_start = tiles;
         B.bulk_tile_start = bulk;
+        B.bulk_tile_start3 = bulk3;
         B.h2_tile_start = bulk2;
         if (on) {
             tiles += B.tiles_x * B.tiles_y * B.n[2];
             bulk += B.tiles_x * B.tiles_y * zb;
+            bulk3 += B.tiles_x * B.tiles_y * zb3;
             bulk2 += B.tiles_x2 * B.tiles_y * zb;
             nodes += (int64_t)B.n[0] * B.n[1] * B.n[2];
         }
-        for (int v = 0; v < 3; v++) {
+        for (int v = 0; v < 4; v++) {
             B.bnd_wg_start[v] = bnd_v[v];
             if (!on) continue;
             int count = 0;
@@ -835,8 +884,9 @@ static int assign_tiles(mplbhip_ctx* c, bool only_active) {
     }
     c->ntiles = tiles;
     c->nbulk = bulk;
+    c->nbulk3 = bulk3;
     c->nbulk2 = bulk2;
-    for (int v = 0; v < 3; v++) c->nbnd_v[v] = bnd_v[v];
+    for (int v = 0; v < 4; v++) c->nbnd_v[v] = bnd_v[v];
     c->grid_nodes = nodes;
     // the x tiles of a row share their edge cache lines: deal each row's
     // tiles to one XCD (hardware hands workgroups to the 8 XCDs round
@@ -852,6 +902,12 @@ static int assign_tiles(mplbhip_ctx* c, bool only_active) {
                         !c->xcd_remap;
         B.xgroup = ok ? B.tiles_x : 0;
         grouped += ok ? 1 : 0;
+        const int next3 = b + 1 < nb ? c->bp[b + 1].bulk_tile_start3 : bulk3;
+        const int cnt3 = next3 - B.bulk_tile_start3;
+        B.xgroup3 = cnt3 > 0 && B.bulk_tile_start3 % 8 == 0 && cnt3 % (8 * B.tiles_x) == 0 &&
+                            !c->xcd_remap
+                        ? B.tiles_x
+                        : 0;
         B.rowwg = B.tiles_x % kTY == 0 && B.n[1] % kTY == 0 ? 1 : 0;
     }
     return grouped;
@@ -876,7 +932,7 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     // large cases.
     c->fold = c->storage != MPLBHIP_STORE_F16 ? 1 : 0;
     if (const char* e = std::getenv("MPLBHIP_XFACE_FOLD")) {
-        c->fold = std::min(2, std::max(0, std::atoi(e)));
+        c->fold = std::min(c->dim == 3 ? 3 : 2, std::max(0, std::atoi(e)));
         c->fold_env = true;
     }
     if (const char* e = std::getenv("MPLBHIP_XCD_REMAP")) c->xcd_remap = std::atoi(e) != 0;
@@ -931,12 +987,13 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         B.tiles_x2 = (n[0] + 2 * kTX - 1) / (2 * kTX);
         const int32_t divisors[4] = {B.tiles_x, B.tiles_y, (int32_t)n[1], B.tiles_x / kTY};
         for (int k = 0; k < 4; k++) fast_div_params(divisors[k], &B.div_m[k], &B.div_sh[k]);
-        // the step's split: bulk tiles over z = 1 .. n-2 (3D), and the
-        // boundary regions as segments, 256 nodes per workgroup, in three
-        // lists: [0] every boundary region, [1] without the simple x-face
-        // interiors, [2] without every simple region of the bulk grid (the
-        // bulk kernel runs those inline at that fold level)
-        for (int v = 0; v < 3; v++) {
+        // the step's split: bulk tiles over z = 1 .. n-2 (3D; every z plane
+        // at fold level 3), and the boundary regions as segments, 256 nodes
+        // per workgroup, in four lists: [0] every boundary region, [1]
+        // without the simple x-face interiors, [2] without every simple
+        // region of the planes z = 1 .. n-2, [3] without every simple region
+        // (the bulk kernel runs those inline at that fold level)
+        for (int v = 0; v < 4; v++) {
             B.seg_start[v] = (int)c->segs.size();
             int count = 0;
             for (int r = 1; r < c->plan.nregions; r++) {
@@ -945,6 +1002,7 @@ int mplbhip_prepare(mplbhip_ctx* c) {
                 if (v == 1 && r < 3 && simple) continue;
                 // every region in 2D; rz = 0, i.e. r < 9, in 3D
                 if (v == 2 && (c->dim == 2 || r < 9) && simple) continue;
+                if (v == 3 && simple) continue;
                 BndSeg S;
                 for (int a = 0; a < 3; a++) {
                     if (a == 2 && c->dim == 2) {
@@ -994,8 +1052,10 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         c->xgroup_env = true;
     }
     if (int rc = setup_halos(c)) return rc;
-    c->overlap = !c->halos.empty();
-    if (const char* e = std::getenv("MPLBHIP_OVERLAP")) c->overlap = std::atoi(e) != 0;
+    if (const char* e = std::getenv("MPLBHIP_OVERLAP")) {
+        c->overlap = std::atoi(e) != 0;
+        c->overlap_env = true;
+    }
     if (int rc = build_obstacles(c)) return rc;
     // the two-node FP16 kernel has no obstacle path
     c->h2_ok = c->storage == MPLBHIP_STORE_F16 && c->obst.empty();
@@ -1035,7 +1095,8 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     }
     // partial slots of the launches over every block (compaction only shrinks them)
     const size_t nslots = (size_t)std::max(
-        c->ntiles, c->nbulk + std::max(c->nbnd_v[0], std::max(c->nbnd_v[1], c->nbnd_v[2])));
+        c->ntiles, std::max(c->nbulk + std::max(c->nbnd_v[0], std::max(c->nbnd_v[1], c->nbnd_v[2])),
+                            c->nbulk3 + c->nbnd_v[3]));
     if (int rc = dalloc(c, c->partials, nslots * kNAcc * sizeof(double))) return rc;
     if (int rc = dalloc(c, c->dsegs, std::max<size_t>(1, c->segs.size()) * sizeof(BndSeg)))
         return rc;
@@ -1108,7 +1169,17 @@ int mplbhip_init_feq(mplbhip_ctx* c) {
     c->init_lift = false;
     c->initialized = true;
     // fStage^0 = collide(feq) is formed lazily at the first step, once the
-    // time step is known (the Apps call SetTimeStep after this)
+    // time step is known (the Apps call SetTimeStep after this), from the
+    // macros as they are now (PreDefinedInitialCondition reads them here,
+    // model_wrapper.cpp:347-391)
+    const size_t mb = (size_t)c->interior_nodes * sizeof(double);
+    for (int v = 0; v < 4; v++) {
+        if (c->macro0[v].bytes != mb)
+            if (int rc = dalloc(c, c->macro0[v], mb)) return rc;
+        HIPCHK(c, hipMemcpyAsync(c->macro0[v].p, c->macro[v].p, mb, hipMemcpyDeviceToDevice,
+                                 c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -1178,6 +1249,7 @@ static int autotune(mplbhip_ctx* c) {
         folds = {0};
         if (c->nbnd_v[1] != c->nbnd_v[0]) folds.push_back(1);
         if (c->nbnd_v[2] != c->nbnd_v[1]) folds.push_back(2);
+        if (c->dim == 3 && c->nbnd_v[3] != c->nbnd_v[2]) folds.push_back(3);
     }
     std::vector<int> groups;
     if (c->xgroup_env) {
@@ -1202,10 +1274,10 @@ static int autotune(mplbhip_ctx* c) {
             c->xgroup = cand[i][1];
             c->h2 = cand[i][2] != 0;
             KParams kp = base_params(c);
-            HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP, kp, c->stream));
+            if (int rc = launch_step_kernels(c, kp, MODE_STEP)) return rc;
             HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
             for (int k = 0; k < kSteps; k++)
-                HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP, kp, c->stream));
+                if (int rc = launch_step_kernels(c, kp, MODE_STEP)) return rc;
             HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
             HIPCHK(c, hipEventSynchronize(c->ev[1]));
             float ms = 0.f;
@@ -1272,17 +1344,23 @@ static int run_steps(mplbhip_ctx* c, int nsteps, bool timed, double* ms_total,
         if (int rc = first_collision(c)) return rc;
     double ktot = 0.0;
     if (timed) {
-        // per-step events bracket the bulk kernel and the boundary kernel
-        std::vector<hipEvent_t> evs(3 * (size_t)nsteps);
+        // per-step events bracket the bulk kernel and the boundary kernel,
+        // each on the stream it runs on (the step's own launch structure)
+        std::vector<hipEvent_t> evs(4 * (size_t)nsteps);
         for (auto& e : evs) HIPCHK(c, hipEventCreate(&e));
         HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
         for (int s = 0; s < nsteps; s++) {
             KParams kp = base_params(c);
-            HIPCHK(c, hipEventRecord(evs[3 * s], c->stream));
-            HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP, kp, c->stream, 1));
-            HIPCHK(c, hipEventRecord(evs[3 * s + 1], c->stream));
-            HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP, kp, c->stream, 2));
-            HIPCHK(c, hipEventRecord(evs[3 * s + 2], c->stream));
+            if (kp.fused) {
+                // one whole-lattice launch: it is the bulk share
+                HIPCHK(c, hipEventRecord(evs[4 * s], c->stream));
+                HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP, kp, c->stream));
+                HIPCHK(c, hipEventRecord(evs[4 * s + 1], c->stream));
+                HIPCHK(c, hipEventRecord(evs[4 * s + 2], c->stream));
+                HIPCHK(c, hipEventRecord(evs[4 * s + 3], c->stream));
+            } else if (int rc = launch_step_kernels(c, kp, MODE_STEP, &evs[4 * s])) {
+                return rc;
+            }
             if (int rc = exchange(c, 1 - c->cur)) return rc;
             c->cur ^= 1;
             c->tstep++;
@@ -1295,8 +1373,8 @@ static int run_steps(mplbhip_ctx* c, int nsteps, bool timed, double* ms_total,
         double btot = 0.0;
         for (int s = 0; s < nsteps; s++) {
             float k = 0.f, kb = 0.f;
-            HIPCHK(c, hipEventElapsedTime(&k, evs[3 * s], evs[3 * s + 1]));
-            HIPCHK(c, hipEventElapsedTime(&kb, evs[3 * s + 1], evs[3 * s + 2]));
+            HIPCHK(c, hipEventElapsedTime(&k, evs[4 * s], evs[4 * s + 1]));
+            HIPCHK(c, hipEventElapsedTime(&kb, evs[4 * s + 2], evs[4 * s + 3]));
             ktot += k;
             btot += kb;
         }
@@ -1365,10 +1443,9 @@ int mplbhip_iterate(mplbhip_ctx* c, double criteria, int check_period,
             double acc[kNAcc];
             HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
                                     static_cast<const BlockParams*>(c->blocks.p),
-                                    (int)c->bp.size(), c->ntiles, c->nbulk,
-                                    c->nbnd_v[c->h2 ? 0 : c->fold], c->fused ? 0 : 1,
-                                    c->h2 ? 0 : c->fold,
-                                    static_cast<double*>(c->red.p), c->stream));
+                                    (int)c->bp.size(), c->ntiles, step_nbulk(c),
+                                    c->nbnd_v[step_fold(c)], c->fused ? 0 : 1,
+                                    step_fold(c), static_cast<double*>(c->red.p), c->stream));
             HIPCHK(c, hipMemcpyAsync(acc, static_cast<double*>(c->red.p) +
                                               c->bp.size() * kNAcc,
                                      sizeof acc, hipMemcpyDeviceToHost, c->stream));
@@ -1430,6 +1507,8 @@ static int compact_grid(mplbhip_ctx* c, bool on) {
 // The caller re-uploads the block parameters afterwards.
 static int restart_at_rest(mplbhip_ctx* c, const std::vector<int>& blocks) {
     KParams kp = base_params(c);
+    // the restarted blocks start from the rest macros just written
+    for (int v = 0; v < 4; v++) kp.macro0[v] = nullptr;
     for (int b : blocks) {
         const BlockParams& B = c->bp[b];
         HIPCHK(c, launch_rest_macros(kp, b, (int64_t)B.n[0] * B.n[1] * B.n[2], c->stream));
@@ -1492,8 +1571,8 @@ int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
         done++;
         HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
                                 static_cast<const BlockParams*>(c->blocks.p), nb,
-                                c->ntiles, c->nbulk, c->nbnd_v[c->h2 ? 0 : c->fold],
-                                c->fused ? 0 : 1, c->h2 ? 0 : c->fold,
+                                c->ntiles, step_nbulk(c), c->nbnd_v[step_fold(c)],
+                                c->fused ? 0 : 1, step_fold(c),
                                 static_cast<double*>(c->red.p), c->stream));
         HIPCHK(c, hipMemcpyAsync(acc.data(), c->red.p, acc.size() * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));
@@ -1607,30 +1686,31 @@ int mplbhip_launch_shape(mplbhip_ctx* c, int* fold, int* xgroup, int* h2,
                          long long* bulk_nodes) {
     if (!c) return MPLBHIP_ERR_ARG;
     if (!c->prepared) return fail(c, MPLBHIP_ERR_ARG, "call mplbhip_prepare first");
-    const int fl = c->h2 ? 0 : c->fold;
+    const int fl = step_fold(c);
     if (fold) *fold = fl;
     if (xgroup) *xgroup = c->xgroup;
     if (h2) *h2 = c->h2 ? 1 : 0;
     if (bulk_nodes) {
         // the bulk kernel's lanes: interior nodes of its planes, plus (fold
         // 1) the simple x-face regions or (fold 2) every simple region of
-        // the plane, as in its lane selection
+        // the planes z = 1 .. n-2 or (fold 3, 3D) every simple region, as in
+        // its lane selection
         long long total = 0;
         for (size_t b = 0; b < c->bp.size(); b++) {
             const int* n = c->plan.blocks[b].n;
-            const long long planes = c->dim == 3 ? n[2] - 2 : 1;
-            long long per_plane = 0;
-            for (int ry = 0; ry < 3; ry++)
-                for (int rx = 0; rx < 3; rx++) {
-                    const int r = rx + 3 * ry;
-                    const long long cnt =
-                        (long long)(rx == 0 ? n[0] - 2 : 1) * (ry == 0 ? n[1] - 2 : 1);
-                    bool lane = r == 0;
-                    if (!lane && fl > 0 && (fl == 2 || ry == 0))
-                        lane = c->plan.progs[b * c->plan.nregions + r].simple != 0;
-                    if (lane) per_plane += cnt;
-                }
-            total += per_plane * planes;
+            const int rzmax = (c->dim == 3 && fl == 3) ? 3 : 1;
+            for (int rz = 0; rz < rzmax; rz++)
+                for (int ry = 0; ry < 3; ry++)
+                    for (int rx = 0; rx < 3; rx++) {
+                        const int r = rx + 3 * ry + 9 * rz;
+                        const long long planes = c->dim == 3 ? (rz == 0 ? n[2] - 2 : 1) : 1;
+                        const long long cnt = (long long)(rx == 0 ? n[0] - 2 : 1) *
+                                              (ry == 0 ? n[1] - 2 : 1) * planes;
+                        bool lane = r == 0;
+                        if (!lane && fl > 0 && (fl >= 2 || (ry == 0 && rz == 0)))
+                            lane = c->plan.progs[b * c->plan.nregions + r].simple != 0;
+                        if (lane) total += cnt;
+                    }
         }
         *bulk_nodes = total;
     }

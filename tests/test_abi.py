@@ -122,11 +122,12 @@ def test_define_order_errors():
         p.add_connection(0, BS.Left, 0, BS.Right, VT.MDPeriodic)
 
 
-@pytest.mark.parametrize("fold", ["0", "1", "2"])
+@pytest.mark.parametrize("fold", ["0", "1", "2", "3"])
 def test_launch_shape_counts_bulk_nodes(monkeypatch, fold):
     """The nodes one bulk launch updates: the interior of planes 1 .. n-2 at
     fold level 0; plus the x-face walls (simple EQM regions) at level 1;
-    at level 2 at most whole planes."""
+    at level 2 at most whole planes; at level 3 every node but the eight
+    corners (whose wall scheme keeps state between steps)."""
     monkeypatch.setenv("MPLBHIP_XFACE_FOLD", fold)
     g = hip.LatticeCase(3, storage=hip.STORE_F32, device=-1)
     nx, ny, nz = 20, 16, 12
@@ -142,5 +143,7 @@ def test_launch_shape_counts_bulk_nodes(monkeypatch, fold):
         assert nodes == interior
     elif fold == "1":
         assert nodes == nx * (ny - 2) * (nz - 2)
-    else:
+    elif fold == "2":
         assert nx * (ny - 2) * (nz - 2) <= nodes <= nx * ny * (nz - 2)
+    else:
+        assert nodes == nx * ny * nz - 8

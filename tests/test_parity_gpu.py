@@ -2,8 +2,10 @@
 
 FP64 storage reproduces the reference arithmetic operation for operation,
 so populations and macros must agree bit for bit (the acceptance bar is
-max-abs 1e-12 on f and macros, 1e-13 relative on the total mass).  FP32 and
-FP16 storage are checked against the FP64 oracle with stated tolerances.
+max-abs 1e-12 on f and macros, 1e-13 relative on the total mass).  Every
+test here runs FP64 storage; the FP32 and FP16 forms are checked against the
+FP64 oracle with stated tolerances in test_precision_gpu.py (whole-lattice
+launch) and test_reduced_split_gpu.py (every split-step form, linked blocks).
 """
 import math
 import os
@@ -234,3 +236,53 @@ def test_f64_iterate_transient_refreshes_macros():
         assert np.array_equal(g.get_macro(0, v), o.get_macro(0, v))
     o.step(5)
     assert np.array_equal(g.get_f(0), o.get_f(0))
+
+
+@pytest.mark.parametrize("dim,axis,sizes", [
+    (2, 0, [(4, 5), (3, 5)]),
+    (3, 0, [(3, 4, 5), (4, 4, 5)]),
+    (3, 1, [(5, 3, 4), (5, 4, 4)]),
+    (3, 2, [(4, 5, 3), (4, 5, 3)]),
+])
+def test_f64_split_box_junction_halos_bitexact(dim, axis, sizes):
+    """A periodic box cut into two blocks joined by VirtualBoundary
+    connections (the junction rims whose halo cells OPS's group order
+    leaves one exchange old, pinned in test_oracle_pins.py), 30 steps."""
+    from tests.test_oracle_pins import _build_split_box
+    from mplb_amd import lattice
+    log("split box dim=%d axis=%d %s" % (dim, axis, sizes))
+    rng = np.random.default_rng(5 + axis)
+    glob = list(sizes[0])
+    glob[axis] += sizes[1][axis]
+    Q = 9 if dim == 2 else 19
+    _, w = lattice.lattice(dim)
+    f_glob = w.reshape((Q,) + (1,) * dim) * (1 + 0.2 * rng.random((Q,) + tuple(reversed(glob))))
+    g = hip.LatticeCase(dim, storage=hip.STORE_F64, device=0)
+    _build_split_box(g, dim, sizes, axis, 0.8, f_glob)
+    o = OracleCase(dim)
+    _build_split_box(o, dim, sizes, axis, 0.8, f_glob)
+    g.step(30)
+    o.step(30)
+    assert compare(g, o, "split box") == 0.0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_f64_residual_closed_form(dim):
+    """The uniform forced periodic flow whose residual ratios have a closed
+    form (test_oracle_pins.test_residual_closed_form), on the device."""
+    from tests.test_oracle_pins import _uniform_forced_box
+    log("residual closed form dim=%d" % dim)
+    size = (6, 5) if dim == 2 else (6, 5, 4)
+    u0 = (0.02, -0.013, 0.011)[:dim]
+    gv = (1e-4, 2e-4, -3e-4)[:dim]
+    dt, tau_l, P = 0.37, 0.8, 7
+    g = hip.LatticeCase(dim, storage=hip.STORE_F64, device=0)
+    _uniform_forced_box(g, dim, size, tau_l, dt, u0, gv)
+    g.update_macros()
+    g.residual(P)
+    g.step(P)
+    res, per = g.residual(P)
+    exp = [(P * dt * gv[d]) ** 2 / (u0[d] + (P + 0.5) * dt * gv[d]) ** 2 for d in range(dim)]
+    assert per[0] < 1e-28
+    assert np.allclose(per[1:], exp, rtol=1e-9, atol=0), (per, exp)
+    assert math.isclose(res, max(exp) / (P * dt), rel_tol=1e-9)
