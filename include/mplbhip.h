@@ -200,6 +200,16 @@ int mplbhip_residual(mplbhip_ctx* ctx, int check_period, double* per_var,
 int mplbhip_last_residual(mplbhip_ctx* ctx, double* per_var, double* max);
 /* FP64 total mass of the current f over every interior node */
 int mplbhip_mass(mplbhip_ctx* ctx, double* mass);
+/* No counterpart in the reference: mplbhip_mass for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops): it calls no hipMalloc or hipFree.  Its partial sums live in a
+ * file-scope device array of 1 Mi doubles (8 MiB) per GPU, so a case may
+ * have at most about 116,000 workgroups of 256 nodes (every block's
+ * nodes, 64 x 4 per z plane); larger cases are refused with
+ * MPLBHIP_ERR_UNSUPPORTED.  Needs at least one step taken (f^0 is formed on
+ * the device by the first step).  Calls on one GPU share the array: calls
+ * that overlap there race.  Same value as mplbhip_mass, bit for bit. */
+int mplbhip_mass_noalloc(mplbhip_ctx* ctx, double* mass);
 /* steps taken since the initial condition */
 int mplbhip_time_step(mplbhip_ctx* ctx, long long* step);
 int mplbhip_sync(mplbhip_ctx* ctx);

@@ -1409,6 +1409,23 @@ hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s)
     return hipGetLastError();
 }
 
+// ---- scratch of the allocation-free entry point --------------------------------
+// Per-workgroup partial sums and per-block totals of the FP64 reductions of
+// mplbhip_mass_noalloc live in this file-scope array instead of memory the
+// library allocates.  Each GPU the library runs on holds its own copy until
+// the process ends; every call on one GPU shares it (calls that overlap on
+// one GPU race).  Host code never passes the array's name to a kernel (on
+// the host it is only a stand-in): it takes the device address once per
+// device from hipGetSymbolAddress, with that device current.
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+hipError_t scratch_address(double** ptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    *ptr = e == hipSuccess ? static_cast<double*>(p) : nullptr;
+    return e;
+}
+
 hipError_t launch_fill(void* p, int64_t bytes, hipStream_t s) {
     const int64_t n = bytes / 4;
     int grid = (int)((n + 255) / 256);

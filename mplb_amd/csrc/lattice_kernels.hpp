@@ -10,6 +10,10 @@ namespace mplbhip {
 constexpr int kTX = 64;   // one wavefront along x
 constexpr int kTY = 4;    // four wavefronts per workgroup along y
 constexpr int kNAcc = 9;  // residual: 4 squared differences, 4 squares, mass
+// doubles in the file-scope scratch array of the allocation-free entry
+// point (8 MiB per GPU): kNAcc partials per workgroup of a whole-lattice
+// launch plus kNAcc totals per block and for the case
+constexpr long long kScratchDoubles = 1LL << 20;
 
 // One block as laid out in HBM: a one node halo on each axis (no z halo in
 // 2D), rows padded so that x = 0 is 64-byte aligned; the Q populations of a

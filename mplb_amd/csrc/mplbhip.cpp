@@ -29,6 +29,7 @@ hipError_t launch_fill(void* p, int64_t bytes, hipStream_t s);
 hipError_t launch_lift(int dim, const KParams& kp, int64_t nodes, hipStream_t s);
 hipError_t launch_rest_macros(const KParams& kp, int b, int64_t nodes, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
+hipError_t scratch_address(double** ptr);
 }  // namespace mplbhip
 
 using namespace mplbhip;
@@ -121,6 +122,9 @@ struct mplbhip_ctx {
     std::vector<double> host_init_f;  // f given by mplbhip_set_f, per block
     std::vector<float> scale, inv_scale;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    // device address of this GPU's file-scope scratch array (the
+    // allocation-free entry point), taken at create with the device current
+    double* scratch = nullptr;
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int graph_steps = 0;
@@ -451,7 +455,8 @@ int mplbhip_create(mplbhip_ctx** out, int device, int space_dim, int storage) {
             hipEventCreate(&c->ev[1]) != hipSuccess ||
             hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+            hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+            scratch_address(&c->scratch) != hipSuccess) {
             delete c;
             return MPLBHIP_ERR_HIP;
         }
@@ -1782,6 +1787,39 @@ int mplbhip_mass(mplbhip_ctx* c, double* mass) {
     std::vector<double> acc;
     if (int rc = materialize(c, false, false, &acc)) return rc;
     *mass = acc[c->bp.size() * kNAcc + 8];
+    return 0;
+}
+
+int mplbhip_mass_noalloc(mplbhip_ctx* c, double* mass) {
+    if (!c || !mass) return MPLBHIP_ERR_ARG;
+    if (!c->prepared) return fail(c, MPLBHIP_ERR_ARG, "call mplbhip_prepare first");
+    const long long need = ((long long)c->ntiles + (long long)c->bp.size() + 1) * kNAcc;
+    if (need > kScratchDoubles)
+        return fail(c, MPLBHIP_ERR_UNSUPPORTED,
+                    "mplbhip_mass_noalloc: the case needs %lld doubles of scratch, the limit is "
+                    "%lld (%d workgroups of 256 nodes); use mplbhip_mass",
+                    need, (long long)kScratchDoubles, c->ntiles);
+    if (int rc = need_device(c)) return rc;
+    if (int rc = ensure_init(c)) return rc;
+    if (c->tstep == 0)
+        return fail(c, MPLBHIP_ERR_ARG,
+                    "mplbhip_mass_noalloc: no step taken yet (f^0 is formed on the device only "
+                    "by the first step); use mplbhip_mass at t = 0");
+    if (!c->scratch) return fail(c, MPLBHIP_ERR_HIP, "no scratch address for this device");
+    HIPCHK(c, hipSetDevice(c->device));
+    // every workgroup of the whole-lattice launch writes its partial slot
+    // and the block sums overwrite theirs, so nothing needs clearing
+    KParams kp = base_params(c);
+    kp.partials = c->scratch;
+    kp.fhost = nullptr;
+    kp.want_residual = 0;
+    HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_MATERIALIZE, kp, c->stream));
+    double* out = c->scratch + (size_t)c->ntiles * kNAcc;
+    HIPCHK(c, launch_reduce(c->scratch, static_cast<const BlockParams*>(c->blocks.p),
+                            (int)c->bp.size(), c->ntiles, c->nbulk, 0, 0, 0, out, c->stream));
+    HIPCHK(c, hipMemcpyAsync(mass, out + c->bp.size() * kNAcc + 8, sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 

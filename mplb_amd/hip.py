@@ -74,6 +74,7 @@ def load():
         "mplbhip_get_macro": (I, [P, I, I, DP]),
         "mplbhip_residual": (I, [P, I, DP, DP]),
         "mplbhip_mass": (I, [P, DP]),
+        "mplbhip_mass_noalloc": (I, [P, DP]),
         "mplbhip_time_step": (I, [P, LLP]),
         "mplbhip_sync": (I, [P]),
         "mplbhip_copy_probe": (I, [I, ctypes.c_longlong, I, DP]),
@@ -302,6 +303,13 @@ class LatticeCase:
     def mass(self):
         m = ctypes.c_double()
         self._ck(self.lib.mplbhip_mass(self.h, ctypes.byref(m)))
+        return m.value
+
+    def mass_noalloc(self):
+        """The FP64 total mass through the entry point that allocates no
+        device memory (partial sums in the library's static scratch)."""
+        m = ctypes.c_double()
+        self._ck(self.lib.mplbhip_mass_noalloc(self.h, ctypes.byref(m)))
         return m.value
 
     def time_step(self):
