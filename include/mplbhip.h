@@ -73,6 +73,13 @@ int mplbhip_define_connection(mplbhip_ctx* ctx, int from_block,
 int mplbhip_define_rotated_connection(mplbhip_ctx* ctx, int from_block,
                                       int from_surface, int to_block,
                                       int to_surface, int vertex_type);
+/* DefineScheme (scheme.cpp:92-109), before mplbhip_prepare: 10
+ * Scheme_StreamCollision (the default), 11 Scheme_StreamCollision_Swap (the
+ * swap scheme of evolution.cpp:195-252: one lattice f updated in place by
+ * collision, force, halo exchange of f, local swap, swap stream and the
+ * boundaries; D3Q19 and FP64 storage).  FDPeriodic boundaries need the
+ * swap scheme: they copy f's halo, which only it exchanges. */
+int mplbhip_define_scheme(mplbhip_ctx* ctx, int scheme);
 /* No reference counterpart (the reference only skips ImmersedSolid nodes,
  * model_kernel.inc:420, scheme_kernel.inc:1418; its bounce-back is
  * commented out, boundary_kernel.inc:490-532): in-domain obstacles of one
@@ -92,7 +99,9 @@ int mplbhip_define_collision(mplbhip_ctx* ctx, double tau_ref,
 /* DefineBlockBoundary (boundary.cpp:61-130); the call order is the
  * application order (ImplementBoundary boundary.cpp:379-386).  Schemes:
  * EQMDiffuseRefl (24, vars = wall velocity u,v[,w]), ExtrapolPressure1ST
- * (16, vars = rho), None (-1) / MDPeriodic (18) (node typing only). */
+ * (16, vars = rho; stream-collision scheme), FDPeriodic (19, swap scheme:
+ * KerCutCellPeriodic[3D] boundary_kernel.inc:615-719, 1278-1666), None (-1)
+ * / MDPeriodic (18) (node typing only). */
 int mplbhip_define_boundary(mplbhip_ctx* ctx, int block, int surface,
                             int scheme, const double* vars, int nvars,
                             int vertex_type);
@@ -161,13 +170,16 @@ int mplbhip_iterate(mplbhip_ctx* ctx, double criteria, int check_period,
  * runs one case per process). */
 int mplbhip_define_block_collision(mplbhip_ctx* ctx, int block, double tau_ref,
                                    double dt);
-/* Step every block until its own velocity residual (the CalcResidualError
- * ratio of u, v[, w], maximum over components, divided by
- * check_period * dt_block) falls below criteria, freezing each block as it
- * converges (a frozen block costs one early-exit per workgroup per step).
- * steps_per_block[b] receives the step at which block b converged (-1 if
- * it did not within max_steps); residual_per_block[b] its last residual;
- * *steps_done the steps taken.  Requires blocks without connections. */
+/* Step every instance until its own velocity residual (the CalcResidualError
+ * ratio of u, v[, w] summed over the instance's blocks, maximum over
+ * components, divided by check_period * dt) falls below criteria, freezing
+ * each instance as it converges (a frozen block costs one early-exit per
+ * workgroup per step; its halo copies stop).  An instance is a block, or
+ * the blocks joined by connections (a periodic channel's self-linked block,
+ * a channel of several linked blocks); its blocks must share one time step.
+ * steps_per_block[b] receives the step at which block b's instance
+ * converged (-1 if it did not within max_steps); residual_per_block[b] its
+ * last residual; *steps_done the steps taken. */
 int mplbhip_iterate_batch(mplbhip_ctx* ctx, double criteria, int check_period,
                           long long max_steps, long long* steps_per_block,
                           double* residual_per_block, long long* steps_done);

@@ -16,7 +16,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
-SOURCES = ["plan.cpp", "mplbhip.cpp", "lattice_kernels.hip"]
+SOURCES = ["plan.cpp", "mplbhip.cpp", "lattice_kernels.hip", "swap_kernels.hip"]
 
 
 WS_OUT = os.path.join(HERE, "libmplbws.so")

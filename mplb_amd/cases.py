@@ -23,6 +23,7 @@ from .enums import VertexType as VT
 
 CS = math.sqrt(3.0)
 NOISE_SEED = 0x4D504C42
+SCHEME_SWAP = 11  # SchemeType Scheme_StreamCollision_Swap (Src/scheme.h:71)
 _MASK = (1 << 64) - 1
 
 
@@ -277,9 +278,12 @@ def from_config(be, cfg):
                                    cfg.get("ToBoundarySurface") or [],
                                    cfg.get("BlockConnectionType") or []):
         be.add_connection(frm, BS[fs], to, BS[ts], VT[ty])
+    if cfg.get("SchemeType") == "Scheme_StreamCollision_Swap":
+        be.set_scheme(SCHEME_SWAP)
     force = cfg["BodyForceType"][0]
     force_macros = any(t.endswith("_Force") for t in cfg["MacroVarTypes"])
-    be.set_collision(cfg["TauRef"][0], force_type=1 if force == "BodyForce_1st" else 0,
+    be.set_collision(cfg["TauRef"][0],
+                     force_type=1 if force in ("BodyForce_1st", "BodyForce_1st_Swap") else 0,
                      force_macros=force_macros)
     k = 0
     while ("BoundaryCondition%d" % k) in cfg and cfg["BoundaryCondition%d" % k]:
@@ -389,3 +393,147 @@ def batched_cavities(be, sizes, res, ulid=0.05, masks=None):
         be.set_macros(b, np.ones(shp), np.zeros(shp), np.zeros(shp))
     be.init_feq()
     return params
+
+
+def _channel_blocks(be, first, sizes, ny, tau_l, g, per_block_collision):
+    """A force-driven channel periodic in x, made of len(sizes) blocks of
+    sizes[k] x ny nodes joined end to end (VirtualBoundary junctions, the
+    last block linked back to the first through MDPeriodic faces), walls
+    Top and Bottom; block ids first .. first + len(sizes) - 1."""
+    nb = len(sizes)
+    ids = [first + k for k in range(nb)]
+    for k, nx in enumerate(sizes):
+        be.add_block(ids[k], (nx, ny))
+    for k in range(nb):
+        a, b = ids[k], ids[(k + 1) % nb]
+        vt = VT.MDPeriodic if k == nb - 1 else VT.VirtualBoundary
+        be.add_connection(a, BS.Right, b, BS.Left, vt)
+        be.add_connection(b, BS.Left, a, BS.Right, vt)
+    for k in range(nb):
+        right = VT.MDPeriodic if k == nb - 1 else VT.VirtualBoundary
+        left = VT.MDPeriodic if k == 0 else VT.VirtualBoundary
+        be.add_boundary(ids[k], BS.Left, BSch.None_, (), left)
+        be.add_boundary(ids[k], BS.Right, BSch.None_, (), right)
+        for s in ("Top", "Bottom"):
+            be.add_boundary(ids[k], BS[s], BSch.EQMDiffuseRefl, (0.0, 0.0), VT.Wall)
+        if per_block_collision:
+            be.set_block_collision(ids[k], tau_ref_for(tau_l, 1.0), 1.0)
+    return ids
+
+
+def channel_multi(be, sizes, ny, tau_l, g):
+    """One force-driven periodic channel of one or more blocks (as one
+    instance of batched_mixed builds it), on its own: dt = 1,
+    BodyForce_1st with the *_Force velocity macros."""
+    ids = _channel_blocks(be, 0, sizes, ny, tau_l, g, False)
+    be.set_collision(tau_ref_for(tau_l, 1.0), force_type=1, force_macros=True)
+    be.prepare()
+    for b in ids:
+        be.set_accel(b, (g, 0.0))
+    for b, nx in zip(ids, sizes):
+        shp = (ny, nx)
+        be.set_macros(b, np.ones(shp), np.zeros(shp), np.zeros(shp))
+    be.init_feq()
+    be.set_dt(1.0)
+    return dict(dt=1.0, blocks=ids)
+
+
+def batched_mixed(be, instances, ulid=0.05):
+    """Independent instances of different kinds in one context, each with
+    its own collision constants: ("cavity", n, re) is the 2D cavity of
+    cavity2d; ("channel", [nx, ...], ny, tau_l, g) a force-driven periodic
+    channel of one or more blocks (channel_multi).  The context runs
+    BodyForce_1st with the *_Force macros throughout; the cavities' blocks
+    have zero acceleration, so their arithmetic is that of a cavity without
+    force.  Returns, per instance, its block ids and dt."""
+    out = []
+    nb = 0
+    lid = CS * ulid
+    accel = []
+    for inst in instances:
+        if inst[0] == "cavity":
+            _, n, re = inst
+            mesh, dt, tau_l = cavity_params(n, re, ulid)
+            be.add_block(nb, (n, n))
+            be.set_block_collision(nb, tau_ref_for(tau_l, dt), dt)
+            for s in ("Left", "Right", "Top", "Bottom"):
+                vel = (lid, 0.0) if s == "Top" else (0.0, 0.0)
+                be.add_boundary(nb, BS[s], BSch.EQMDiffuseRefl, vel, VT.Wall)
+            out.append(dict(kind="cavity", blocks=[nb], dt=dt, sizes=[(n, n)]))
+            nb += 1
+        else:
+            _, sizes, ny, tau_l, g = inst
+            ids = _channel_blocks(be, nb, sizes, ny, tau_l, g, True)
+            accel += [(b, g) for b in ids]
+            out.append(dict(kind="channel", blocks=ids, dt=1.0,
+                            sizes=[(nx, ny) for nx in sizes]))
+            nb += len(ids)
+    be.set_collision(1.0, force_type=1, force_macros=True)
+    be.prepare()
+    for b, g in accel:
+        be.set_accel(b, (g, 0.0))
+    for inst in out:
+        for b, (nx, ny) in zip(inst["blocks"], inst["sizes"]):
+            shp = (ny, nx)
+            be.set_macros(b, np.ones(shp), np.zeros(shp), np.zeros(shp))
+    be.init_feq()
+    return out
+
+
+def cavity3d_swap(be, n, re=100.0, ulid=0.05, noise=True, size=None,
+                  order=("Left", "Right", "Top", "Bottom", "Front", "Back")):
+    """Apps/3DCavity-Swap/lbm3d_cavity_swap.cpp:72-110: the 3D cavity on the
+    swap scheme (Scheme_StreamCollision_Swap, Collision_BGKIsothermal2nd_Swap,
+    BodyForce_None_Swap), EQMDiffuseRefl walls, the lid along x on Top."""
+    mesh, dt, tau_l = cavity_params(n, re, ulid)
+    size = tuple(size) if size is not None else (n, n, n)
+    be.add_block(0, size)
+    be.set_scheme(SCHEME_SWAP)
+    be.set_collision(tau_ref_for(tau_l, dt), force_type=0)
+    lid = CS * ulid
+    for s in order:
+        vel = (lid, 0.0, 0.0) if s == "Top" else (0.0, 0.0, 0.0)
+        be.add_boundary(0, BS[s], BSch.EQMDiffuseRefl, vel, VT.Wall)
+    be.prepare()
+    shp = tuple(reversed(size))
+    rho = density_noise(shp, amplitude=1e-3) if noise else np.ones(shp)
+    z = np.zeros(shp)
+    be.set_macros(0, rho, z, z, z)
+    be.init_feq()
+    be.set_dt(dt)
+    return dict(dt=dt, tau_l=tau_l, lid=lid)
+
+
+def swap_box(be, size, tau_l, rho, u, v, w, force=None, fd_axis=None):
+    """A 3D box on the swap scheme, self-linked on every face (MDPeriodic)
+    or, along fd_axis, joined to itself by FDPeriodic connections with
+    FDPeriodic boundaries on those two faces and stationary EQMDiffuseRefl
+    walls on the others."""
+    be.add_block(0, size)
+    be.set_scheme(SCHEME_SWAP)
+    faces = [(BS.Left, BS.Right), (BS.Bottom, BS.Top), (BS.Back, BS.Front)]
+    dt = 1.0
+    be.set_collision(tau_ref_for(tau_l, dt), force_type=1 if force else 0,
+                     force_macros=bool(force))
+    for ax, (lo, hi) in enumerate(faces):
+        if fd_axis is None:
+            be.add_connection(0, lo, 0, hi, VT.MDPeriodic)
+            be.add_connection(0, hi, 0, lo, VT.MDPeriodic)
+        elif ax == fd_axis:
+            be.add_connection(0, lo, 0, hi, VT.FDPeriodic)
+            be.add_connection(0, hi, 0, lo, VT.FDPeriodic)
+    for ax, (lo, hi) in enumerate(faces):
+        for s in (lo, hi):
+            if fd_axis is None:
+                be.add_boundary(0, s, BSch.None_, (), VT.MDPeriodic)
+            elif ax == fd_axis:
+                be.add_boundary(0, s, BSch.FDPeriodic, (), VT.FDPeriodic)
+            else:
+                be.add_boundary(0, s, BSch.EQMDiffuseRefl, (0.0, 0.0, 0.0), VT.Wall)
+    be.prepare()
+    if force:
+        be.set_accel(0, force)
+    be.set_macros(0, rho, u, v, w)
+    be.init_feq()
+    be.set_dt(dt)
+    return dict(dt=dt)

@@ -30,6 +30,7 @@ hipError_t launch_lift(int dim, const KParams& kp, int64_t nodes, hipStream_t s)
 hipError_t launch_rest_macros(const KParams& kp, int b, int64_t nodes, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t s);
 hipError_t scratch_address(double** ptr);
+hipError_t launch_swap(int part, const KParams& kp, int nbnd, hipStream_t s);
 }  // namespace mplbhip
 
 using namespace mplbhip;
@@ -60,6 +61,10 @@ struct mplbhip_ctx {
     std::vector<std::array<double, 3>> accel;
     bool prepared = false;
     bool initialized = false;
+    // Scheme_StreamCollision_Swap: one lattice f (fs[0]) updated in place;
+    // swap_formed once f^0 is in it
+    bool swap = false;
+    bool swap_formed = false;
     bool init_from_f = false;
     bool init_lift = false;  // f^0 = feq + first-order non-equilibrium part
     long long tstep = 0;
@@ -266,12 +271,17 @@ static KParams base_params(mplbhip_ctx* c) {
     return kp;
 }
 
-// fStage halo exchange into buffer `into` (ops_halo_transfer order)
+// fStage halo exchange into buffer `into` (ops_halo_transfer order); the
+// links of frozen instances (iterate_batch) are left alone, so their
+// buffers keep the state they converged with.  The swap scheme exchanges
+// its one lattice f in place: a halo cell written later in the exchange is
+// read as it is (last exchanged, then swapped), as in the reference.
 static int exchange(mplbhip_ctx* c, int into) {
     for (HaloLink& h : c->halos) {
+        if (!c->bp[h.from].active || !c->bp[h.to].active) continue;
         HaloParams hp = h.hp;
         hp.src_cur = c->fs[into].p;
-        hp.src_other = c->fs[1 - into].p;
+        hp.src_other = c->fs[c->swap ? into : 1 - into].p;
         hp.dst = c->fs[into].p;
         hp.other_flag = h.d_flag;
         HIPCHK(c, launch_halo(hp, c->stream));
@@ -320,9 +330,31 @@ static int launch_step_kernels(mplbhip_ctx* c, const KParams& kp, int mode,
     return 0;
 }
 
+// SwapStreamCollision (evolution.cpp:195-252) on the one lattice f:
+// collision and force, halo exchange of f, local swap, swap stream,
+// boundaries; with check, the macros (and residual partials) of the result
+static int enqueue_swap_step(mplbhip_ctx* c, bool check, bool residual) {
+    KParams kp = base_params(c);
+    HIPCHK(c, launch_swap(0, kp, 0, c->stream));
+    if (int rc = exchange(c, 0)) return rc;
+    HIPCHK(c, launch_swap(1, kp, 0, c->stream));
+    HIPCHK(c, launch_swap(2, kp, 0, c->stream));
+    HIPCHK(c, launch_swap(3, kp, c->nbnd_v[0], c->stream));
+    if (check) {
+        kp.want_residual = residual ? 1 : 0;
+        HIPCHK(c, launch_swap(5, kp, 0, c->stream));
+    }
+    return 0;
+}
+
+// the per-block sums of the step's partials: split (bulk + boundary slots)
+// or one whole-lattice launch's tiles (small cases, the swap scheme)
+static int reduce_split(const mplbhip_ctx* c) { return c->fused || c->swap ? 0 : 1; }
+
 // one step from parity c->cur (step kernels, then the halo copies, which
 // wait for both kernels); leaves c->cur flipped
 static int enqueue_step(mplbhip_ctx* c, bool check, bool residual) {
+    if (c->swap) return enqueue_swap_step(c, check, residual);
     KParams kp = base_params(c);
     kp.want_residual = residual ? 1 : 0;
     const int mode = check ? MODE_STEP_CHECK : MODE_STEP;
@@ -368,11 +400,33 @@ static int ensure_init(mplbhip_ctx* c) {
 
 // materialize the current f: macros always, optionally f in host layout
 // (device buffer c->fhost) and the residual against MacroVarsCopy
+static int swap_form(mplbhip_ctx* c);
+
 static int materialize(mplbhip_ctx* c, bool want_f, bool residual,
                        std::vector<double>* acc_out) {
     if (int rc = ensure_init(c)) return rc;
     KParams kp = base_params(c);
     const size_t fbytes = (size_t)c->Q * c->interior_nodes * sizeof(double);
+    if (c->swap) {
+        // the one lattice holds f itself: measure it in place
+        if (int rc = swap_form(c)) return rc;
+        if (want_f && c->fhost.bytes != fbytes)
+            if (int rc = dalloc(c, c->fhost, fbytes)) return rc;
+        kp.fhost = want_f ? static_cast<double*>(c->fhost.p) : nullptr;
+        kp.want_residual = residual ? 1 : 0;
+        HIPCHK(c, launch_swap(5, kp, 0, c->stream));
+        if (acc_out) {
+            HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
+                                    static_cast<const BlockParams*>(c->blocks.p),
+                                    (int)c->bp.size(), c->ntiles, c->nbulk, 0, 0, 0,
+                                    static_cast<double*>(c->red.p), c->stream));
+            acc_out->resize((c->bp.size() + 1) * kNAcc);
+            HIPCHK(c, hipMemcpyAsync(acc_out->data(), c->red.p, acc_out->size() * sizeof(double),
+                                     hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return 0;
+    }
     if ((want_f || c->tstep == 0) && c->fhost.bytes != fbytes) {
         if (int rc = dalloc(c, c->fhost, fbytes)) return rc;
     }
@@ -570,6 +624,20 @@ int mplbhip_define_collision(mplbhip_ctx* c, double tau_ref, int force_type,
     return 0;
 }
 
+int mplbhip_define_scheme(mplbhip_ctx* c, int scheme) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    if (c->prepared) return fail(c, MPLBHIP_ERR_ARG, "the scheme is defined before prepare");
+    if (scheme != SCHEME_StreamCollision && scheme != SCHEME_Swap)
+        return fail(c, MPLBHIP_ERR_UNSUPPORTED, "scheme %d is not implemented", scheme);
+    if (scheme == SCHEME_Swap && (c->dim != 3 || c->storage != MPLBHIP_STORE_F64))
+        return fail(c, MPLBHIP_ERR_UNSUPPORTED,
+                    "the swap scheme runs D3Q19 (the reference's swap kernels are 3D only, "
+                    "scheme_kernel.inc:957-1398) with FP64 storage");
+    c->plan.scheme = scheme;
+    c->swap = scheme == SCHEME_Swap;
+    return 0;
+}
+
 int mplbhip_define_obstacles(mplbhip_ctx* c, int block, const unsigned char* solid) {
     if (!c || !solid) return MPLBHIP_ERR_ARG;
     if (c->prepared) return fail(c, MPLBHIP_ERR_ARG, "obstacles are defined before prepare");
@@ -592,7 +660,7 @@ int mplbhip_define_boundary(mplbhip_ctx* c, int block, int surface, int scheme,
     if (nvars < 0 || nvars > 8 || (nvars > 0 && !vars))
         return fail(c, MPLBHIP_ERR_ARG, "bad given variables");
     if (scheme != SCH_EQMDiffuseRefl && scheme != SCH_ExtrapolPressure1ST &&
-        scheme != SCH_None && scheme != SCH_MDPeriodic)
+        scheme != SCH_None && scheme != SCH_MDPeriodic && scheme != SCH_FDPeriodic)
         return fail(c, MPLBHIP_ERR_UNSUPPORTED,
                     "boundary scheme %d is not implemented", scheme);
     if (scheme == SCH_EQMDiffuseRefl && nvars < c->dim)
@@ -757,6 +825,9 @@ static int setup_halos(mplbhip_ctx* c) {
         h.hp.inward = 0;
         for (int q = 0; q < c->Q; q++)
             if (cv[h.hp.qmap[q]][m.at] * m.st < 0) h.hp.inward |= 1u << q;
+        // the swap scheme exchanges the swapped slots and FDPeriodic copies
+        // whole halo cells: every population
+        if (c->swap) h.hp.inward = (1u << c->Q) - 1u;
         if (anyother && c->device >= 0) {
             HIPCHK(c, hipMalloc(&h.d_flag, ncell));
             HIPCHK(c, hipMemcpy(h.d_flag, flag.data(), ncell, hipMemcpyHostToDevice));
@@ -1062,6 +1133,8 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         c->overlap_env = true;
     }
     if (int rc = build_obstacles(c)) return rc;
+    if (c->swap && !c->obst.empty())
+        return fail(c, MPLBHIP_ERR_UNSUPPORTED, "obstacles are not supported by the swap scheme");
     // the two-node FP16 kernel has no obstacle path
     c->h2_ok = c->storage == MPLBHIP_STORE_F16 && c->obst.empty();
     c->h2 = c->h2_ok;
@@ -1173,6 +1246,7 @@ int mplbhip_init_feq(mplbhip_ctx* c) {
     c->init_from_f = false;
     c->init_lift = false;
     c->initialized = true;
+    c->swap_formed = false;
     // fStage^0 = collide(feq) is formed lazily at the first step, once the
     // time step is known (the Apps call SetTimeStep after this), from the
     // macros as they are now (PreDefinedInitialCondition reads them here,
@@ -1189,6 +1263,8 @@ int mplbhip_init_feq(mplbhip_ctx* c) {
 }
 
 int mplbhip_init_lift(mplbhip_ctx* c) {
+    if (c && c->swap)
+        return fail(c, MPLBHIP_ERR_UNSUPPORTED, "the swap scheme starts from feq or a given f");
     if (int rc = mplbhip_init_feq(c)) return rc;
     c->init_lift = true;
     return 0;
@@ -1211,6 +1287,7 @@ int mplbhip_set_f(mplbhip_ctx* c, int block, const double* f) {
     c->init_from_f = true;
     c->init_lift = false;
     c->initialized = true;
+    c->swap_formed = false;
     return 0;
 }
 
@@ -1310,8 +1387,31 @@ static int autotune(mplbhip_ctx* c) {
     return 0;
 }
 
+// the swap scheme's f^0 in its one lattice: feq of the initial macros, or
+// the f given by mplbhip_set_f; the halos start at zero like a new dat
+static int swap_form(mplbhip_ctx* c) {
+    if (c->swap_formed) return 0;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, launch_fill(c->fs[0].p, (int64_t)c->fs[0].bytes, c->stream));
+    KParams kp = base_params(c);
+    if (c->init_from_f) {
+        const size_t fbytes = (size_t)c->Q * c->interior_nodes * sizeof(double);
+        if (c->fhost.bytes != fbytes)
+            if (int rc = dalloc(c, c->fhost, fbytes)) return rc;
+        HIPCHK(c, hipMemcpyAsync(c->fhost.p, c->host_init_f.data(), fbytes,
+                                 hipMemcpyHostToDevice, c->stream));
+        kp.fhost = static_cast<double*>(c->fhost.p);
+        kp.fhost_in = 1;
+    }
+    HIPCHK(c, launch_swap(4, kp, 0, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->swap_formed = true;
+    return 0;
+}
+
 // the collision of f^0 happens here, with the time step then in force
 static int first_collision(mplbhip_ctx* c) {
+    if (c->swap) return swap_form(c);
     // every block starts from buffer 0
     for (int i = 0; i < 2; i++)
         HIPCHK(c, launch_fill(c->parity[i].p, (int64_t)c->parity[i].bytes, c->stream));
@@ -1356,6 +1456,16 @@ static int run_steps(mplbhip_ctx* c, int nsteps, bool timed, double* ms_total,
         HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
         for (int s = 0; s < nsteps; s++) {
             KParams kp = base_params(c);
+            if (c->swap) {
+                // the whole swap step is the bulk share
+                HIPCHK(c, hipEventRecord(evs[4 * s], c->stream));
+                if (int rc = enqueue_swap_step(c, false, false)) return rc;
+                HIPCHK(c, hipEventRecord(evs[4 * s + 1], c->stream));
+                HIPCHK(c, hipEventRecord(evs[4 * s + 2], c->stream));
+                HIPCHK(c, hipEventRecord(evs[4 * s + 3], c->stream));
+                c->tstep++;
+                continue;
+            }
             if (kp.fused) {
                 // one whole-lattice launch: it is the bulk share
                 HIPCHK(c, hipEventRecord(evs[4 * s], c->stream));
@@ -1449,7 +1559,7 @@ int mplbhip_iterate(mplbhip_ctx* c, double criteria, int check_period,
             HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
                                     static_cast<const BlockParams*>(c->blocks.p),
                                     (int)c->bp.size(), c->ntiles, step_nbulk(c),
-                                    c->nbnd_v[step_fold(c)], c->fused ? 0 : 1,
+                                    c->nbnd_v[step_fold(c)], reduce_split(c),
                                     step_fold(c), static_cast<double*>(c->red.p), c->stream));
             HIPCHK(c, hipMemcpyAsync(acc, static_cast<double*>(c->red.p) +
                                               c->bp.size() * kNAcc,
@@ -1538,13 +1648,33 @@ int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
     if (int rc = ensure_init(c)) return rc;
     if (check_period <= 0 || max_steps < 0 || criteria < 0)
         return fail(c, MPLBHIP_ERR_ARG, "bad criteria, check period or step limit");
-    if (!c->plan.links.empty())
-        return fail(c, MPLBHIP_ERR_UNSUPPORTED,
-                    "instances with block connections cannot be frozen one by one");
     if (c->restart_factor > 0 && c->plan.persist_total > 0)
         return fail(c, MPLBHIP_ERR_UNSUPPORTED,
                     "instances with persisted boundary state cannot restart at rest");
+    if (c->restart_factor > 0 && !c->plan.links.empty())
+        return fail(c, MPLBHIP_ERR_UNSUPPORTED,
+                    "instances with block connections cannot restart at rest");
+    if (c->swap)
+        return fail(c, MPLBHIP_ERR_UNSUPPORTED, "batched instances run the stream-collision scheme");
     const int nb = (int)c->bp.size();
+    // instances: the blocks joined by connections (a periodic channel's
+    // self-linked block, or several linked blocks) converge and freeze
+    // together; each instance's blocks share one time step
+    std::vector<int> root(nb);
+    for (int b = 0; b < nb; b++) root[b] = b;
+    auto find = [&](int b) {
+        while (root[b] != b) b = root[b] = root[root[b]];
+        return b;
+    };
+    for (const LinkDesc& l : c->plan.links) root[find(l.from)] = find(l.to);
+    std::vector<std::vector<int>> inst_blocks(nb);
+    for (int b = 0; b < nb; b++) inst_blocks[find(b)].push_back(b);
+    for (int r = 0; r < nb; r++)
+        for (int b : inst_blocks[r])
+            if (c->bp[b].dt != c->bp[inst_blocks[r][0]].dt)
+                return fail(c, MPLBHIP_ERR_ARG,
+                            "linked blocks %d and %d have different time steps", b,
+                            inst_blocks[r][0]);
     std::vector<long long> conv(nb, -1);
     std::vector<double> res(nb, -1.0);
     std::vector<double> minres(nb, INFINITY);
@@ -1582,17 +1712,21 @@ int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
         HIPCHK(c, hipMemcpyAsync(acc.data(), c->red.p, acc.size() * sizeof(double),
                                  hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        bool changed = false;
+        bool changed = false, linked_frozen = false;
         std::vector<int> restart;
         for (int b = 0; b < nb; b++) {
-            if (!c->bp[b].active) continue;
-            const double* a = acc.data() + (size_t)b * kNAcc;
+            if (!c->bp[b].active || find(b) != b) continue;
+            // the instance's sums over its blocks (CalcResidualError sums
+            // over every block of a case, flowfield_wrapper.cpp:31-89)
+            double a[kNAcc] = {0};
+            for (int m : inst_blocks[b])
+                for (int k = 0; k < kNAcc; k++) a[k] += acc[(size_t)m * kNAcc + k];
             double mx = 0.0;
             for (int v = 1; v < 1 + c->dim; v++) {
                 const double rel = (a[v] / a[4 + v]) / (check_period * c->bp[b].dt);
                 if (mx <= rel) mx = rel;
             }
-            res[b] = mx;
+            for (int m : inst_blocks[b]) res[m] = mx;
             // fallback: an instance whose residual climbs well above the
             // lowest it reached (or is no longer a number) restarts once at rest
             if (c->restart_factor > 0 && c->batch_restarts[b] == 0 &&
@@ -1604,12 +1738,18 @@ int mplbhip_iterate_batch(mplbhip_ctx* c, double criteria, int check_period,
             }
             if (mx < minres[b]) minres[b] = mx;
             if (mx < criteria) {
-                c->bp[b].active = 0;
-                conv[b] = c->tstep;
-                remaining--;
+                for (int m : inst_blocks[b]) {
+                    c->bp[m].active = 0;
+                    conv[m] = c->tstep;
+                    remaining--;
+                }
                 changed = true;
+                linked_frozen = linked_frozen || inst_blocks[b].size() > 1 || !c->halos.empty();
             }
         }
+        // captured graphs hold the halo copies of the instances live when
+        // they were captured: a frozen instance's links must drop out
+        if (linked_frozen) drop_graphs(c);
         if (!restart.empty())
             if (int rc = restart_at_rest(c, restart)) return rc;
         if (changed && remaining > 0) {
@@ -1813,7 +1953,8 @@ int mplbhip_mass_noalloc(mplbhip_ctx* c, double* mass) {
     kp.partials = c->scratch;
     kp.fhost = nullptr;
     kp.want_residual = 0;
-    HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_MATERIALIZE, kp, c->stream));
+    if (c->swap) HIPCHK(c, launch_swap(5, kp, 0, c->stream));
+    else HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_MATERIALIZE, kp, c->stream));
     double* out = c->scratch + (size_t)c->ntiles * kNAcc;
     HIPCHK(c, launch_reduce(c->scratch, static_cast<const BlockParams*>(c->blocks.p),
                             (int)c->bp.size(), c->ntiles, c->nbulk, 0, 0, 0, out, c->stream));

@@ -142,6 +142,26 @@ static int face_code_of_surface(int s) {
     }
 }
 
+// The surface the FDPeriodic kernels test for the face part `code` (10 IP
+// ... 31 KM) of geometry vg: the face's own surface, except where
+// KerCutCellPeriodic3D tests another one (boundary_kernel.inc:1392 IP of
+// VG_IPKM_I tests Right, :1468 KP of VG_JMKP_I tests Front, :1550 IP of
+// VG_IPJMKM_I tests Top).
+static int fd_surface_test(int vg, int code) {
+    if (vg == 10310 && code == 10) return BS_Right;
+    if (vg == 21300 && code == 30) return BS_Front;
+    if (vg == 1021310 && code == 10) return BS_Top;
+    switch (code) {
+        case 10: return BS_Left;
+        case 11: return BS_Right;
+        case 20: return BS_Bottom;
+        case 21: return BS_Top;
+        case 30: return BS_Back;
+        case 31: return BS_Front;
+        default: return -100;
+    }
+}
+
 // SetBlockGeometryProperty (flowfield_wrapper.cpp:291-623) at one node of
 // the whole range (halo slabs are outside it and not queried).
 int Plan::geometry(int b, int i, int j, int k) const {
@@ -256,9 +276,23 @@ bool Plan::build() {
                 return false;
             }
     for (const BCDesc& bc : bcs) {
-        if (bc.scheme == SCH_FDPeriodic || bc.scheme == SCH_BounceBack) {
-            error = "boundary scheme not supported (FDPeriodic/BounceBack have "
-                    "no active reference kernel restated here)";
+        if (bc.scheme == SCH_BounceBack) {
+            error = "boundary scheme BounceBack not supported (the reference's kernel is "
+                    "commented out, boundary_kernel.inc:490-532)";
+            return false;
+        }
+        if (bc.scheme == SCH_FDPeriodic && scheme != SCHEME_Swap) {
+            // KerCutCellPeriodic[3D] copies f from the halo cells, which only
+            // the swap scheme exchanges (scheme.cpp:96-105): under
+            // stream-collision they hold the zeros of a new dat
+            error = "FDPeriodic copies f's halo, which the stream-collision scheme never "
+                    "exchanges (only fStage's, scheme.cpp:96-99): use it with "
+                    "Scheme_StreamCollision_Swap";
+            return false;
+        }
+        if (bc.scheme == SCH_ExtrapolPressure1ST && scheme == SCHEME_Swap) {
+            error = "ExtrapolPressure1ST under the swap scheme is not supported (it reads a "
+                    "neighbour that other boundary nodes rewrite in place)";
             return false;
         }
     }
@@ -274,6 +308,8 @@ bool Plan::build() {
             const int vg = geometry(b, p[0], p[1], p[2]);
             region_vt[(size_t)b * nregions + r] = vt;
             region_vg[(size_t)b * nregions + r] = vg;
+            P.vt = vt;
+            P.vg = vg;
             P.force = (vt == VT_Fluid || vt == VT_MDPeriodic) ? 1 : 0;
             P.persist = -1;
             int code[3] = {r % 3, (r / 3) % 3, r / 9};
@@ -298,7 +334,7 @@ bool Plan::build() {
                 const BCDesc& bc = bcs[ib];
                 if (bc.block != b) continue;
                 if (bc.scheme != SCH_EQMDiffuseRefl &&
-                    bc.scheme != SCH_ExtrapolPressure1ST)
+                    bc.scheme != SCH_ExtrapolPressure1ST && bc.scheme != SCH_FDPeriodic)
                     continue;
                 int rr[6];
                 surface_range(dim, n, bc.surface, rr);
@@ -307,6 +343,29 @@ bool Plan::build() {
                 if (P.nops >= kMaxOps) {
                     error = "too many boundary applications on one region";
                     return false;
+                }
+                if (bc.scheme == SCH_FDPeriodic) {
+                    // KerCutCellPeriodic[3D] (boundary_kernel.inc:615-719,
+                    // 1278-1666): one copy of every population per face
+                    // part of the geometry whose surface the kernel tests,
+                    // in its I, J, K order (the last copy wins)
+                    int ax[3], sg[3], inner;
+                    const int nf = vg_decode(vg, ax, sg, &inner);
+                    if (nf == 0 || !inner) continue;
+                    for (int a = 0; a < nf; a++) {
+                        const int code = (ax[a] + 1) * 10 + (sg[a] > 0 ? 0 : 1);
+                        if (nf > 1 && fd_surface_test(vg, code) != bc.surface) continue;
+                        if (P.nops >= kMaxOps) {
+                            error = "too many boundary applications on one region";
+                            return false;
+                        }
+                        BCOp& op = P.op[P.nops++];
+                        std::memset(&op, 0, sizeof op);
+                        op_order[(size_t)b * nregions + r].push_back(ib);
+                        op.type = OP_FDPERIODIC;
+                        op.off[ax[a]] = -sg[a];  // the cell outside that face
+                    }
+                    continue;
                 }
                 BCOp& op = P.op[P.nops++];
                 std::memset(&op, 0, sizeof op);
@@ -404,6 +463,8 @@ bool Plan::build() {
                         if (!(w >> OPP[q] & 1u)) needs |= 1u << OPP[q];
                         w |= 1u << q;
                     }
+            } else if (op.type == OP_FDPERIODIC) {
+                w = all;  // every population copied from outside the block
             } else {
                 w |= op.m_ext;
                 needs |= all & ~w;  // the density sums every population

@@ -40,14 +40,17 @@ enum BoundaryScheme : int {
 enum VG : int { VG_Fluid = 0, VG_ImmersedSolid = -1 };
 
 // ---- device-visible plain data ------------------------------------------
-enum OpType : int32_t { OP_EQM = 1, OP_EXTRAPOL = 2 };
+enum OpType : int32_t { OP_EQM = 1, OP_EXTRAPOL = 2, OP_FDPERIODIC = 3 };
+// SchemeType (Src/scheme.h:69-72)
+enum SchemeKind : int { SCHEME_StreamCollision = 10, SCHEME_Swap = 11 };
 enum RegionKind : int32_t { RK_SKIP = 0, RK_FULL = 1, RK_MASKED = 2 };
 
 struct BCOp {
     int32_t type;
     uint32_t m_in, m_out, m_par;  // EQMDiffuseRefl velocity classes
     uint32_t m_ext;               // extrapolation: populations copied
-    int32_t off[3];               // extrapolation: inward neighbour offset
+    int32_t off[3];               // extrapolation: inward neighbour offset;
+                                  // FDPeriodic: the halo cell copied from
     int32_t nb_region;            // region of that neighbour
     int32_t nb_nops;              // neighbour ops applied before the read
     int32_t pad;
@@ -62,6 +65,8 @@ struct RegionProg {
     int32_t ext[3];      // region extent (nodes) per axis, for persist indexing
     int32_t simple;      // pull (full or masked) + EQM ops only, no persisted
                          // state: the bulk kernel may run it inline
+    int32_t vt, vg;      // node type and geometry of the region (the swap
+                         // scheme's local swap and swap stream decode vg)
     int32_t nops;
     BCOp op[kMaxOps];
 };
@@ -97,6 +102,7 @@ struct BlockDef {
 
 struct Plan {
     int dim = 0, Q = 0;
+    int scheme = SCHEME_StreamCollision;
     std::vector<BlockDef> blocks;
     std::vector<LinkDesc> links;
     std::vector<BCDesc> bcs;

@@ -57,6 +57,7 @@ def load():
         "mplbhip_define_rotated_connection": (I, [P, I, I, I, I, I]),
         "mplbhip_define_obstacles": (I, [P, I, ctypes.c_void_p]),
         "mplbhip_define_collision": (I, [P, D, I, I]),
+        "mplbhip_define_scheme": (I, [P, I]),
         "mplbhip_define_boundary": (I, [P, I, I, I, DP, I, I]),
         "mplbhip_prepare": (I, [P]),
         "mplbhip_node_info": (I, [P, I, I, IP]),
@@ -157,6 +158,11 @@ class LatticeCase:
     def add_rotated_connection(self, frm, fsurf, to, tsurf, vtype):
         self._ck(self.lib.mplbhip_define_rotated_connection(self.h, frm, int(fsurf), to,
                                                             int(tsurf), int(vtype)))
+
+    def set_scheme(self, scheme):
+        """DefineScheme: 10 Scheme_StreamCollision (default), 11
+        Scheme_StreamCollision_Swap (D3Q19, FP64 storage)."""
+        self._ck(self.lib.mplbhip_define_scheme(self.h, int(scheme)))
 
     def set_collision(self, tau_ref, force_type=0, force_macros=False):
         self._ck(self.lib.mplbhip_define_collision(self.h, tau_ref, int(force_type),

@@ -58,6 +58,8 @@ enum { BS_LEFT = 0, BS_RIGHT = 1, BS_TOP = 2, BS_BOTTOM = 3, BS_FRONT = 4,
 enum { SCH_EXTRAPOLPRESSURE1ST = 16, SCH_MDPERIODIC = 18,
        SCH_FDPERIODIC = 19, SCH_BOUNCEBACK = 20, SCH_EQMDIFFUSEREFL = 24,
        SCH_NONE = -1 };
+/* SchemeType (Src/scheme.h:69-72) */
+enum { SCHEME_STREAM_COLLISION = 10, SCHEME_SWAP = 11 };
 /* BndryDvType (Src/boundary_host_device.h:34-39) */
 enum { DV_INCOMING = 1, DV_OUTGOING = 2, DV_PARALLEL = 3, DV_INVALID = -1 };
 
@@ -97,6 +99,7 @@ struct orc_case {
     OBC bc[MAXBC];
     double tau, dt;
     int force_type, force_macros;
+    int scheme;            /* SCHEME_STREAM_COLLISION or SCHEME_SWAP */
     int prepared;
     char err[256];
 };
@@ -224,6 +227,7 @@ orc_case* orc_create(int space_dim, int lattice) {
     c->tau = 1.0;
     c->dt = 1.0; /* DT{1} flowfield.cpp:88 */
     c->force_type = 0;
+    c->scheme = SCHEME_STREAM_COLLISION;
     return c;
 }
 
@@ -374,8 +378,6 @@ int orc_add_boundary(orc_case* c, int block, int surface, int scheme,
                      const double* vars, int nvars, int vtype) {
     if (c->nbc >= MAXBC) return fail(c, "too many boundaries");
     if (block < 0 || block >= c->nb) return fail(c, "undefined block");
-    if (scheme == SCH_FDPERIODIC)
-        return fail(c, "FDPeriodic is not restated by the oracle");
     if (nvars > 8) return fail(c, "too many given variables");
     OBC* bc = &c->bc[c->nbc++];
     memset(bc, 0, sizeof *bc);
@@ -797,7 +799,12 @@ static void collide(orc_case* c) {
 /* ops_halo_transfer of fStage over the halos declared by Field::CreateHalos,
  * field.h:215-395: blocks in id order, neighbours in BoundarySurface order,
  * slabs one node deep that span the transverse halos. */
-static void halo_transfer(orc_case* c) {
+static void halo_transfer_field(orc_case* c, int use_f);
+static void halo_transfer(orc_case* c) { halo_transfer_field(c, 0); }
+
+/* use_f: the halos of f (the swap scheme registers f, scheme.cpp:102-105),
+ * else those of fStage (scheme.cpp:96-99) */
+static void halo_transfer_field(orc_case* c, int use_f) {
     const int Q = c->Q;
     for (int ib = 0; ib < c->nb; ib++) {
         const int order[6] = {BS_LEFT, BS_RIGHT, BS_TOP, BS_BOTTOM, BS_FRONT,
@@ -826,8 +833,10 @@ static void halo_transfer(orc_case* c) {
                             link_target(c, L, p, d);
                             const size_t ns = nidx(a, p[0], p[1], p[2]);
                             const size_t nd = nidx(t, d[0], d[1], d[2]);
+                            double* tf = use_f ? t->f : t->fs;
+                            const double* af = use_f ? a->f : a->fs;
                             for (int q = 0; q < Q; q++)
-                                t->fs[nd * Q + qmap[q]] = a->fs[ns * Q + q];
+                                tf[nd * Q + qmap[q]] = af[ns * Q + q];
                         }
             }
         }
@@ -1061,6 +1070,54 @@ static void bc_extrapol(orc_case* c, OBlock* b, const int r[6], int surface,
             }
 }
 
+/* KerCutCellPeriodic (boundary_kernel.inc:615-719) and
+ * KerCutCellPeriodic3D (:1278-1666): every population of a boundary node
+ * is copied from the node just outside the block along the normal of a
+ * face of its geometry (its halo cell).  A face node copies whatever the
+ * surface; an edge or corner node copies for each of its faces, in the
+ * kernel's I, J, K order, whose surface is the boundary's (a later copy
+ * overwrites an earlier one), except where the kernel tests another
+ * surface (B4): at VG_IPKM_I the IP copy tests Right (:1392), at
+ * VG_JMKP_I the KP copy tests Front (:1468), at VG_IPJMKM_I the IP copy
+ * tests Top (:1550).  Halo cells are never written here, so the order
+ * within the face range does not matter. */
+static int fd_surface_test(int vg, int part_code) {
+    /* the surface a part (face code 10 IP ... 31 KM) of geometry vg tests */
+    if (vg == 10310 && part_code == 10) return BS_RIGHT;   /* VG_IPKM_I */
+    if (vg == 21300 && part_code == 30) return BS_FRONT;   /* VG_JMKP_I */
+    if (vg == 1021310 && part_code == 10) return BS_TOP;   /* VG_IPJMKM_I */
+    switch (part_code) {
+        case 10: return BS_LEFT;
+        case 11: return BS_RIGHT;
+        case 20: return BS_BOTTOM;
+        case 21: return BS_TOP;
+        case 30: return BS_BACK;
+        case 31: return BS_FRONT;
+    }
+    return -100;
+}
+
+static void bc_fdperiodic(orc_case* c, OBlock* b, const int r[6], int surface) {
+    const int Q = c->Q;
+    for (int k = r[4]; k < r[5]; k++)
+        for (int j = r[2]; j < r[3]; j++)
+            for (int i = r[0]; i < r[1]; i++) {
+                const size_t n = nidx(b, i, j, k);
+                const int vg = b->vg[n];
+                int ax[3], sg[3], inner;
+                const int nf = vg_decode(vg, ax, sg, &inner);
+                if (nf == 0 || !inner) continue;
+                for (int a = 0; a < nf; a++) {
+                    const int code = (ax[a] + 1) * 10 + (sg[a] > 0 ? 0 : 1);
+                    if (nf > 1 && fd_surface_test(vg, code) != surface) continue;
+                    int off[3] = {0, 0, 0};
+                    off[ax[a]] = -sg[a]; /* the cell outside the face */
+                    const size_t src = nidx(b, i + off[0], j + off[1], k + off[2]);
+                    for (int q = 0; q < Q; q++) b->f[n * Q + q] = b->f[src * Q + q];
+                }
+            }
+}
+
 /* ImplementBoundary[3D] -> TreatBlockBoundary[3D], boundary.cpp:379-398,
  * boundary_wrapper.cpp:12-131 */
 static void boundaries(orc_case* c) {
@@ -1073,7 +1130,164 @@ static void boundaries(orc_case* c) {
             bc_eqm(c, b, r, bc->vars);
         else if (bc->scheme == SCH_EXTRAPOLPRESSURE1ST)
             bc_extrapol(c, b, r, bc->surface, bc->vars[0]);
+        else if (bc->scheme == SCH_FDPERIODIC)
+            bc_fdperiodic(c, b, r, bc->surface);
     }
+}
+
+/* ---- the swap scheme (Scheme_StreamCollision_Swap, 3D) -------------------
+ * SwapStreamCollision (evolution.cpp:195-252) keeps one lattice, f, with
+ * halos (scheme.cpp:102-105): macros -> collision in place -> body force
+ * added in place -> halo transfer of f -> local swap -> swap stream ->
+ * boundaries.  The velocity order puts opposites half a set apart
+ * (opp(i) = i + 9 for i = 1..9 in D3Q19), which the swaps rely on (J.
+ * Latt, "How to implement your DdQq dynamics with only q variables per
+ * node", the reference's cited method). */
+
+/* KerSwapCollideBGKIsothermal3D, model_kernel.inc:385-408: every node of
+ * the whole range, whatever its type */
+static void swap_collide(orc_case* c) {
+    const int Q = c->Q;
+    const double tau = c->tau, dt = c->dt;
+    const double dtOvertauPlusdt = dt / (tau + 0.5 * dt);
+    for (int ib = 0; ib < c->nb; ib++) {
+        OBlock* b = &c->b[ib];
+        OMP_FOR
+        for (int k = 0; k < b->n[2]; k++)
+            for (int j = 0; j < b->n[1]; j++)
+                for (int i = 0; i < b->n[0]; i++) {
+                    const size_t n = nidx(b, i, j, k);
+                    const double rho = b->m[0][n], u = b->m[1][n], v = b->m[2][n],
+                                 w = b->m[3][n];
+                    double* f = &b->f[n * Q];
+                    for (int q = 0; q < Q; q++) {
+                        const double fe = feq(c, q, rho, u, v, w);
+                        f[q] = fe + (1 - dtOvertauPlusdt) * (f[q] - fe);
+                    }
+                }
+    }
+}
+
+/* KerSwapCalcBodyForce1ST3D, model_kernel.inc:543-568: f += F at Fluid and
+ * MDPeriodic nodes (BodyForce_1st_Swap; BodyForce_None_Swap adds nothing) */
+static void swap_body_force(orc_case* c) {
+    if (c->force_type != 1) return;
+    const int Q = c->Q, D = c->dim;
+    const double CS = c->CS;
+    const double* XI = c->XI;
+    for (int ib = 0; ib < c->nb; ib++) {
+        OBlock* b = &c->b[ib];
+        OMP_FOR
+        for (int k = 0; k < b->n[2]; k++)
+            for (int j = 0; j < b->n[1]; j++)
+                for (int i = 0; i < b->n[0]; i++) {
+                    const size_t n = nidx(b, i, j, k);
+                    const int vt = b->nt[n];
+                    if (!(vt == VT_FLUID || vt == VT_MDPERIODIC)) continue;
+                    for (int q = 0; q < Q; q++) {
+                        double cf = 0;
+                        for (int d = 0; d < D; d++) cf += CS * XI[q * D + d] * b->g[d];
+                        const double F = c->W[q] * b->m[0][n] * cf;
+                        b->f[n * Q + q] += F;
+                    }
+                }
+    }
+}
+
+static int fluid_like(int vt) {
+    return vt == VT_FLUID || vt == VT_MDPERIODIC || vt == VT_VIRTUALBOUNDARY;
+}
+
+static void swap_pair(double* a, double* b) {
+    const double t = *a;
+    *a = *b;
+    *b = t;
+}
+
+/* KerLocalSwap3D, scheme_kernel.inc:959-1194, in the kernel's own
+ * sequence: a fluid node exchanges f_i and f_{i+half}; a boundary node
+ * (inner faces, edges and corners) sets f_i = f_opp(i) for the velocities
+ * that point out of the block along every normal of its geometry (sign *
+ * c <= 0 on each, not all zero), and swaps the pairs that are zero on
+ * every normal (twice, once from each side: an identity) */
+static void local_swap(orc_case* c) {
+    const int Q = c->Q, half = (Q - 1) / 2;
+    for (int ib = 0; ib < c->nb; ib++) {
+        OBlock* b = &c->b[ib];
+        OMP_FOR
+        for (int k = 0; k < b->n[2]; k++)
+            for (int j = 0; j < b->n[1]; j++)
+                for (int i = 0; i < b->n[0]; i++) {
+                    const size_t n = nidx(b, i, j, k);
+                    const int vt = b->nt[n];
+                    double* f = &b->f[n * Q];
+                    if (fluid_like(vt)) {
+                        for (int q = 1; q <= half; q++) swap_pair(&f[q], &f[q + half]);
+                        continue;
+                    }
+                    if (vt == VT_IMMERSEDSOLID) continue;
+                    int ax[3], sg[3], inner;
+                    const int nf = vg_decode(b->vg[n], ax, sg, &inner);
+                    if (nf == 0 || !inner) continue;
+                    for (int q = 1; q < Q; q++) {
+                        int allle = 1, allzero = 1;
+                        for (int a = 0; a < nf; a++) {
+                            const int cn = sg[a] * XIi(c, q, ax[a]);
+                            allle = allle && cn <= 0;
+                            allzero = allzero && cn == 0;
+                        }
+                        if (!allle) continue;
+                        if (allzero) {
+                            if (nf < 3) swap_pair(&f[q], &f[c->OPP[q]]);
+                        } else {
+                            f[q] = f[c->OPP[q]];
+                        }
+                    }
+                }
+    }
+}
+
+/* KerSwapStream3D, scheme_kernel.inc:1196-1398: a fluid node exchanges
+ * f_{i+half}(x) with f_i(x + c_i) for i = 1..half; a boundary node only
+ * where the opposite velocity points out of the block, or along it, on
+ * every normal of its geometry (sign * c_opp <= 0 on each).  Each
+ * exchange touches two values no other exchange touches, so the node
+ * order does not matter. */
+static void swap_stream(orc_case* c) {
+    const int Q = c->Q, half = (Q - 1) / 2;
+    for (int ib = 0; ib < c->nb; ib++) {
+        OBlock* b = &c->b[ib];
+        for (int k = 0; k < b->n[2]; k++)
+            for (int j = 0; j < b->n[1]; j++)
+                for (int i = 0; i < b->n[0]; i++) {
+                    const size_t n = nidx(b, i, j, k);
+                    const int vt = b->nt[n];
+                    if (vt == VT_IMMERSEDSOLID) continue;
+                    int ax[3] = {0, 0, 0}, sg[3] = {0, 0, 0}, inner = 0, nf = 0;
+                    if (!fluid_like(vt)) {
+                        nf = vg_decode(b->vg[n], ax, sg, &inner);
+                        if (nf == 0 || !inner) continue;
+                    }
+                    for (int q = 1; q <= half; q++) {
+                        int ok = 1;
+                        for (int a = 0; a < nf; a++)
+                            ok = ok && sg[a] * XIi(c, c->OPP[q], ax[a]) <= 0;
+                        if (!ok) continue;
+                        const size_t nb = nidx(b, i + XIi(c, q, 0), j + XIi(c, q, 1),
+                                               k + XIi(c, q, 2));
+                        swap_pair(&b->f[n * Q + q + half], &b->f[nb * Q + q]);
+                    }
+                }
+    }
+}
+
+int orc_set_scheme(orc_case* c, int scheme) {
+    if (scheme != SCHEME_STREAM_COLLISION && scheme != SCHEME_SWAP)
+        return fail(c, "scheme %d is not restated", scheme);
+    if (scheme == SCHEME_SWAP && (c->dim != 3 || c->Q != 19))
+        return fail(c, "the swap scheme has D3Q19 kernels only (scheme_kernel.inc:957-1398)");
+    c->scheme = scheme;
+    return 0;
 }
 
 int orc_apply_boundaries(orc_case* c) {
@@ -1085,6 +1299,16 @@ int orc_apply_boundaries(orc_case* c) {
 int orc_step(orc_case* c, int nsteps) {
     if (!c->prepared) return fail(c, "call orc_prepare first");
     for (int s = 0; s < nsteps; s++) {
+        if (c->scheme == SCHEME_SWAP) {
+            orc_update_macros(c);
+            swap_collide(c);
+            swap_body_force(c);
+            halo_transfer_field(c, 1);
+            local_swap(c);
+            swap_stream(c);
+            boundaries(c);
+            continue;
+        }
         orc_update_macros(c);
         envelope_copy(c);
         body_force(c);

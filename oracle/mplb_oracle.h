@@ -65,6 +65,8 @@ int orc_step(orc_case* c, int nsteps);
 /* test hook: the boundary stage of a step alone (ImplementBoundary on the
  * current f, every BlockBoundary in definition order) */
 int orc_apply_boundaries(orc_case* c);
+/* SchemeType: 10 Scheme_StreamCollision (default), 11 Scheme_StreamCollision_Swap */
+int orc_set_scheme(orc_case* c, int scheme);
 /* UpdateMacroVars[3D] (model_wrapper.cpp:112-275, 468-588) */
 void orc_update_macros(orc_case* c);
 /* distribution function of one block, interior only, layout [q][k][j][i] */

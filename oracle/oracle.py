@@ -48,6 +48,7 @@ def load():
         "orc_set_accel": (I, [P, I, DP]),
         "orc_step": (I, [P, I]),
         "orc_apply_boundaries": (I, [P]),
+        "orc_set_scheme": (I, [P, I]),
         "orc_update_macros": (None, [P]),
         "orc_get_f": (I, [P, I, DP]),
         "orc_set_f": (I, [P, I, DP]),
@@ -156,6 +157,10 @@ class OracleCase:
 
     def update_macros(self):
         self.lib.orc_update_macros(self.h)
+
+    def set_scheme(self, scheme):
+        """SchemeType: 10 StreamCollision (default), 11 StreamCollision_Swap."""
+        self._ck(self.lib.orc_set_scheme(self.h, int(scheme)))
 
     def apply_boundaries(self):
         """The boundary stage of a step alone, on the current f."""

@@ -103,11 +103,23 @@ def test_plan_only_context_refuses_compute():
 
 
 def test_unsupported_scheme_is_an_error():
+    """BounceBack has no active reference kernel; FDPeriodic copies f's halo,
+    which only the swap scheme exchanges (refused at prepare under
+    stream-collision); the swap scheme itself is D3Q19 FP64 only."""
     p = PlanOnly(2)
     p.add_block(0, (8, 8))
     p.set_collision(0.1)
     with pytest.raises(hip.MplbError, match="not implemented"):
-        p.add_boundary(0, BS.Left, BSch.FDPeriodic, (), VT.Wall)
+        p.add_boundary(0, BS.Left, BSch.BounceBack, (), VT.Wall)
+    p.add_boundary(0, BS.Left, BSch.FDPeriodic, (), VT.FDPeriodic)
+    with pytest.raises(hip.MplbError, match="Swap"):
+        p.prepare()
+    q = PlanOnly(2)
+    with pytest.raises(hip.MplbError, match="D3Q19"):
+        q.set_scheme(11)
+    r = hip.LatticeCase(3, storage=hip.STORE_F32, device=-1)
+    with pytest.raises(hip.MplbError, match="FP64"):
+        r.set_scheme(11)
 
 
 def test_define_order_errors():

@@ -145,3 +145,47 @@ def test_batch_compaction_matches_full_launches(monkeypatch):
     assert np.allclose(r0, r1, rtol=1e-12, atol=0)
     for b in range(len(sizes)):
         assert np.array_equal(f0[b], f1[b]), b
+
+
+MIXED = [("cavity", 16, 50.0), ("channel", [12], 9, 0.9, 2e-5),
+         ("cavity", 20, 80.0), ("channel", [7, 6], 11, 0.8, 1e-5)]
+
+
+def _oracle_instance(inst):
+    o = OracleCase(2)
+    if inst[0] == "cavity":
+        p = cases.cavity2d(o, inst[1], re=inst[2], ulid=0.05)
+    else:
+        p = cases.channel_multi(o, inst[1], inst[2], inst[3], inst[4])
+    return o, p["dt"]
+
+
+def test_batch_mixed_cavities_and_channels():
+    """Cavities and force-driven periodic channels (one self-linked block,
+    and a channel of two linked blocks) in one batch: each instance
+    converges at the check step where its own velocity residual, summed over
+    its blocks, first meets the criterion, exactly as a single run of it on
+    the oracle, and keeps the state it converged with (bit for bit)."""
+    crit, period, limit = 1e-4, 50, 40000
+    g = hip.LatticeCase(2, storage=hip.STORE_F64, device=0)
+    insts = cases.batched_mixed(g, MIXED)
+    steps, resid, done = g.iterate_batch(crit, period, limit)
+    log("mixed batch steps %s done %d" % (list(steps), done))
+    assert np.all(steps > 0)
+    for inst, spec in zip(insts, MIXED):
+        o, dt = _oracle_instance(spec)
+        t, r = 0, None
+        while t < limit:
+            o.step(period)
+            t += period
+            o.update_macros()
+            _, ratios = o.residual(period)
+            r = max(ratios[1], ratios[2]) / (period * dt)
+            if r < crit:
+                break
+        for k, b in enumerate(inst["blocks"]):
+            assert steps[b] == t, (spec, b, steps[b], t)
+            assert abs(resid[b] - r) <= 1e-9 * r
+            assert np.array_equal(g.get_f(b), o.get_f(k)), (spec, b)
+    # the channels did move and the instances took different times
+    assert len(set(int(s) for s in steps)) > 1

@@ -651,3 +651,174 @@ def test_oracle_lattice_tables(dim):
     for a in range(dim):
         odd = (f[at(2 * a + 1)] - f[at(2 * a + 2)]) / (2 * CS * s * np.array(wref))
         assert np.allclose(odd, [cq[a] for cq in cref], atol=1e-9), a
+
+
+# ---- the swap scheme (SwapStreamCollision, evolution.cpp:195-252) ---------------
+def _swap_cavity(be, n, noise, lid=0.0, order=("Left", "Right", "Top", "Bottom", "Front",
+                                              "Back"), scheme=11, rho=None):
+    """A 3D cavity (EQMDiffuseRefl walls, the lid along x on Top) on the
+    swap scheme (scheme 11) or the stream-collision scheme (10)."""
+    mesh, dt, tau_l = cases.cavity_params(n, 100.0, 0.05)
+    be.add_block(0, (n, n, n))
+    be.set_collision(cases.tau_ref_for(tau_l, dt))
+    if scheme == 11:
+        be.set_scheme(11)
+    for s in order:
+        vel = (lid, 0.0, 0.0) if s == "Top" else (0.0, 0.0, 0.0)
+        be.add_boundary(0, BS[s], BSch.EQMDiffuseRefl, vel, VT.Wall)
+    be.prepare()
+    shp = (n, n, n)
+    r = rho if rho is not None else (cases.density_noise(shp, amplitude=1e-3) if noise
+                                     else np.ones(shp))
+    z = np.zeros(shp)
+    be.set_macros(0, r, z, z, z)
+    be.init_feq()
+    be.set_dt(dt)
+    return dt
+
+
+def test_swap_scheme_interior_equals_streaming():
+    """Away from the walls the swap algorithm (collision in place, local
+    swap, swap stream) is plain BGK streaming: after k steps from the same
+    state, every node farther than k + 1 from the envelope equals the
+    stream-collision scheme's bit for bit (its collision adds tau omega F
+    with F = 0, an exact zero)."""
+    n, k = 12, 3
+    a, b = OracleCase(3), OracleCase(3)
+    _swap_cavity(a, n, True, lid=0.05 * CS, scheme=11)
+    _swap_cavity(b, n, True, lid=0.05 * CS, scheme=10)
+    f0 = b.get_f(0) * (1 + 0.01 * np.random.default_rng(7).standard_normal((19, n, n, n)))
+    a.set_f(0, f0)
+    b.set_f(0, f0)
+    a.step(k)
+    b.step(k)
+    fa, fb = a.get_f(0), b.get_f(0)
+    d = k + 1
+    inner = (slice(None),) + (slice(d, n - d),) * 3
+    assert np.array_equal(fa[inner], fb[inner])
+    assert not np.array_equal(fa, fb)   # the walls differ: bounce-back then EQM
+
+
+def test_swap_scheme_faces_equal_streaming_after_one_step():
+    """At a face of a stationary or moving EQMDiffuseRefl wall the swap
+    scheme's bounce-back (local swap) and face-limited swap stream deliver
+    to the face node and to its fluid neighbours exactly what pull
+    streaming delivers, and the wall scheme overwrites the rest: after one
+    step from an arbitrary state every node that is not on an edge or a
+    corner equals the stream-collision scheme bit for bit.  (Edges and
+    corners differ: there the two schemes' rules are not the same.)"""
+    n = 10
+    a, b = OracleCase(3), OracleCase(3)
+    _swap_cavity(a, n, True, lid=0.05 * CS, scheme=11)
+    _swap_cavity(b, n, True, lid=0.05 * CS, scheme=10)
+    f0 = b.get_f(0) * (1 + 0.01 * np.random.default_rng(9).standard_normal((19, n, n, n)))
+    a.set_f(0, f0)
+    b.set_f(0, f0)
+    a.step(1)
+    b.step(1)
+    fa, fb = a.get_f(0), b.get_f(0)
+    walls = np.zeros((n, n, n), int)
+    for ax in range(3):
+        idx = [slice(None)] * 3
+        for end in (0, n - 1):
+            idx[ax] = end
+            walls[tuple(idx)] += 1
+    same = np.all(fa == fb, axis=0)
+    assert np.all(same[walls <= 1])
+    assert not np.all(same[walls >= 2])
+
+
+def test_swap_scheme_rest_is_fixed_point_and_mirror_symmetric():
+    """Rest inside stationary walls stays at rest; a density perturbation
+    symmetric under x, y and z mirrors stays symmetric (every face, edge
+    and corner rule of the local swap and the swap stream has its mirror
+    image), and the lid along x keeps the z mirror symmetry."""
+    n = 9
+    o = OracleCase(3)
+    _swap_cavity(o, n, False)
+    f0 = o.get_f(0)
+    o.step(5)
+    assert np.max(np.abs(o.get_f(0) - f0)) < 1e-15
+    x = np.arange(n) - (n - 1) / 2
+    rho = 1 + 1e-3 * np.exp(-(x[:, None, None] ** 2 + x[None, :, None] ** 2 +
+                              x[None, None, :] ** 2) / 4.0)
+    o = OracleCase(3)
+    _swap_cavity(o, n, False, rho=rho)
+    o.step(20)
+    o.update_macros()
+    r = o.get_macro(0, 0)
+    assert np.max(np.abs(r - rho)) > 1e-6   # it moved
+    # mirrored nodes add their populations in another order: round-off only
+    for ax in range(3):
+        assert np.allclose(r, np.flip(r, axis=ax), rtol=0, atol=1e-13), ax
+    o = OracleCase(3)
+    _swap_cavity(o, n, False, lid=0.05 * CS)
+    o.step(20)
+    o.update_macros()
+    u, w = o.get_macro(0, 1), o.get_macro(0, 3)
+    assert np.max(np.abs(u)) > 1e-4
+    assert np.allclose(u, np.flip(u, axis=0), rtol=0, atol=1e-13)
+    assert np.allclose(w, -np.flip(w, axis=0), rtol=0, atol=1e-13)
+
+
+def _fd_expected(names, surface, pos):
+    """The halo offsets whose copies a node makes (the last wins), from the
+    reference kernels' case lists (boundary_kernel.inc:615-719,
+    1278-1666), the surface mismatches included."""
+    quirk = {("IP", "IPKM"): "Right", ("KP", "JMKP"): "Front", ("IP", "IPJMKM"): "Top"}
+    key = "".join(names)
+    offs = []
+    for nm in names:
+        test = quirk.get((nm, key), _FACE[nm][0])
+        if len(names) == 1 or test == surface:
+            off = tuple(-x for x in _FACE[nm][1])
+            offs.append(off)
+    return offs
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fdperiodic_copies_from_the_halo_cell(dim):
+    """FDPeriodic (KerCutCellPeriodic[3D]) copies every population of a
+    boundary node from the node just outside the block.  In the
+    stream-collision scheme f's halo is never written (only fStage's halos
+    are exchanged, scheme.cpp:96-99), so the copied values are the zeros
+    of a new dat: the test reads which nodes take which copy from that,
+    for FDPeriodic on every face in turn, against the case lists above
+    (including the reference's surface mismatches)."""
+    n = (5, 6) if dim == 2 else (5, 6, 7)
+    surfaces = ["Left", "Right", "Top", "Bottom"] + (["Front", "Back"] if dim == 3 else [])
+    for fd in surfaces:
+        o = OracleCase(dim)
+        o.add_block(0, n)
+        o.set_collision(0.1)
+        for s in surfaces:
+            if s != fd:
+                o.add_boundary(0, BS[s], BSch.None_, (), VT.Wall)
+        o.add_boundary(0, BS[fd], BSch.FDPeriodic, (), VT.FDPeriodic)
+        o.prepare()
+        shp = tuple(reversed(n))
+        o.set_macros(0, np.ones(shp), *[np.zeros(shp)] * dim)
+        o.init_feq()
+        f = 0.5 + np.random.default_rng(3).random((o.Q,) + shp)
+        o.set_f(0, f)
+        o.apply_boundaries()
+        out = o.get_f(0)
+        nn = list(n) + [1] * (3 - dim)
+        copied = 0
+        for k in range(nn[2]):
+            for j in range(nn[1]):
+                for i in range(nn[0]):
+                    names = _vg_names(dim, nn, i, j, k)
+                    pos = (j, i) if dim == 2 else (k, j, i)
+                    got = out[(slice(None),) + pos]
+                    if fd not in [_FACE[x][0] for x in names]:
+                        assert np.array_equal(got, f[(slice(None),) + pos])
+                        continue
+                    offs = _fd_expected(names, fd, pos)
+                    if offs:
+                        # every copy reads a halo cell: zeros
+                        assert np.all(got == 0.0), (fd, pos, names)
+                        copied += 1
+                    else:
+                        assert np.array_equal(got, f[(slice(None),) + pos]), (fd, pos)
+        assert copied > 0
