@@ -1,7 +1,12 @@
 // case_driver.cpp -- runs any HiLeMMS JSON case file (Config=<file>) through
 // the same call sequence as the reference Apps' simulate(config), on one
 // GPU.  Build with DIM=2 or DIM=3.  Initial state: rest, density 1.
-//   case_driver Config=<file.json> [Dump=<file>]
+//   case_driver Config=<file.json> [Dump=<file>] [Force=<g0>] [ForceSchedule=<g0>]
+// Force gives the body-force hook a constant acceleration g0 along x;
+// ForceSchedule a time-dependent one, g0 (1 + floor(k / 7) mod 3) at step k
+// (time k dt); either is written into g_MacroBodyforce as an App's hook
+// would.
+#include <cmath>
 #include <iostream>
 #include <string>
 
@@ -47,7 +52,37 @@ void SetInitialMacrosVars() {
         }
     }
 }
-void UpdateMacroscopicBodyForce(const Real time) {}
+static double g_force_amp = 0.0;
+static bool g_force_schedule = false;
+
+#ifdef OPS_3D
+void KerSetAccel(ACC<Real>& g, const Real* val) {
+    g(0, 0, 0, 0) = val[0];
+    g(1, 0, 0, 0) = val[1];
+    g(2, 0, 0, 0) = val[2];
+}
+#else
+void KerSetAccel(ACC<Real>& g, const Real* val) {
+    g(0, 0, 0) = val[0];
+    g(1, 0, 0) = val[1];
+}
+#endif
+
+void UpdateMacroscopicBodyForce(const Real time) {
+    if (g_force_amp == 0.0) return;
+    const long long k = std::llround(time / TimeStep());
+    const Real val[3] = {g_force_schedule ? g_force_amp * (1 + (k / 7) % 3) : g_force_amp, 0.0,
+                         0.0};
+    for (auto idBlock : g_Block()) {
+        Block& block{idBlock.second};
+        std::vector<int> rng(block.WholeRange().begin(), block.WholeRange().end());
+        for (auto& idForce : g_MacroBodyforce())
+            ops_par_loop(KerSetAccel, "KerSetAccel", block.Get(), SpaceDim(), rng.data(),
+                         ops_arg_dat(idForce.second.at(block.ID()), SpaceDim(), LOCALSTENCIL,
+                                     "double", OPS_RW),
+                         ops_arg_gbl(val, 3, "double", OPS_READ));
+    }
+}
 
 static void simulate(const Configuration& config) {
     DefineCase(config.caseName, config.spaceDim, config.transient);
@@ -79,10 +114,15 @@ static void simulate(const Configuration& config) {
 #endif
     }
     SetTimeStep(config.meshSize / SoundSpeed());
+    // the cycle an App passes for its scheme (Apps/3DCavity-Swap uses
+    // SwapStreamCollision, lbm3d_cavity_swap.cpp:101-107)
+    void (*cycle)(Real) = config.schemeType == Scheme_StreamCollision_Swap
+                              ? &SwapStreamCollision
+                              : &StreamCollision;
     if (config.transient)
-        Iterate(config.timeStepsToRun, config.checkPeriod, config.currentTimeStep);
+        Iterate(cycle, config.timeStepsToRun, config.checkPeriod, config.currentTimeStep);
     else
-        Iterate(config.convergenceCriteria, config.checkPeriod, config.currentTimeStep);
+        Iterate(cycle, config.convergenceCriteria, config.checkPeriod, config.currentTimeStep);
 }
 
 int main(int argc, const char** argv) {
@@ -98,6 +138,11 @@ int main(int argc, const char** argv) {
     double c0, e0, c1, e1;
     ops_timers(&c0, &e0);
     ReadConfiguration(cfg);
+    g_force_amp = OptD(o, "Force", 0.0);
+    if (o.count("ForceSchedule")) {
+        g_force_amp = OptD(o, "ForceSchedule", 0.0);
+        g_force_schedule = true;
+    }
     if (o.count("DryRun")) {
         // parsed case, no device: blocks, boundaries, steps
         const Configuration& c = Config();

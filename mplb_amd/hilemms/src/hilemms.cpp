@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../../include/mplbhip.h"
+#include "hdf5_io.h"
 #include "mplb.h"
 
 // ---------------------------------------------------------------------------
@@ -65,6 +66,7 @@ struct State {
     mplbhip_ctx* ctx{nullptr};
     bool partitioned{false};
     bool forceMacros{false};
+    SizeType restartStep{0};  // DefineComponents(..., timeStep): restart from files
 };
 
 State& S() {
@@ -125,6 +127,41 @@ int var_slot(const Component& c, int varId) {
     return -1;
 }
 
+// checkpoint files (Src/flowfield.cpp:118-137, Src/field.h:180-205) are
+// written unless MPLB_HDF5=0 (benchmarks that do not want them)
+bool hdf5_enabled() {
+    const char* e = std::getenv("MPLB_HDF5");
+    return !e || std::atoi(e) != 0;
+}
+
+template <typename T>
+void write_field(Field<T>& fld, SizeType step) {
+    for (const auto& idBlock : S().blocks) {
+        if (!fld.Has(idBlock.first)) continue;
+        const std::string& name = idBlock.second.Name();
+        hilemms_h5::write_dat(hilemms_h5::file_name(S().caseName, name, (long long)step), name,
+                              idBlock.first, S().spaceDim, fld.at(idBlock.first));
+    }
+}
+
+template <typename T>
+void read_field(Field<T>& fld, SizeType step) {
+    for (const auto& idBlock : S().blocks) {
+        const std::string& name = idBlock.second.Name();
+        hilemms_h5::read_dat(hilemms_h5::file_name(S().caseName, name, (long long)step), name,
+                             S().spaceDim, fld.at(idBlock.first));
+    }
+}
+
+// the host mirror of f (AoS with its halo, as the reference's field f),
+// created when a checkpoint or a restart first needs it
+void ensure_f_mirror() {
+    State& s = S();
+    if (!s.blocks.empty() && s.f.Has(s.blocks.begin()->first)) return;
+    s.f.SetDataDim(NUMXI);
+    s.f.CreateFieldFromScratch(s.blocks);
+}
+
 void download_macros() {
     const Component& c = the_component();
     for (auto& idField : S().macroVars) {
@@ -139,26 +176,91 @@ void download_macros() {
     }
 }
 
-// a block acceleration must be uniform: libmplbhip keeps one per block
+// the acceleration of one block from its g_MacroBodyforce field, which must
+// be uniform: libmplbhip keeps one per block
+std::array<double, 3> block_accel(ops_dat d, int block) {
+    const double* p = reinterpret_cast<const double*>(d->data.data());
+    std::array<double, 3> g{0, 0, 0};
+    for (int a = 0; a < S().spaceDim; a++) g[a] = p[d->offset(0, 0, 0) + a];
+    for (int k = 0; k < d->size[2]; k++)
+        for (int j = 0; j < d->size[1]; j++)
+            for (int i = 0; i < d->size[0]; i++)
+                for (int a = 0; a < S().spaceDim; a++)
+                    if (p[d->offset(i, j, k) + a] != g[a])
+                        die("Error! the body force must be uniform within a "
+                            "block (block %d)", block);
+    return g;
+}
+
+// accelerations in force on the device, per block
+std::map<int, std::array<double, 3>>& uploaded_accel() {
+    static std::map<int, std::array<double, 3>> m;
+    return m;
+}
+
+// true when the app's hook has written an acceleration that differs from
+// the one in force (a rewrite of the same values only clears the flag)
+bool force_changed() {
+    if (S().macroBodyforce.empty()) return false;
+    RealField& F = S().macroBodyforce.begin()->second;
+    bool changed = false;
+    for (const auto& idBlock : S().blocks) {
+        ops_dat d = F.at(idBlock.first);
+        if (!d->dirty) continue;
+        const auto g = block_accel(d, idBlock.first);
+        auto it = uploaded_accel().find(idBlock.first);
+        if (it != uploaded_accel().end() && it->second == g)
+            d->dirty = false;
+        else
+            changed = true;
+    }
+    return changed;
+}
+
 void upload_force_if_dirty() {
     if (S().macroBodyforce.empty()) return;
     RealField& F = S().macroBodyforce.begin()->second;
     for (const auto& idBlock : S().blocks) {
         ops_dat d = F.at(idBlock.first);
         if (!d->dirty) continue;
-        const double* p = reinterpret_cast<const double*>(d->data.data());
-        double g[3] = {0, 0, 0};
-        for (int a = 0; a < S().spaceDim; a++) g[a] = p[d->offset(0, 0, 0) + a];
-        for (int k = 0; k < d->size[2]; k++)
-            for (int j = 0; j < d->size[1]; j++)
-                for (int i = 0; i < d->size[0]; i++)
-                    for (int a = 0; a < S().spaceDim; a++)
-                        if (p[d->offset(i, j, k) + a] != g[a])
-                            die("Error! the body force must be uniform within a "
-                                "block (block %d)", idBlock.first);
-        ck(mplbhip_set_accel(S().ctx, idBlock.first, g), "set_accel");
+        const auto g = block_accel(d, idBlock.first);
+        auto it = uploaded_accel().find(idBlock.first);
+        if (it == uploaded_accel().end() || it->second != g) {
+            ck(mplbhip_set_accel(S().ctx, idBlock.first, g.data()), "set_accel");
+            uploaded_accel()[idBlock.first] = g;
+        }
         d->dirty = false;
     }
+}
+
+// n device steps from time step iter.  The reference calls the App's
+// UpdateMacroscopicBodyForce(iter * dt) before every step
+// (evolution.cpp:137-151); so does this loop, on the host, at every step's
+// time.  The device applies one acceleration per block to a step (its
+// velocity shift and its source term alike), taken from the hook's first
+// call of this Iterate; a hook that changes it at a later step stops the
+// run with an error instead of being ignored.  (The Apps' hooks are empty.)
+// criteria < 0: the transient form; the macros of check steps are refreshed
+// on the device.  Returns the steps taken (fewer than n only when a check
+// step met criteria); *res the last residual or -1.
+long long advance(SizeType iter, SizeType n, double criteria, SizeType period, double* res,
+                  bool first) {
+    for (SizeType k = 0; k < n; k++) {
+        UpdateMacroscopicBodyForce((iter + k) * TimeStep());
+        if (first && k == 0) {
+            upload_force_if_dirty();
+        } else if (force_changed()) {
+            die("Error! UpdateMacroscopicBodyForce changed the body force at time step %llu "
+                "inside Iterate: the device loop applies the acceleration set before it; a "
+                "time-dependent force is not supported (call Iterate per force value)",
+                (unsigned long long)(iter + k));
+        }
+    }
+    long long done = 0;
+    *res = -1.0;
+    if (n > 0)
+        ck(mplbhip_iterate(S().ctx, criteria, (int)period, (long long)n, &done, res), "Iterate");
+    return done;
 }
 
 }  // namespace
@@ -341,12 +443,27 @@ void PrepareFlowField() {
         if (mv.first == Variable_U_Force || mv.first == Variable_V_Force ||
             mv.first == Variable_W_Force)
             s.forceMacros = true;
-    if (c.collisionType != Collision_BGKIsothermal2nd)
-        die("Error! only Collision_BGKIsothermal2nd is implemented");
-    if (c.bodyForceType != BodyForce_None && c.bodyForceType != BodyForce_1st)
-        die("Error! only BodyForce_None and BodyForce_1st are implemented");
-    ck(mplbhip_define_collision(s.ctx, c.tauRef, (int)c.bodyForceType, s.forceMacros),
-       "collision");
+    // the swap scheme runs the _Swap collision and force kernels
+    // (model_wrapper.cpp:49-60, 305-336); either scheme with the other's
+    // kernels would mix two lattice layouts, so that is an error here
+    const bool swap = s.scheme == Scheme_StreamCollision_Swap;
+    if (swap) {
+        if (c.collisionType != Collision_BGKIsothermal2nd_Swap)
+            die("Error! the swap scheme needs Collision_BGKIsothermal2nd_Swap");
+        if (c.bodyForceType != BodyForce_None_Swap && c.bodyForceType != BodyForce_1st_Swap)
+            die("Error! the swap scheme needs BodyForce_None_Swap or BodyForce_1st_Swap");
+        ck(mplbhip_define_scheme(s.ctx, (int)Scheme_StreamCollision_Swap), "DefineScheme");
+    } else {
+        if (c.collisionType != Collision_BGKIsothermal2nd)
+            die("Error! only Collision_BGKIsothermal2nd is implemented for "
+                "Scheme_StreamCollision");
+        if (c.bodyForceType != BodyForce_None && c.bodyForceType != BodyForce_1st)
+            die("Error! only BodyForce_None and BodyForce_1st are implemented for "
+                "Scheme_StreamCollision");
+    }
+    const int forceType = (c.bodyForceType == BodyForce_1st ||
+                           c.bodyForceType == BodyForce_1st_Swap) ? 1 : 0;
+    ck(mplbhip_define_collision(s.ctx, c.tauRef, forceType, s.forceMacros), "collision");
     for (const BlockBoundary& bc : s.boundaries) {
         if (bc.componentID != c.id) die("Error! boundary names an undefined component");
         ck(mplbhip_define_boundary(s.ctx, bc.blockIndex, (int)bc.boundarySurface,
@@ -376,6 +493,40 @@ void PrepareFlowField() {
                 }
     }
     s.partitioned = true;
+    if (s.restartStep != 0) {
+        // the restart state read from the files: macros (with their
+        // residual copies) and f, which the first step collides
+        for (const auto& idBlock : s.blocks) {
+            const int b = idBlock.first;
+            std::vector<double> m[4];
+            for (int v = 0; v < 4; v++) {
+                const int id = v == 0 ? c.macroVars.at(Variable_Rho).id
+                                      : (v == 1 ? c.uId : (v == 2 ? c.vId : -1));
+#ifdef OPS_3D
+                const int idw = v == 3 ? c.wId : id;
+#else
+                const int idw = id;
+#endif
+                if (idw >= 0 && s.macroVars.count(idw))
+                    m[v] = interior(s.macroVars.at(idw).at(b));
+                else
+                    m[v].assign(m[0].size(), 0.0);
+            }
+            ck(mplbhip_set_macros(s.ctx, b, m[0].data(), m[1].data(), m[2].data(),
+                                  s.spaceDim == 3 ? m[3].data() : nullptr),
+               "restart macros");
+            ops_dat d = s.f.at(b);
+            const double* p = reinterpret_cast<const double*>(d->data.data());
+            const size_t nn = (size_t)d->size[0] * d->size[1] * d->size[2];
+            std::vector<double> f((size_t)NUMXI * nn);
+            size_t n = 0;
+            for (int k = 0; k < d->size[2]; k++)
+                for (int j = 0; j < d->size[1]; j++)
+                    for (int i = 0; i < d->size[0]; i++, n++)
+                        for (int q = 0; q < NUMXI; q++) f[q * nn + n] = p[d->offset(i, j, k) + q];
+            ck(mplbhip_set_f(s.ctx, b, f.data()), "restart f");
+        }
+    }
 }
 
 Real GetMaximumResidual(const SizeType checkPeriod) {
@@ -409,9 +560,39 @@ void DispResidualError(const int iter, const SizeType checkPeriod) {
 
 void CopyCurrentMacroVar() {}  // the device keeps MacroVarsCopy itself
 
-void WriteFlowfieldToHdf5(const SizeType) {}
-void WriteDistributionsToHdf5(const SizeType) {}
-void WriteNodePropertyToHdf5(const SizeType) {}
+// Src/flowfield.cpp:118-137: macros, coordinates and body force; f; node
+// type and geometry, each block into <case>_<block>_T<step>.h5
+void WriteFlowfieldToHdf5(const SizeType timeStep) {
+    if (!hdf5_enabled()) return;
+    for (auto& idField : S().macroVars) write_field(idField.second, timeStep);
+    write_field(S().coordinateXYZ, timeStep);
+    for (auto& idField : S().macroBodyforce) write_field(idField.second, timeStep);
+}
+
+void WriteDistributionsToHdf5(const SizeType timeStep) {
+    if (!hdf5_enabled()) return;
+    State& s = S();
+    ensure_f_mirror();
+    for (const auto& idBlock : s.blocks) {
+        ops_dat d = s.f.at(idBlock.first);
+        const size_t nn = (size_t)d->size[0] * d->size[1] * d->size[2];
+        std::vector<double> f((size_t)NUMXI * nn);
+        ck(mplbhip_get_f(s.ctx, idBlock.first, f.data()), "WriteDistributionsToHdf5");
+        double* p = reinterpret_cast<double*>(d->data.data());
+        size_t n = 0;
+        for (int k = 0; k < d->size[2]; k++)
+            for (int j = 0; j < d->size[1]; j++)
+                for (int i = 0; i < d->size[0]; i++, n++)
+                    for (int q = 0; q < NUMXI; q++) p[d->offset(i, j, k) + q] = f[q * nn + n];
+    }
+    write_field(s.f, timeStep);
+}
+
+void WriteNodePropertyToHdf5(const SizeType timeStep) {
+    if (!hdf5_enabled()) return;
+    write_field(S().geometryProperty, timeStep);
+    for (auto& idField : S().nodeType) write_field(idField.second, timeStep);
+}
 
 // ---------------------------------------------------------------------------
 // model.h
@@ -425,7 +606,6 @@ void DefineComponents(const std::vector<std::string>& compoNames,
     if (s.blocks.empty()) die("Error:please call DefineBlock first!");
     if (compoNames.size() != 1)
         die("Error! this build runs one component per case (got %zu)", compoNames.size());
-    if (timeStep != 0) die("Error! restart from HDF5 files is not available in this build");
     const std::string latt = lattNames.at(0);
     if ((s.spaceDim == 2 && latt != "d2q9") || (s.spaceDim == 3 && latt != "d3q19"))
         die("Error! lattice %s is not implemented (d2q9 in 2D, d3q19 in 3D)", latt.c_str());
@@ -482,13 +662,19 @@ void DefineComponents(const std::vector<std::string>& compoNames,
     IntField nt{"NodeType_" + comp.name};
     nt.CreateFieldFromScratch(s.blocks);
     s.nodeType.emplace(comp.id, nt);
+    // a restart reads f of every block from <case>_<block>_T<step>.h5
+    // (model.cpp:299-311); it reaches the device at Partition
+    s.restartStep = timeStep;
+    if (timeStep != 0) {
+        ensure_f_mirror();
+        read_field(s.f, timeStep);
+    }
 }
 
 void DefineMacroVars(std::vector<VariableTypes> types, std::vector<std::string> names,
                      std::vector<int> varId, std::vector<int> compoId, const SizeType timeStep) {
     State& s = S();
     if (s.components.empty()) die("Error:please call DefineComponent first!");
-    if (timeStep != 0) die("Error! restart from HDF5 files is not available in this build");
     for (size_t i = 0; i < names.size(); i++) {
         MacroVariable mv;
         mv.name = names.at(i);
@@ -497,6 +683,8 @@ void DefineMacroVars(std::vector<VariableTypes> types, std::vector<std::string> 
         s.components.at(compoId.at(i)).macroVars.emplace(mv.type, mv);
         RealField field{mv.name};
         field.CreateFieldFromScratch(s.blocks);
+        // a restart reads the macros from the files (model.cpp:379-388)
+        if (timeStep != 0) read_field(field, timeStep);
         s.macroVars.emplace(mv.id, field);
         if (!s.transient) {
             RealField copy{mv.name + "Copy"};
@@ -613,10 +801,14 @@ SchemeType Scheme() { return S().scheme; }
 int SchemeHaloNum() { return schemeHaloPt; }
 void SetSchemeHaloNum(const int n) { schemeHaloPt = n; }
 void DefineScheme(const SchemeType scheme) {
-    if (scheme != Scheme_StreamCollision)
-        die("Error! only Scheme_StreamCollision is implemented in this build");
+    if (scheme != Scheme_StreamCollision && scheme != Scheme_StreamCollision_Swap)
+        die("Error! only Scheme_StreamCollision and Scheme_StreamCollision_Swap are "
+            "implemented in this build");
     S().scheme = scheme;
-    ops_printf("The stream-collision scheme is chosen!\n");
+    if (scheme == Scheme_StreamCollision_Swap)
+        ops_printf("The swap stream-collision scheme is chosen!\n");
+    else
+        ops_printf("The stream-collision scheme is chosen!\n");
 }
 
 // ---------------------------------------------------------------------------
@@ -664,12 +856,28 @@ void StreamCollision(const Real time) {
     ck(mplbhip_step(S().ctx, 1), "StreamCollision");
 }
 
-void SwapStreamCollision(const Real) {
-    die("Error! the swap scheme is not implemented in this build");
+// SwapStreamCollision (evolution.cpp:195-252): macros from f, collision and
+// force in place, halo exchange of f, local swap, swap stream, boundaries --
+// one mplbhip_step of a context defined with the swap scheme.
+void SwapStreamCollision(const Real time) {
+    if (S().scheme != Scheme_StreamCollision_Swap)
+        die("Error! SwapStreamCollision needs DefineScheme(Scheme_StreamCollision_Swap)");
+    UpdateMacroscopicBodyForce(time);
+    upload_force_if_dirty();
+    ck(mplbhip_step(S().ctx, 1), "SwapStreamCollision");
 }
 
 namespace hilemms_detail {
-bool IsStreamCollision(void (*cycle)(Real)) { return cycle == &StreamCollision; }
+// Either cycle function hands the loop to the library, which runs the scheme
+// the context was defined with; the cycle must match that scheme.
+bool IsStreamCollision(void (*cycle)(Real)) {
+    const bool swap = S().scheme == Scheme_StreamCollision_Swap;
+    if (cycle == &StreamCollision && swap)
+        die("Error! StreamCollision given to a case defined with the swap scheme");
+    if (cycle == &SwapStreamCollision && !swap)
+        die("Error! SwapStreamCollision given to a stream-collision case");
+    return cycle == &StreamCollision || cycle == &SwapStreamCollision;
+}
 void CheckpointUpdate(SizeType iter) {
     ops_printf("%d iterations!\n", (int)iter);
 #ifdef OPS_3D
@@ -702,20 +910,17 @@ void Finish() {
 }  // namespace hilemms_detail
 
 // Iterate(steps, checkPointPeriod) (evolution.cpp:71-100): runs on the
-// device between checkpoints; macros come back at every checkpoint.
+// device between checkpoints (with the force hook at every step's time);
+// macros come back at every checkpoint.
 void Iterate(const SizeType steps, const SizeType checkPointPeriod, const SizeType start) {
     ops_printf("Starting the iteration...\n");
-    State& s = S();
-    UpdateMacroscopicBodyForce(start * TimeStep());
     upload_force_if_dirty();
     SizeType iter = start;
     while (iter < start + steps) {
         const SizeType to_check = checkPointPeriod - (iter % checkPointPeriod);
         const SizeType n = std::min<SizeType>(to_check, start + steps - iter);
-        long long done = 0;
         double res = 0;
-        ck(mplbhip_iterate(s.ctx, -1.0, (int)checkPointPeriod, (long long)n, &done, &res),
-           "Iterate");
+        advance(iter, n, -1.0, checkPointPeriod, &res, iter == start);
         iter += n;
         if (iter % checkPointPeriod == 0) {
             ops_printf("%d iterations!\n", (int)iter);
@@ -733,17 +938,16 @@ void Iterate(const Real convergenceCriteria, const SizeType checkPointPeriod,
              const SizeType start) {
     ops_printf("Starting the iteration...\n");
     State& s = S();
-    UpdateMacroscopicBodyForce(start * TimeStep());
     upload_force_if_dirty();
     SizeType iter = start;
     Real residualError{1};
     do {
-        long long done = 0;
         double res = -1;
-        ck(mplbhip_iterate(s.ctx, convergenceCriteria, (int)checkPointPeriod,
-                           (long long)checkPointPeriod, &done, &res),
-           "Iterate");
-        iter += (SizeType)done;
+        // up to the next check step (the loop's first pass may start
+        // between two of them)
+        const SizeType n = checkPointPeriod - (iter % checkPointPeriod);
+        iter += (SizeType)advance(iter, n, convergenceCriteria, checkPointPeriod, &res,
+                                  iter == start);
         if (iter % checkPointPeriod == 0 && res >= -0.5) {
             const Component& c = the_component();
             double raw[4] = {0, 0, 0, 0}, mx = 0;

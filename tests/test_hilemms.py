@@ -18,7 +18,8 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 @pytest.mark.skipif(not os.path.isdir(REF_APPS), reason="reference Apps not mounted")
 @pytest.mark.parametrize("app,dim", [("2DCavity/lbm2d_cavity.cpp", 2),
                                      ("3DCavity/lbm3d_cavity.cpp", 3),
-                                     ("3DLChannel/lbm3d_L.cpp", 3)])
+                                     ("3DLChannel/lbm3d_L.cpp", 3),
+                                     ("3DCavity-Swap/lbm3d_cavity_swap.cpp", 3)])
 def test_reference_apps_link_unchanged(app, dim, tmp_path):
     """The unmodified App sources build against the HiLeMMS headers and link
     against libmplbhip.so (compile and link only: nothing of the reference
@@ -37,7 +38,9 @@ def test_example_drivers_build():
 
 @pytest.mark.parametrize("name,dim", [("case_cavity2d_16.json", 2),
                                       ("case_cavity3d_10_toporder.json", 3),
-                                      ("case_lchannel_9.json", 3)])
+                                      ("case_lchannel_9.json", 3),
+                                      ("case_cavity3d_swap_12.json", 3),
+                                      ("case_channel3d_fdperiodic.json", 3)])
 def test_json_reader_matches_python(name, dim):
     path = os.path.join(GOLDEN, name)
     cfg = json.load(open(path))
