@@ -22,69 +22,231 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kWaves = 4;  // waves per workgroup
 
+// FP32 -> BF16, round to nearest even (finite inputs), in integer
+// arithmetic: measured faster here than the packed hardware conversion
+// (v_cvt_pk_bf16_f32) in these staging loops
 __device__ __forceinline__ short to_bf16(float f) {
     uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+    u += 0x7FFFu + ((u >> 16) & 1u);
     return (short)(u >> 16);
 }
+// two at once, a in the low half
+__device__ __forceinline__ uint32_t to_bf16x2(float a, float b) {
+    return (uint32_t)(uint16_t)to_bf16(a) | ((uint32_t)(uint16_t)to_bf16(b) << 16);
+}
+__device__ __forceinline__ uint2 to_bf16x4(const float4& f) {
+    return uint2{to_bf16x2(f.x, f.y), to_bf16x2(f.z, f.w)};
+}
 
-// One wave: a 32 (pixels) x 32 (output channels) tile of
-//   out[m][co] = sum_{tap, ci} in[shift_tap(m)][ci] * w[tap][ci][co]
-// with zero padding at the image edges.  MFMA 32x32x8 lane layout: lane l
-// holds A[l%32][4(l/32) .. +3] and B[4(l/32) .. +3][l%32]; the accumulator
+// Forward 3x3 convolution as an implicit GEMM, out[m][co] = sum_{tap, ci}
+// in[shift_tap(m)][ci] * w[tap][ci][co], zero padding at the image edges.
+//
+// A workgroup (4 waves) owns a tile of TP = 128 WM output pixels -- R whole
+// rows of W0 = min(wd, 128) columns -- and CG = 32 WN output channels.
+// Input channels go in chunks of kCC: the chunk's input tile with its
+// one-pixel halo is converted to BF16 once into LDS (zeros outside the
+// image), and so are the chunk's weights, in the MFMA operand layout; the
+// waves then run the 9 taps from LDS only.  Wave v takes pixels
+// [32 WM v, 32 WM (v + 1)) of the tile and all CG channels: WM x WN
+// accumulators of 32x32, each A fragment feeding WN of them and each B
+// fragment WM.  WM = 2 for large images (fewer, fuller workgroups), 1 where
+// that leaves too few workgroups; WN = 1 for layers of at most 32 output
+// channels.
+//
+// MFMA 32x32x8 (BF16 in, FP32 accumulate) lane layout: lane l holds
+// A[l%32][4(l/32) .. +3] and B[4(l/32) .. +3][l%32]; the accumulator
 // element 4i+j of lane l is C[8i + 4(l/32) + j][l%32].
+constexpr int kCC = 32;              // input channels per LDS chunk
+constexpr int kPixStride = kCC + 4;  // BF16 elements per staged pixel (the
+                                     // 8-byte pad spreads a wave's reads
+                                     // over all LDS banks)
+constexpr int kMaxW0 = 128;
+
+struct ConvTile {
+    int W0, R, tiles_x, tiles_y;
+};
+
+__host__ __device__ inline ConvTile conv_tile(int h, int wd, int tp) {
+    ConvTile t;
+    t.W0 = wd < kMaxW0 ? wd : kMaxW0;
+    t.R = tp / t.W0;
+    if (t.R < 1) t.R = 1;
+    if (t.R > h) t.R = h;
+    t.tiles_x = (wd + t.W0 - 1) / t.W0;
+    t.tiles_y = (h + t.R - 1) / t.R;
+    return t;
+}
+
+// pixels staged for a tile of tp pixels: the largest (R + 2)(W0 + 2) over
+// the widths; the host refuses shapes above it
+__host__ __device__ constexpr int conv_max_staged(int tp) { return tp == 256 ? 520 : 390; }
+
+// q = e / d, r = e - q d for 0 <= e < 2^22 and 1 <= d <= 2^10 through a
+// float reciprocal (error below one unit before the correction)
+struct FastDiv {
+    float inv;
+    int d;
+};
+__device__ __forceinline__ FastDiv fast_div(int d) { return FastDiv{1.0f / (float)d, d}; }
+__device__ __forceinline__ int fdiv(int e, const FastDiv& f, int* r) {
+    int q = (int)((float)e * f.inv);
+    int rr = e - q * f.d;
+    if (rr < 0) { q--; rr += f.d; }
+    if (rr >= f.d) { q++; rr -= f.d; }
+    *r = rr;
+    return q;
+}
+
+template <int WM, int WN>
 __global__ __launch_bounds__(64 * kWaves) void k_conv_fwd(const float* __restrict__ in,
                                                           const float* __restrict__ w,
                                                           const float* __restrict__ bias,
                                                           float* __restrict__ out, int n,
                                                           int h, int wd, int cin, int cout,
                                                           int relu) {
+    constexpr int TP = 128 * WM, CG = 32 * WN;
+    __shared__ __attribute__((aligned(16))) short s_in[conv_max_staged(TP) * kPixStride];
+    // B fragments: [tap][kstep][g][co 0 .. CG-1][4]
+    __shared__ __attribute__((aligned(16))) short s_w[9 * (kCC / 8) * 2 * CG * 4];
+    const ConvTile T = conv_tile(h, wd, TP);
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int64_t M = (int64_t)n * h * wd;
-    const int64_t m0 = ((int64_t)blockIdx.x * kWaves + wave) * 32;
-    const int co0 = blockIdx.y * 32;
-    if (m0 >= M) return;
     const int row = lane & 31, g = lane >> 5;
-    // the A row of this lane: one output pixel
-    const int64_t m = m0 + row;
-    const bool mvalid = m < M;
-    const int px = mvalid ? (int)(m % wd) : 0;
-    const int py = mvalid ? (int)((m / wd) % h) : 0;
-    const int64_t pn = mvalid ? m / ((int64_t)wd * h) : 0;
-    const int co = co0 + row;  // the B column of this lane
-    f32x16 acc = {};
-    for (int tap = 0; tap < 9; tap++) {
-        const int yy = py + tap / 3 - 1, xx = px + tap % 3 - 1;
-        const bool inside = mvalid && yy >= 0 && yy < h && xx >= 0 && xx < wd;
-        const float* src = in + ((pn * h + (inside ? yy : 0)) * wd + (inside ? xx : 0)) * cin;
-        const float* wt = w + (int64_t)tap * cin * cout;
-        for (int c0 = 0; c0 < cin; c0 += 8) {
-            const int ci = c0 + 4 * g;
-            bf16x4 a, b;
-            if (inside) {
-                const float4 v = *reinterpret_cast<const float4*>(src + ci);
-                a[0] = to_bf16(v.x); a[1] = to_bf16(v.y); a[2] = to_bf16(v.z); a[3] = to_bf16(v.w);
-            } else {
-                a = bf16x4{0, 0, 0, 0};
-            }
+    const int tiles = T.tiles_x * T.tiles_y;
+    const int pn = (int)blockIdx.x / tiles;
+    const int t = (int)blockIdx.x % tiles;
+    const int y0 = (t / T.tiles_x) * T.R, x0 = (t % T.tiles_x) * T.W0;
+    const int co0 = (int)blockIdx.y * CG;
+    const int SW = T.W0 + 2, SH = T.R + 2;
+    const FastDiv dW0 = fast_div(T.W0);
+    // this lane's A rows (pixels) in its WM pixel tiles; a lane past the
+    // tile reads pixel (0, 0) instead (its results are never stored)
+    int pbase[WM];
 #pragma unroll
-            for (int t = 0; t < 4; t++)
-                b[t] = co < cout ? to_bf16(wt[(int64_t)(ci + t) * cout + co]) : (short)0;
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    for (int mt = 0; mt < WM; mt++) {
+        int tx;
+        int ty = fdiv(wave * 32 * WM + mt * 32 + row, dW0, &tx);
+        if (ty >= T.R) ty = tx = 0;
+        pbase[mt] = ((ty + 1) * SW + tx + 1) * kPixStride + 4 * g;
+    }
+    f32x16 acc[WM][WN] = {};
+    const float* img = in + (int64_t)pn * h * wd * cin;
+    for (int c0 = 0; c0 < cin; c0 += kCC) {
+        const int cc = cin - c0 < kCC ? cin - c0 : kCC;  // a multiple of 8
+        const int q8 = cc / 8;
+        __syncthreads();  // the previous chunk's reads are done
+        // input tile + halo in BF16, channels c0 .. c0 + cc: item e = one
+        // staged pixel's 8 channels (consecutive items: consecutive channel
+        // groups, then pixels); each thread issues the loads of kBatch
+        // items before converting any, so they are in flight together
+        {
+            const int total = SH * SW * q8;
+            const FastDiv dq8 = fast_div(q8), dsw = fast_div(SW);
+            constexpr int kBatch = 4;
+            for (int e0 = threadIdx.x; e0 < total; e0 += 64 * kWaves * kBatch) {
+                float4 f[kBatch][2];
+                int slot[kBatch];
+#pragma unroll
+                for (int k = 0; k < kBatch; k++) {
+                    const int e = e0 + k * 64 * kWaves;
+                    f[k][0] = f[k][1] = float4{0.f, 0.f, 0.f, 0.f};
+                    slot[k] = -1;
+                    if (e < total) {
+                        int c8, sx;
+                        const int sp = fdiv(e, dq8, &c8);
+                        const int sy = fdiv(sp, dsw, &sx);
+                        const int yy = y0 + sy - 1, xx = x0 + sx - 1;
+                        slot[k] = sp * kPixStride + 8 * c8;
+                        if (yy >= 0 && yy < h && xx >= 0 && xx < wd) {
+                            const float* src = img + ((int64_t)yy * wd + xx) * cin + c0 + 8 * c8;
+                            f[k][0] = *reinterpret_cast<const float4*>(src);
+                            f[k][1] = *reinterpret_cast<const float4*>(src + 4);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kBatch; k++) {
+                    if (slot[k] < 0) continue;
+                    *reinterpret_cast<uint2*>(&s_in[slot[k]]) = to_bf16x4(f[k][0]);
+                    *reinterpret_cast<uint2*>(&s_in[slot[k] + 4]) = to_bf16x4(f[k][1]);
+                }
+            }
+        }
+        // weights of the chunk in B-fragment order, zero past cout: thread
+        // -> channel co = tid % CG and the (tap, ci) pairs j = tid / CG +
+        // k (256 / CG), loaded kBatch at a time
+        {
+            const int co = (int)threadIdx.x % CG;
+            const int oc = co0 + co;
+            constexpr int step = 64 * kWaves / CG;
+            constexpr int kBatch = 8;
+            const int pairs = 9 * cc;
+            const FastDiv dcc = fast_div(cc);
+            for (int j0 = (int)threadIdx.x / CG; j0 < pairs; j0 += step * kBatch) {
+                float v[kBatch];
+#pragma unroll
+                for (int k = 0; k < kBatch; k++) {
+                    const int j = j0 + k * step;
+                    int ci;
+                    const int tap = fdiv(j, dcc, &ci);
+                    v[k] = j < pairs && oc < cout ? w[((int64_t)tap * cin + c0 + ci) * cout + oc] : 0.f;
+                }
+#pragma unroll
+                for (int k = 0; k < kBatch; k++) {
+                    const int j = j0 + k * step;
+                    if (j >= pairs) break;
+                    int ci;
+                    const int tap = fdiv(j, dcc, &ci);
+                    const int ks = ci >> 3, gg = (ci >> 2) & 1, tt = ci & 3;
+                    s_w[((((tap * (kCC / 8) + ks) * 2 + gg) * CG) + co) * 4 + tt] = to_bf16(v[k]);
+                }
+            }
+        }
+        __syncthreads();
+        for (int tap = 0; tap < 9; tap++) {
+            const int shift = ((tap / 3 - 1) * SW + (tap % 3 - 1)) * kPixStride;
+            for (int ks = 0; ks < q8; ks++) {
+                bf16x4 a[WM], b[WN];
+#pragma unroll
+                for (int mt = 0; mt < WM; mt++)
+                    a[mt] = *reinterpret_cast<const bf16x4*>(&s_in[pbase[mt] + shift + ks * 8]);
+#pragma unroll
+                for (int nt = 0; nt < WN; nt++)
+                    b[nt] = *reinterpret_cast<const bf16x4*>(
+                        &s_w[((((tap * (kCC / 8) + ks) * 2 + g) * CG) + nt * 32 + row) * 4]);
+#pragma unroll
+                for (int mt = 0; mt < WM; mt++)
+#pragma unroll
+                    for (int nt = 0; nt < WN; nt++)
+                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a[mt], b[nt],
+                                                                                acc[mt][nt], 0, 0, 0);
+            }
         }
     }
+    // epilogue: bias, relu, FP32 store (32 consecutive channels per row)
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int nt = 0; nt < WN; nt++) {
+        const int co = co0 + nt * 32 + row;
+        if (co >= cout) continue;
+        const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int64_t mo = m0 + 8 * i + 4 * g + j;
-            if (mo < M && co < cout) {
-                float v = acc[4 * i + j] + (bias ? bias[co] : 0.f);
-                if (relu) v = v > 0.f ? v : 0.f;
-                out[mo * cout + co] = v;
+        for (int mt = 0; mt < WM; mt++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                int tx1;
+                const int p1 = wave * 32 * WM + mt * 32 + 8 * i + 4 * g;
+                const int ty1 = fdiv(p1, dW0, &tx1);
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    int tx = tx1 + j, ty = ty1;
+                    if (tx >= T.W0) ty = fdiv(p1 + j, dW0, &tx);
+                    if (ty >= T.R || y0 + ty >= h || x0 + tx >= wd) continue;
+                    float v = acc[mt][nt][4 * i + j] + bv;
+                    if (relu) v = v > 0.f ? v : 0.f;
+                    out[(((int64_t)pn * h + y0 + ty) * wd + x0 + tx) * cout + co] = v;
+                }
             }
-        }
+    }
 }
 
 // weights of the data gradient: wt[tap][co][ci] = w[8 - tap][ci][co]
@@ -378,10 +540,32 @@ int mplbws_conv3x3_fwd(const float* in, const float* w, const float* bias, float
     if (!in || !w || !out || n <= 0 || h <= 0 || wd <= 0 || cin <= 0 || cout <= 0 ||
         cin % 8 != 0)
         return MPLBWS_ERR_ARG;
-    const int64_t M = (int64_t)n * h * wd;
-    const dim3 grd((unsigned)((M + 32 * kWaves - 1) / (32 * kWaves)), (unsigned)((cout + 31) / 32));
-    hipLaunchKernelGGL(k_conv_fwd, grd, dim3(64 * kWaves), 0, 0, in, w, bias, out, n, h, wd, cin,
-                       cout, relu);
+    // WM = 2 (256-pixel tiles) unless that gives fewer than 1024
+    // workgroups; WN = 1 for at most 32 output channels
+    const int wn = cout > 32 ? 2 : 1;
+    const int64_t groups = (cout + 32 * wn - 1) / (32 * wn);
+    const ConvTile T2 = conv_tile(h, wd, 256);
+    const int wm = (int64_t)n * T2.tiles_x * T2.tiles_y * groups >= 1024 &&
+                           (T2.R + 2) * (T2.W0 + 2) <= conv_max_staged(256)
+                       ? 2
+                       : 1;
+    const ConvTile T = conv_tile(h, wd, 128 * wm);
+    const int64_t wgs = (int64_t)n * T.tiles_x * T.tiles_y;
+    if (wgs >= ((int64_t)1 << 31) || (T.R + 2) * (T.W0 + 2) > conv_max_staged(128 * wm))
+        return MPLBWS_ERR_ARG;
+    const dim3 grd((unsigned)wgs, (unsigned)groups), blk(64 * kWaves);
+    if (wm == 2 && wn == 2)
+        hipLaunchKernelGGL((k_conv_fwd<2, 2>), grd, blk, 0, 0, in, w, bias, out, n, h, wd, cin,
+                           cout, relu);
+    else if (wm == 2)
+        hipLaunchKernelGGL((k_conv_fwd<2, 1>), grd, blk, 0, 0, in, w, bias, out, n, h, wd, cin,
+                           cout, relu);
+    else if (wn == 2)
+        hipLaunchKernelGGL((k_conv_fwd<1, 2>), grd, blk, 0, 0, in, w, bias, out, n, h, wd, cin,
+                           cout, relu);
+    else
+        hipLaunchKernelGGL((k_conv_fwd<1, 1>), grd, blk, 0, 0, in, w, bias, out, n, h, wd, cin,
+                           cout, relu);
     return status();
 }
 

@@ -408,6 +408,26 @@ class UNet:
             self.b[k].upload(d["b_" + k])
 
 
+def save_weights(net, path, n_train):
+    """The trained network as an .npz file: every layer's weights and
+    biases plus its shape (grid size, encoder widths, bottleneck) and the
+    number of training cases, which fixes the held-out range of
+    case_params."""
+    meta = np.array([net.n, net.bott, n_train] + list(net.enc), dtype=np.int64)
+    np.savez(path, __meta__=meta, **net.state_dict())
+
+
+def load_weights(path, batch=32):
+    """A UNet with the weights save_weights wrote, and its number of
+    training cases."""
+    d = np.load(path, allow_pickle=False)
+    meta = [int(x) for x in d["__meta__"]]
+    n, bott, n_train, enc = meta[0], meta[1], meta[2], tuple(meta[3:])
+    net = UNet(n=n, batch=batch, enc=enc, bott=bott)
+    net.load_state({k: d[k] for k in d.files if k != "__meta__"})
+    return net, n_train
+
+
 # ---- data -------------------------------------------------------------------
 def case_params(count, seed, re_range=(100.0, 1000.0), n=128):
     """Re per case (splitmix64 stream, log-uniform in re_range) and the

@@ -115,7 +115,12 @@ def _empty(shape, zero=False):
                                                 (3, 4, 9, 64, 64, 1), (1, 6, 6, 128, 32, 0),
                                                 (2, 16, 16, 32, 16, 1),
                                                 # several row chunks per image, a ragged row
-                                                (2, 40, 130, 16, 16, 1)])
+                                                (2, 40, 130, 16, 16, 1),
+                                                # the U-Net's own shapes: 128-wide rows
+                                                # with a 72-channel concatenation, and a
+                                                # 16^2 level with 4 input-channel chunks
+                                                (2, 32, 128, 72, 24, 1),
+                                                (3, 16, 16, 128, 64, 0)])
 def test_gpu_conv_fwd_dgrad_wgrad(n, h, w, ci, co, relu):
     print("conv %s" % ((n, h, w, ci, co, relu),), flush=True)
     lib = ws.load()
