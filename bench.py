@@ -35,6 +35,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def traffic_profile_name(workload, storage, fold, xgroup):
+    """The committed rocprofv3 --pmc summary for a launch shape, e.g.
+    k_bulk_d3q19_256_f32_fold2_xgroup1_traffic.json."""
+    return "k_bulk_%s_%s_fold%d_xgroup%d_traffic.json" % (workload, storage, fold, xgroup)
+
+
 def traffic_from_profiles(name):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration)."""
@@ -49,12 +55,26 @@ def traffic_from_profiles(name):
         return None
 
 
-def cpu_baseline(n, seconds):
-    """The FP64 CPU oracle (a port of the reference step) on the host cores,
-    on the full n^3 domain, for as many steps as fill about `seconds`."""
-    from oracle.oracle import OracleCase, get_threads
-    o = OracleCase(3)
-    cases.cavity3d(o, n, re=400.0, ulid=0.05, noise=True)
+def host_info():
+    """What the CPU baseline ran on: the processor model, the cores this
+    process may use, and the OpenMP placement settings."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": len(os.sched_getaffinity(0)),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+            "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+            "OMP_PLACES": os.environ.get("OMP_PLACES")}
+
+
+def _time_oracle(o, nodes, seconds):
+    from oracle.oracle import get_threads
     t0 = time.perf_counter()
     o.step(1)  # first touch of every array, and the step-time estimate
     one = time.perf_counter() - t0
@@ -62,10 +82,20 @@ def cpu_baseline(n, seconds):
     t0 = time.perf_counter()
     o.step(steps)
     dt = time.perf_counter() - t0
-    return {"value": round(n ** 3 * steps / dt / 1e6, 2), "unit": "MLUPS",
-            "cores": get_threads(), "kind": "port",
-            "sample": "FP64 oracle, D3Q19 cavity %d^3 (the benchmark workload), "
-                      "%d steps in %.1f s" % (n, steps, dt)}
+    return {"value": round(nodes * steps / dt / 1e6, 2), "unit": "MLUPS",
+            "cores": get_threads(), "kind": "port", "host": host_info()}, steps, dt
+
+
+def cpu_baseline(n, seconds):
+    """The FP64 CPU oracle (a port of the reference step) on the host cores,
+    on the full n^3 domain, for as many steps as fill about `seconds`."""
+    from oracle.oracle import OracleCase
+    o = OracleCase(3)
+    cases.cavity3d(o, n, re=400.0, ulid=0.05, noise=True)
+    out, steps, dt = _time_oracle(o, n ** 3, seconds)
+    out["sample"] = ("FP64 oracle, D3Q19 cavity %d^3 (the benchmark workload), "
+                     "%d steps in %.1f s" % (n, steps, dt))
+    return out
 
 
 def bench_lchannel(args):
@@ -102,8 +132,10 @@ def bench_lchannel(args):
     kt = 100
     _, t_kernel = g.step_timed(kt)
     t_bulk, t_bnd = g.kernel_times()
-    interior = sum((x - 2) * (y - 2) * (z - 2) for x, y, z in p["sizes"])
-    bulk_bytes = interior * 2 * 19 * ELEM[st]
+    # the nodes one bulk launch updates (interior plus the boundary nodes
+    # folded into it), as for the cavity line
+    fold, xgroup, h2, bulk_nodes = g.launch_shape()
+    bulk_bytes = bulk_nodes * 2 * 19 * ELEM[st]
     achieved = bulk_bytes / (t_bulk / kt / 1e3) / 1e9
     log("bench: L-channel %.0f MLUPS (wall, halos and mass included; %.4f ms/step), "
         "bulk %.4f ms = %.0f GB/s, boundary %.4f ms per step, mass drift %.3e"
@@ -125,6 +157,18 @@ def bench_lchannel(args):
                "mass_rel": abs(g.mass() - ref.mass()) / ref.mass()}
         ref.close()
         log("bench: L-channel error vs FP64: %s" % err)
+    cpu = None
+    if not args.no_cpu:
+        from oracle.oracle import OracleCase
+        o = OracleCase(3)
+        cases.lchannel_periodic(o, a=a, length=length, tau_l=0.8, g=1e-6)
+        cpu, csteps, cdt = _time_oracle(o, nodes, args.cpu_seconds)
+        cpu["sample"] = ("FP64 oracle, the same three-block L-channel, %d steps in %.1f s"
+                         % (csteps, cdt))
+        del o
+        log("bench: CPU baseline %s" % cpu)
+    tname = traffic_profile_name("lchannel", st, fold, xgroup)
+    traffic = traffic_from_profiles(tname)
     return {
         "metric": "MLUPS, three-block L-channel",
         "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": K,
@@ -137,9 +181,18 @@ def bench_lchannel(args):
                    "nodes": nodes},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_bulk (D3Q19, %s, step)" % st,
+                     "traffic": traffic,
+                     "traffic_source": ("profiles/%s (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE "
+                                        "for this launch shape, scaled by the copy-kernel "
+                                        "calibration; not measured in this run)" % tname)
+                     if traffic else
+                     "no committed counter summary for this launch shape (%s)" % tname,
+                     "kernel": "k_bulk (D3Q19, %s, step)" % st,
+                     "bulk_nodes_per_launch": bulk_nodes,
+                     "launch_shape": {"fold": fold, "xgroup": xgroup, "h2": h2},
                      "boundary_and_bulk_ms_per_step": round(t_kernel / kt, 5),
                      "bulk_ms_per_step": round(t_bulk / kt, 5)},
+        "cpu_baseline": cpu,
         "mass_drift_max_rel": drift,
         "mass_every_100": masses,
         "wall_s": round(wall, 3),
@@ -282,10 +335,13 @@ def bench_warmstart(args):
     enc = tuple(int(c) for c in args.ws_enc.split(","))
     coarse = tuple((int(k), float(w)) for k, w in
                    (item.split(":") for item in args.ws_coarse.split(",") if item))
-    res, _ = warmstart.run(n=args.ws_n, n_train=args.ws_train, n_test=args.ws_test,
-                           epochs=args.ws_epochs, enc=enc, bott=args.ws_bott, coarse=coarse,
-                           diagnose=args.ws_diagnose, log=log)
+    res, net = warmstart.run(n=args.ws_n, n_train=args.ws_train, n_test=args.ws_test,
+                             epochs=args.ws_epochs, enc=enc, bott=args.ws_bott, coarse=coarse,
+                             diagnose=args.ws_diagnose, log=log)
     log("bench: warm start %s" % res)
+    if args.ws_save:
+        warmstart.save_weights(net, args.ws_save, n_train=args.ws_train)
+        log("bench: network weights written to %s" % args.ws_save)
     return {
         "metric": "time-to-residual 1e-8, learned vs rest initialisation",
         "value": res["speedup"], "unit": "x (rest seconds / learned seconds)", "n_gpus": 1,
@@ -306,7 +362,9 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--storage", choices=sorted(STORE), default="f32")
+    ap.add_argument("--storage", choices=sorted(STORE), default=None,
+                    help="population storage (default: f32; f16 for --workload lchannel, "
+                         "the three-block L-channel's configured form)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true",
@@ -319,6 +377,8 @@ def main():
     ap.add_argument("--ws-bott", type=int, default=64, help="bottleneck width")
     ap.add_argument("--ws-coarse", default="2:4,4:4",
                     help="large-scale loss terms, pooling-levels:weight pairs ('' for none)")
+    ap.add_argument("--ws-save", default="",
+                    help="write the trained network's weights (.npz) to this path")
     ap.add_argument("--ws-diagnose", action="store_true",
                     help="also solve from exact velocity / exact density splits")
     ap.add_argument("--workload", choices=("cavity3d", "cavity2d", "lchannel", "batch",
@@ -332,6 +392,8 @@ def main():
     ap.add_argument("--check-period", type=int, default=1000)
     ap.add_argument("--max-steps", type=int, default=400000)
     args = ap.parse_args()
+    if args.storage is None:
+        args.storage = "f16" if args.workload == "lchannel" else "f32"
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the multi-GPU path "
               "(face-slab halos over xGMI) is a design, not a run. Use --gpus 1.",
@@ -360,11 +422,21 @@ def main():
     cases.cavity3d(g, n, re=400.0, ulid=0.05, noise=True)
     if W > 0:
         g.step(W)
+    # the timed region: K steps of the production path (mplbhip_step, the
+    # same replays of captured 32-step graphs that Iterate runs)
     g.sync()
     t0 = time.perf_counter()
-    ms_gpu_span, ms_kernel = g.step_timed(K)
+    g.step(K)
     g.sync()
     wall = time.perf_counter() - t0
+    # then K more steps launched one by one with HIP events around each
+    # kernel on the stream it runs on (a captured graph cannot carry
+    # timing events): the kernels' average durations for the roofline
+    g.sync()
+    t1 = time.perf_counter()
+    ms_gpu_span, ms_kernel = g.step_timed(K)
+    g.sync()
+    wall_events = time.perf_counter() - t1
     ms_bulk, ms_bnd = g.kernel_times()
     nodes = n ** 3
     mlups = nodes * K / wall / 1e6
@@ -377,9 +449,10 @@ def main():
     bulk_bytes = bulk_nodes * 2 * q * ELEM[st]
     interior_bytes = (n - 2) ** 3 * 2 * q * ELEM[st]
     achieved = bulk_bytes / bulk_s / 1e9
-    log("bench: wall %.3f s, gpu span %.3f ms, bulk kernel avg %.4f ms (%.0f GB/s), "
-        "boundary kernel avg %.4f ms, %.0f MLUPS"
-        % (wall, ms_gpu_span, bulk_s * 1e3, achieved, ms_bnd / K, mlups))
+    log("bench: wall %.3f s (graph replays), %.3f s with per-kernel events (gpu span "
+        "%.3f ms), bulk kernel avg %.4f ms (%.0f GB/s), boundary kernel avg %.4f ms, "
+        "%.0f MLUPS" % (wall, wall_events, ms_gpu_span, bulk_s * 1e3, achieved, ms_bnd / K,
+                        mlups))
 
     copy_bytes = 2 << 30
     copy_ms = hip.copy_probe(copy_bytes, iters=20)
@@ -387,7 +460,7 @@ def main():
 
     err = None
     if not args.no_error and st != "f64":
-        steps_done = W + K
+        steps_done = W + 2 * K
         log("bench: FP64 reference run, %d steps, for the max-abs error" % steps_done)
         ref = hip.LatticeCase(3, storage=hip.STORE_F64, device=0)
         cases.cavity3d(ref, n, re=400.0, ulid=0.05, noise=True)
@@ -410,6 +483,7 @@ def main():
         log("bench: CPU oracle baseline (about %.0f s at %d^3)" % (args.cpu_seconds, n))
         cpu = cpu_baseline(n, args.cpu_seconds)
 
+    tname = traffic_profile_name("d3q19_%d" % n, st, fold, xgroup)
     out = {
         "metric": METRIC,
         "value": round(mlups, 1),
@@ -430,8 +504,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(
-                         "k_bulk_d3q19_%d_%s_traffic.json" % (n, st)),
+                     "traffic": traffic_from_profiles(tname),
+                     "traffic_source": ("profiles/%s (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE "
+                                        "for this launch shape, scaled by the copy-kernel "
+                                        "calibration; not measured in this run)" % tname)
+                     if traffic_from_profiles(tname) else
+                     "no committed counter summary for this launch shape (%s)" % tname,
                      "kernel": "k_bulk (D3Q19, %s, step)" % st,
                      "algorithmic_bytes": bulk_bytes,
                      "bulk_nodes_per_launch": bulk_nodes,
@@ -440,6 +518,10 @@ def main():
                      "kernel_avg_ms": round(bulk_s * 1e3, 5),
                      "boundary_kernel_avg_ms": round(ms_bnd / K, 5),
                      "step_kernels_avg_ms": round(ms_kernel / K, 5),
+                     "ms_per_step_event_timed": round(wall_events / K * 1e3, 5),
+                     "timing": "value: K steps of graph replays (the production path); "
+                               "kernel times: HIP events around every kernel of K more "
+                               "steps launched one by one on the same streams",
                      "copy_probe_GBs": round(copy_gbs, 1)},
         "cpu_baseline": cpu,
     }
