@@ -73,6 +73,21 @@ int mplbhip_define_connection(mplbhip_ctx* ctx, int from_block,
 int mplbhip_define_rotated_connection(mplbhip_ctx* ctx, int from_block,
                                       int from_surface, int to_block,
                                       int to_surface, int vertex_type);
+/* DefineComponents lattNames (model.cpp:208-310; SetupD2Q9Latt :99-112,
+ * SetupD3Q19Latt :114-131, SetupD3Q15Latt :133-150), before mplbhip_prepare:
+ * the number of velocities, 9 (2D), 19 (3D, the default) or 15 (3D).  D3Q15
+ * cases run every step as one whole-lattice launch. */
+int mplbhip_define_lattice(mplbhip_ctx* ctx, int q);
+/* DefineCollision (model.cpp:406-445), before mplbhip_prepare:
+ * 0 Collision_BGKIsothermal2nd (the default), 1 Collision_BGKThermal4th
+ * (KerCollideBGKThermal[3D], model_kernel.inc:267-310, 457-502: per node
+ * tau = tauRef / (rho sqrt(T)), the 4th-order equilibrium at T and
+ * f - w (f - feq) + tau w F; FP64 storage). */
+int mplbhip_define_collision_type(mplbhip_ctx* ctx, int type);
+/* The Variable_T macro field of one block (interior [k][j][i], T > 0),
+ * which the thermal collision reads; as in the reference nothing updates
+ * it.  After mplbhip_prepare, before the first step. */
+int mplbhip_set_temperature(mplbhip_ctx* ctx, int block, const double* T);
 /* DefineScheme (scheme.cpp:92-109), before mplbhip_prepare: 10
  * Scheme_StreamCollision (the default), 11 Scheme_StreamCollision_Swap (the
  * swap scheme of evolution.cpp:195-252: one lattice f updated in place by

@@ -280,6 +280,8 @@ def from_config(be, cfg):
         be.add_connection(frm, BS[fs], to, BS[ts], VT[ty])
     if cfg.get("SchemeType") == "Scheme_StreamCollision_Swap":
         be.set_scheme(SCHEME_SWAP)
+    if cfg["CollisionType"][0] == "Collision_BGKThermal4th":
+        be.set_collision_type(1)
     force = cfg["BodyForceType"][0]
     force_macros = any(t.endswith("_Force") for t in cfg["MacroVarTypes"])
     be.set_collision(cfg["TauRef"][0],

@@ -26,6 +26,12 @@ namespace mplbhip {
 // sqrt(3) rounded to double: equals the host's sqrt(3.0) (model.cpp:67-68)
 constexpr double kCS = 1.7320508075688772935274463415059;
 
+// The lattice key D of every template below: 2 = D2Q9, 3 = D3Q19,
+// 15 = D3Q15 (Src/model.cpp:133-150); ldim(D) is its space dimension and
+// lq(D) its number of velocities.
+__host__ __device__ constexpr int ldim(int D) { return D == 2 ? 2 : 3; }
+__host__ __device__ constexpr int lq(int D) { return D == 2 ? 9 : (D == 15 ? 15 : 19); }
+
 __host__ __device__ constexpr int cvel(int D, int q, int a) {
     constexpr int t2[9][2] = {{0, 0},  {-1, 1}, {-1, 0}, {-1, -1}, {0, -1},
                               {1, -1}, {1, 0},  {1, 1},  {0, 1}};
@@ -34,7 +40,11 @@ __host__ __device__ constexpr int cvel(int D, int q, int a) {
         {-1, 1, 0}, {-1, 0, -1}, {-1, 0, 1}, {0, -1, -1}, {0, -1, 1},
         {1, 0, 0},  {0, 1, 0},   {0, 0, 1},  {1, 1, 0},  {1, -1, 0},
         {1, 0, 1},  {1, 0, -1},  {0, 1, 1},  {0, 1, -1}};
-    return D == 2 ? (a < 2 ? t2[q][a] : 0) : t3[q][a];
+    constexpr int t15[15][3] = {
+        {0, 0, 0},    {-1, 0, 0},  {0, -1, 0},  {0, 0, -1}, {-1, -1, -1},
+        {-1, -1, 1},  {-1, 1, -1}, {-1, 1, 1},  {1, 0, 0},  {0, 1, 0},
+        {0, 0, 1},    {1, 1, 1},   {1, 1, -1},  {1, -1, 1}, {1, -1, -1}};
+    return D == 2 ? (a < 2 ? t2[q][a] : 0) : (D == 15 ? t15[q][a] : t3[q][a]);
 }
 __host__ __device__ constexpr double wgt(int D, int q) {
     constexpr double w2[9] = {4.0 / 9.0,  1.0 / 36.0, 1.0 / 9.0,
@@ -45,13 +55,19 @@ __host__ __device__ constexpr double wgt(int D, int q) {
         1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
         1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0,
         1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
-    return D == 2 ? w2[q] : w3[q];
+    // 2/9, 1/9, 1/72 as the reference writes them: 2 / ((Real)9) ...
+    constexpr double w15[15] = {
+        2.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 72.0,
+        1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0, 1.0 / 9.0,  1.0 / 9.0,
+        1.0 / 9.0,  1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0};
+    return D == 2 ? w2[q] : (D == 15 ? w15[q] : w3[q]);
 }
 __host__ __device__ constexpr int opp(int D, int q) {
     constexpr int o2[9] = {0, 5, 6, 7, 8, 1, 2, 3, 4};
     constexpr int o3[19] = {0, 10, 11, 12, 13, 14, 15, 16, 17, 18,
                             1, 2,  3,  4,  5,  6,  7,  8,  9};
-    return D == 2 ? o2[q] : o3[q];
+    constexpr int o15[15] = {0, 8, 9, 10, 11, 12, 13, 14, 1, 2, 3, 4, 5, 6, 7};
+    return D == 2 ? o2[q] : (D == 15 ? o15[q] : o3[q]);
 }
 
 // ---- storage traits -------------------------------------------------------
@@ -111,6 +127,37 @@ __device__ __forceinline__ T feq_ref(int q, T rho, T u, T v, T w) {
     return T(wgt(D, q)) * rho * res;
 }
 
+// CalcBGKFeq at temperature Tm with polyOrder 4 (model_host_device.h:34-83),
+// every term in the reference's order; LATTDIM enters as an int
+template <int D>
+__device__ __forceinline__ double feq4_ref(int q, double rho, double u, double v, double w,
+                                           double Tm) {
+#pragma clang fp contract(off)
+    constexpr int LD = ldim(D);
+    const double cs = kCS;
+    double cu, c2, u2;
+    if constexpr (LD == 2) {
+        cu = (cs * double(cvel(D, q, 0)) * u + cs * double(cvel(D, q, 1)) * v);
+        c2 = (cs * double(cvel(D, q, 0)) * cs * double(cvel(D, q, 0)) +
+              cs * double(cvel(D, q, 1)) * cs * double(cvel(D, q, 1)));
+        u2 = u * u + v * v;
+    } else {
+        cu = (cs * double(cvel(D, q, 0)) * u + cs * double(cvel(D, q, 1)) * v +
+              cs * double(cvel(D, q, 2)) * w);
+        c2 = (cs * double(cvel(D, q, 0)) * cs * double(cvel(D, q, 0)) +
+              cs * double(cvel(D, q, 1)) * cs * double(cvel(D, q, 1)) +
+              cs * double(cvel(D, q, 2)) * cs * double(cvel(D, q, 2)));
+        u2 = u * u + v * v + w * w;
+    }
+    const double cu2 = cu * cu;
+    double res = 1.0 + cu + 0.5 * (cu2 - u2 + (Tm - 1.0) * (c2 - LD));
+    res = res + cu * (cu2 - 3.0 * u2 + 3.0 * (Tm - 1.0) * (c2 - LD - 2.0)) / 6.0;
+    res = res + (cu2 * cu2 - 6.0 * cu2 * u2 + 3.0 * u2 * u2) / 24.0 +
+          (Tm - 1.0) * ((c2 - (LD + 2)) * (cu2 - u2) - 2.0 * cu2) / 4.0 +
+          (Tm - 1.0) * (Tm - 1.0) * (c2 * c2 - 2.0 * (LD + 2) * c2 + LD * (LD + 2)) / 8.0;
+    return wgt(D, q) * rho * res;
+}
+
 template <int D, typename T>
 __device__ __forceinline__ T cu_wall(int q, T u, T v, T w) {
 #pragma clang fp contract(off)
@@ -128,7 +175,7 @@ __device__ __forceinline__ T cu_wall(int q, T u, T v, T w) {
 template <int D, typename T>
 __device__ void apply_eqm(T* f, const BCOp& op) {
 #pragma clang fp contract(off)
-    constexpr int Q = D == 2 ? 9 : 19;
+    constexpr int Q = lq(D);
     const T u = T(op.vars[0]), v = T(op.vars[1]), w = T(op.vars[2]);
     T rhoIn = T(0), rhoPar = T(0), dRho = T(0);
 #pragma unroll
@@ -158,7 +205,7 @@ __device__ void apply_eqm(T* f, const BCOp& op) {
 template <int D, typename T>
 __device__ void extrapol_rescale(T* f, double rho_given) {
 #pragma clang fp contract(off)
-    constexpr int Q = D == 2 ? 9 : 19;
+    constexpr int Q = lq(D);
     T rho = T(0);
 #pragma unroll
     for (int q = 0; q < Q; q++) rho += f[q];
@@ -225,7 +272,7 @@ template <int D, typename PR = RegionProg>
 __device__ __forceinline__ int64_t persist_slot(const PR& P, int x, int y, int z) {
     const int l0 = P.ext[0] > 1 ? x - 1 : 0;
     const int l1 = P.ext[1] > 1 ? y - 1 : 0;
-    const int l2 = (D == 3 && P.ext[2] > 1) ? z - 1 : 0;
+    const int l2 = (D != 2 && P.ext[2] > 1) ? z - 1 : 0;
     return (int64_t)P.persist + l0 + (int64_t)P.ext[0] * (l1 + (int64_t)P.ext[1] * l2);
 }
 
@@ -243,7 +290,7 @@ __device__ __forceinline__ void solid_residual(const KParams& kp, int64_t mnode,
     if constexpr (MODE == MODE_STEP_CHECK || MODE == MODE_MATERIALIZE) {
         if (kp.want_residual) {
 #pragma unroll
-            for (int v = 0; v < D + 1; v++) {
+            for (int v = 0; v < ldim(D) + 1; v++) {
                 const double m = kp.macro[v][mnode];
                 const double old = kp.macro_copy[v][mnode];
                 acc[v] = (m - old) * (m - old);
@@ -265,7 +312,7 @@ __device__ void gather_masked(const KParams& kp, const BlockParams& B,
                               const void* per_prev, uint32_t om) {
     using T = typename Store<ST>::T;
     using E = typename Store<ST>::E;
-    constexpr int Q = D == 2 ? 9 : 19;
+    constexpr int Q = lq(D);
     const E* prev = static_cast<const E*>(fs_prev) + B.base;
     const int32_t node = z * B.sz + y * B.sx + x;
     const T* per = static_cast<const T*>(per_prev);
@@ -298,7 +345,7 @@ __device__ void neighbour_values(const KParams& kp, const BlockParams& B,
                                  typename Store<ST>::T* f, const void* fs_prev,
                                  const void* per_prev) {
     using T = typename Store<ST>::T;
-    constexpr int Q = D == 2 ? 9 : 19;
+    constexpr int Q = lq(D);
     const RegionProg& N = kp.progs[B.prog_base + op.nb_region];
     const int nx = x + op.off[0], ny = y + op.off[1], nz = z + op.off[2];
     const int64_t nm = B.mbase + ((int64_t)nz * B.n[1] + ny) * B.n[0] + nx;
@@ -431,7 +478,7 @@ __device__ __forceinline__ void finish_node(const KParams& kp, const BlockParams
                                             int z, double* acc) {
     using T = typename Store<ST>::T;
     using E = typename Store<ST>::E;
-    constexpr int Q = D == 2 ? 9 : 19;
+    constexpr int Q = lq(D);
     constexpr bool EXACT = ST == MPLBHIP_STORE_F64;
     (void)full;
     if constexpr (EXACT) {
@@ -442,7 +489,7 @@ __device__ __forceinline__ void finish_node(const KParams& kp, const BlockParams
         for (int q = 0; q < Q; q++) rho += f[q];
         double uvw[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int d = 0; d < D; d++) {
+        for (int d = 0; d < ldim(D); d++) {
             double u = 0;
 #pragma unroll
             for (int q = 0; q < Q; q++) u += kCS * double(cvel(D, q, d)) * f[q];
@@ -453,7 +500,7 @@ __device__ __forceinline__ void finish_node(const KParams& kp, const BlockParams
         if constexpr (MODE == MODE_MATERIALIZE || MODE == MODE_STEP_CHECK) {
             const double m[4] = {rho, uvw[0], uvw[1], uvw[2]};
 #pragma unroll
-            for (int v = 0; v < D + 1; v++) {
+            for (int v = 0; v < ldim(D) + 1; v++) {
                 if (kp.want_residual) {
                     const double old = kp.macro_copy[v][mnode];
                     acc[v] = (m[v] - old) * (m[v] - old);
@@ -474,6 +521,28 @@ __device__ __forceinline__ void finish_node(const KParams& kp, const BlockParams
 #pragma unroll
                 for (int q = 0; q < Q; q++) kp.fhost[h + q * nn] = f[q];
             }
+        } else if (kp.thermal) {
+            // KerCollideBGKThermal[3D] (model_kernel.inc:267-310, 457-502):
+            // tau = tauRef / (rho sqrt(T)) per node, the 4th-order
+            // equilibrium at T, and f - w (f - feq) (+ tau w F)
+            const double Tm = kp.temp[mnode];
+            const double tau = B.tau / (rho * sqrt(Tm));
+            const double om = B.dt / (tau + 0.5 * B.dt);
+            const bool addf = force_node && kp.force_type == 1;
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const double fe = feq4_ref<D>(q, rho, uvw[0], uvw[1], uvw[2], Tm);
+                double fsq = f[q] - om * (f[q] - fe);
+                if (addf) {
+                    double cf = 0;
+#pragma unroll
+                    for (int d = 0; d < ldim(D); d++)
+                        cf += kCS * double(cvel(D, q, d)) * B.g[d];
+                    const double F = wgt(D, q) * rho * cf;
+                    fsq = fsq + tau * om * F;
+                }
+                *at_u(next, (int64_t)q * B.sq, (uint32_t)node) = fsq;
+            }
         } else {
             // BGK collision with the optional first-order force source
             const double tw = B.tau_omega, om1 = B.one_minus_omega;
@@ -485,7 +554,7 @@ __device__ __forceinline__ void finish_node(const KParams& kp, const BlockParams
                 if (addf) {
                     double cf = 0;
 #pragma unroll
-                    for (int d = 0; d < D; d++)
+                    for (int d = 0; d < ldim(D); d++)
                         cf += kCS * double(cvel(D, q, d)) * B.g[d];
                     const double F = wgt(D, q) * rho * cf;
                     fsq = fe + om1 * (f[q] - fe) + tw * F;
@@ -506,7 +575,7 @@ __device__ __forceinline__ void finish_node(const KParams& kp, const BlockParams
         for (int q = 0; q < Q; q++) {
             drho += f[q];
 #pragma unroll
-            for (int d = 0; d < D; d++) {
+            for (int d = 0; d < ldim(D); d++) {
                 if (cvel(D, q, d) > 0) j[d] += f[q];
                 if (cvel(D, q, d) < 0) j[d] -= f[q];
             }
@@ -515,7 +584,7 @@ __device__ __forceinline__ void finish_node(const KParams& kp, const BlockParams
         const float inv = __builtin_amdgcn_rcpf(rho);  // v_rcp_f32, 1 ulp
         float ul[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-        for (int d = 0; d < D; d++) {
+        for (int d = 0; d < ldim(D); d++) {
             ul[d] = j[d] * inv;
             if (kp.force_macros && force_node) ul[d] += B.fhalf_dtg[d];
         }
@@ -527,7 +596,7 @@ __device__ __forceinline__ void finish_node(const KParams& kp, const BlockParams
             const double m[4] = {1.0 + dr, kCS * double(ul[0]), kCS * double(ul[1]),
                                  kCS * double(ul[2])};
 #pragma unroll
-            for (int v = 0; v < D + 1; v++) {
+            for (int v = 0; v < ldim(D) + 1; v++) {
                 if (kp.want_residual) {
                     const double old = kp.macro_copy[v][mnode];
                     acc[v] = (m[v] - old) * (m[v] - old);
@@ -577,7 +646,7 @@ __device__ __forceinline__ void finish_node(const KParams& kp, const BlockParams
                 if (p < q) continue;
                 float cu = 0.f;
 #pragma unroll
-                for (int d = 0; d < D; d++) {
+                for (int d = 0; d < ldim(D); d++) {
                     if (cvel(D, q, d) > 0) cu += ul[d];
                     if (cvel(D, q, d) < 0) cu -= ul[d];
                 }
@@ -589,7 +658,7 @@ __device__ __forceinline__ void finish_node(const KParams& kp, const BlockParams
                 if (addf) {
                     float cg = 0.f;
 #pragma unroll
-                    for (int d = 0; d < D; d++) cg += float(cvel(D, q, d)) * float(B.g[d]);
+                    for (int d = 0; d < ldim(D); d++) cg += float(cvel(D, q, d)) * float(B.g[d]);
                     const float fq = B.fforce * wq * rho * cg;
                     gq += fq;
                     gp -= fq;
@@ -612,7 +681,7 @@ template <int D, int ST, int MODE, int SCOPE>
 __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
     using T = typename Store<ST>::T;
     using E = typename Store<ST>::E;
-    constexpr int Q = D == 2 ? 9 : 19;
+    constexpr int Q = lq(D);
     constexpr bool EXACT = ST == MPLBHIP_STORE_F64;
     int b, x, y, z, wg;
     bool active;
@@ -680,7 +749,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
         const int32_t node = z * B.sz + y * B.sx + x;
         const int64_t mnode = B.mbase + ((int64_t)z * B.n[1] + y) * B.n[0] + x;
         const int r = region_code(x, B.n[0]) + 3 * region_code(y, B.n[1]) +
-                      (D == 3 ? 9 * region_code(z, B.n[2]) : 0);
+                      (D != 2 ? 9 * region_code(z, B.n[2]) : 0);
         const RegionProg* P = r != 0 ? &kp.progs[B.prog_base + r] : nullptr;
         const uint32_t om = obst_mask(kp, B, mnode);
         const bool solid = (P && P->kind == RK_SKIP) || (om & kObstSolid);
@@ -719,7 +788,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
                 if constexpr (MODE == MODE_INIT_FEQ) {
                     const double* const* m0 = kp.macro0[0] ? kp.macro0 : kp.macro;
                     fv = feq_ref<D, double>(q, m0[0][mnode], m0[1][mnode], m0[2][mnode],
-                                            D == 3 ? m0[3][mnode] : 0.0);
+                                            D != 2 ? m0[3][mnode] : 0.0);
                 } else {
                     const int64_t nn = (int64_t)B.n[0] * B.n[1] * B.n[2];
                     fv = kp.fhost[B.fbase + q * nn +
@@ -809,9 +878,9 @@ template <int D, int ST, int MODE, bool OBST, int FOLD>
 __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
     using T = typename Store<ST>::T;
     using E = typename Store<ST>::E;
-    constexpr int Q = D == 2 ? 9 : 19;
+    constexpr int Q = lq(D);
     // fold level 3 spans every z plane (3D): its grid has its own tile starts
-    constexpr bool ZALL = FOLD == 3 && D == 3;
+    constexpr bool ZALL = FOLD == 3 && D != 2;
     const int wg = kp.xcd_remap ? logical_wg(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int b = ZALL ? find_block<&BlockParams::bulk_tile_start3>(kp, wg)
                        : find_block<&BlockParams::bulk_tile_start>(kp, wg);
@@ -832,14 +901,14 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
         const int tr = t - t1 * per_row;
         const int t2 = fast_div(t1, B.div_m[2], B.div_sh[2]);
         y = t1 - t2 * B.n[1];
-        z = D == 3 ? (ZALL ? t2 : 1 + t2) : 0;
+        z = D != 2 ? (ZALL ? t2 : 1 + t2) : 0;
         x = (tr * kTY + (int)threadIdx.y) * kTX + threadIdx.x;
     } else {
         const int t1 = fast_div(t, B.div_m[0], B.div_sh[0]);
         const int tx = t - t1 * B.tiles_x;
         const int t2 = fast_div(t1, B.div_m[1], B.div_sh[1]);
         const int ty = t1 - t2 * B.tiles_y;
-        z = D == 3 ? (ZALL ? t2 : 1 + t2) : 0;
+        z = D != 2 ? (ZALL ? t2 : 1 + t2) : 0;
         x = tx * kTX + threadIdx.x;
         y = ty * kTY + threadIdx.y;
     }
@@ -966,7 +1035,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
 // fhost that MODE_INIT_F then collides.  No reference counterpart.
 template <int D>
 __global__ void k_lift(KParams kp, int64_t nodes) {
-    constexpr int Q = D == 2 ? 9 : 19;
+    constexpr int Q = lq(D);
     const double* const* m0 = kp.macro0[0] ? kp.macro0 : kp.macro;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nodes;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -987,11 +1056,11 @@ __global__ void k_lift(KParams kp, int64_t nodes) {
                           (int)(local / ((int64_t)B.n[0] * B.n[1]))};
         const int64_t stride[3] = {1, B.n[0], (int64_t)B.n[0] * B.n[1]};
         double g[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // g[a][b] = d_a u_b
-        for (int a = 0; a < D; a++) {
+        for (int a = 0; a < ldim(D); a++) {
             const bool hp = c[a] + 1 < B.n[a] &&
                             !(obst_mask(kp, B, i + stride[a]) & kObstSolid);
             const bool hm = c[a] > 0 && !(obst_mask(kp, B, i - stride[a]) & kObstSolid);
-            for (int b = 0; b < D; b++) {
+            for (int b = 0; b < ldim(D); b++) {
                 const double* u = m0[1 + b];
                 const double u0 = u[i];
                 double d = 0.0;
@@ -1006,11 +1075,11 @@ __global__ void k_lift(KParams kp, int64_t nodes) {
         const double div = g[0][0] + g[1][1] + g[2][2];
         for (int q = 0; q < Q; q++) {
             double s = 0.0;
-            for (int a = 0; a < D; a++)
-                for (int b = 0; b < D; b++) s += cvel(D, q, a) * cvel(D, q, b) * g[a][b];
+            for (int a = 0; a < ldim(D); a++)
+                for (int b = 0; b < ldim(D); b++) s += cvel(D, q, a) * cvel(D, q, b) * g[a][b];
             const double f1 = -3.0 * tl * wgt(D, q) * rho * (s - div / 3.0);
             out[q * nn] = feq_ref<D, double>(q, rho, m0[1][i], m0[2][i],
-                                             D == 3 ? m0[3][i] : 0.0) + f1;
+                                             D != 2 ? m0[3][i] : 0.0) + f1;
         }
     }
 }
@@ -1030,13 +1099,13 @@ hipError_t launch_lift(int dim, const KParams& kp, int64_t nodes, hipStream_t s)
 template <int D>
 __device__ __forceinline__ void collide_dev(const KParams& kp, const BlockParams& B,
                                             const float* g, float* h) {
-    constexpr int Q = D == 2 ? 9 : 19;
+    constexpr int Q = lq(D);
     float drho = 0.f, j[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         drho += g[q];
 #pragma unroll
-        for (int d = 0; d < D; d++) {
+        for (int d = 0; d < ldim(D); d++) {
             if (cvel(D, q, d) > 0) j[d] += g[q];
             if (cvel(D, q, d) < 0) j[d] -= g[q];
         }
@@ -1045,7 +1114,7 @@ __device__ __forceinline__ void collide_dev(const KParams& kp, const BlockParams
     const float inv = __builtin_amdgcn_rcpf(rho);  // v_rcp_f32, 1 ulp
     float ul[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int d = 0; d < D; d++) {
+    for (int d = 0; d < ldim(D); d++) {
         ul[d] = j[d] * inv;
         if (kp.force_macros) ul[d] += B.fhalf_dtg[d];
     }
@@ -1056,7 +1125,7 @@ __device__ __forceinline__ void collide_dev(const KParams& kp, const BlockParams
     for (int q = 0; q < Q; q++) {
         float cu = 0.f;
 #pragma unroll
-        for (int d = 0; d < D; d++) {
+        for (int d = 0; d < ldim(D); d++) {
             if (cvel(D, q, d) > 0) cu += ul[d];
             if (cvel(D, q, d) < 0) cu -= ul[d];
         }
@@ -1066,7 +1135,7 @@ __device__ __forceinline__ void collide_dev(const KParams& kp, const BlockParams
         if (addf) {
             float cg = 0.f;
 #pragma unroll
-            for (int d = 0; d < D; d++) cg += float(cvel(D, q, d)) * float(B.g[d]);
+            for (int d = 0; d < ldim(D); d++) cg += float(cvel(D, q, d)) * float(B.g[d]);
             gn += B.fforce * wq * rho * cg;
         }
         h[q] = gn * kp.inv_scale[q];
@@ -1091,7 +1160,7 @@ __device__ __forceinline__ float h_hi(uint32_t v) {
 // boundary region (fold level 0).  Rows of 128 nodes per wave.
 template <int D>
 __global__ __launch_bounds__(kTX * kTY) void k_bulk_h2(KParams kp) {
-    constexpr int Q = D == 2 ? 9 : 19;
+    constexpr int Q = lq(D);
     const int wg = (int)blockIdx.x;
     const int b = find_block<&BlockParams::h2_tile_start>(kp, wg);
     const BlockParams B = load_block(kp, b);
@@ -1100,7 +1169,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk_h2(KParams kp) {
     const int tx = t % B.tiles_x2;
     t /= B.tiles_x2;
     const int ty = t % B.tiles_y;
-    const int z = D == 3 ? 1 + t / B.tiles_y : 0;
+    const int z = D != 2 ? 1 + t / B.tiles_y : 0;
     const int lane = threadIdx.x;
     const int x0 = tx * 2 * kTX + 2 * lane;
     const int y = ty * kTY + threadIdx.y;
@@ -1123,7 +1192,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk_h2(KParams kp) {
     for (int q = 0; q < Q; q++) {
         const int cx = cvel(D, q, 0);
         // the aligned pair of the source row, before the x shift
-        const int32_t dyz = cvel(D, q, 1) * B.sx + (D == 3 ? cvel(D, q, 2) * B.sz : 0);
+        const int32_t dyz = cvel(D, q, 1) * B.sx + (D != 2 ? cvel(D, q, 2) * B.sz : 0);
         const __half* row = prev + (int64_t)q * B.sq - dyz;
         const uint32_t P = ld_u32(row + node);
         float lo, hi;
@@ -1312,7 +1381,7 @@ static void launch_step(const KParams& kp, int part, hipStream_t s) {
     if (part != 2 && kp.nbulk > 0) {
         const dim3 g(kp.nbulk);
         if (kp.obst) {
-            if (kp.fold == 3 && D == 3)
+            if (kp.fold == 3 && D != 2)
                 hipLaunchKernelGGL((k_bulk<D, ST, MODE, true, 3>), g, blk, 0, s, kp);
             else if (kp.fold >= 2)
                 hipLaunchKernelGGL((k_bulk<D, ST, MODE, true, 2>), g, blk, 0, s, kp);
@@ -1320,7 +1389,7 @@ static void launch_step(const KParams& kp, int part, hipStream_t s) {
                 hipLaunchKernelGGL((k_bulk<D, ST, MODE, true, 1>), g, blk, 0, s, kp);
             else hipLaunchKernelGGL((k_bulk<D, ST, MODE, true, 0>), g, blk, 0, s, kp);
         } else {
-            if (kp.fold == 3 && D == 3)
+            if (kp.fold == 3 && D != 2)
                 hipLaunchKernelGGL((k_bulk<D, ST, MODE, false, 3>), g, blk, 0, s, kp);
             else if (kp.fold >= 2)
                 hipLaunchKernelGGL((k_bulk<D, ST, MODE, false, 2>), g, blk, 0, s, kp);
@@ -1358,8 +1427,47 @@ static hipError_t launch_mode(int mode, int part, const KParams& kp, hipStream_t
     return hipGetLastError();
 }
 
+// lattices that run every step as one whole-lattice launch (D3Q15): only
+// the SCOPE_ALL kernels are instantiated for them
+template <int D, int ST>
+static hipError_t launch_mode_fused(int mode, int part, const KParams& kp, hipStream_t s) {
+    const dim3 blk(kTX, kTY, 1), grd(kp.ntiles, 1, 1);
+    if (mode == MODE_STEP || mode == MODE_STEP_CHECK) {
+        if (!kp.fused) return hipErrorInvalidValue;
+        if (part == 2) return hipSuccess;  // the one launch is the "bulk" part
+    }
+    switch (mode) {
+        case MODE_STEP:
+            hipLaunchKernelGGL((k_lattice<D, ST, MODE_STEP, SCOPE_ALL>), grd, blk, 0, s, kp);
+            break;
+        case MODE_STEP_CHECK:
+            hipLaunchKernelGGL((k_lattice<D, ST, MODE_STEP_CHECK, SCOPE_ALL>), grd, blk, 0, s, kp);
+            break;
+        case MODE_MATERIALIZE:
+            hipLaunchKernelGGL((k_lattice<D, ST, MODE_MATERIALIZE, SCOPE_ALL>), grd, blk, 0, s, kp);
+            break;
+        case MODE_INIT_FEQ:
+            hipLaunchKernelGGL((k_lattice<D, ST, MODE_INIT_FEQ, SCOPE_ALL>), grd, blk, 0, s, kp);
+            break;
+        case MODE_INIT_F:
+            hipLaunchKernelGGL((k_lattice<D, ST, MODE_INIT_F, SCOPE_ALL>), grd, blk, 0, s, kp);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// dim: the lattice key, 2 (D2Q9), 3 (D3Q19) or 15 (D3Q15)
 hipError_t launch_lattice(int dim, int storage, int mode, const KParams& kp, hipStream_t s,
                           int part) {
+    if (dim == 15) {
+        if (storage == MPLBHIP_STORE_F64)
+            return launch_mode_fused<15, MPLBHIP_STORE_F64>(mode, part, kp, s);
+        if (storage == MPLBHIP_STORE_F32)
+            return launch_mode_fused<15, MPLBHIP_STORE_F32>(mode, part, kp, s);
+        return launch_mode_fused<15, MPLBHIP_STORE_F16>(mode, part, kp, s);
+    }
     if (dim == 2) {
         if (storage == MPLBHIP_STORE_F64) return launch_mode<2, MPLBHIP_STORE_F64>(mode, part, kp, s);
         if (storage == MPLBHIP_STORE_F32) return launch_mode<2, MPLBHIP_STORE_F32>(mode, part, kp, s);

@@ -154,6 +154,11 @@ struct KParams {
     int32_t nbulk2;         // its workgroups
     int32_t nbulk;          // bulk workgroups (partials 0 .. nbulk-1)
     int32_t nbnd;           // boundary workgroups (partials nbulk ..)
+    // Collision_BGKThermal4th (FP64 storage): per-node relaxation time and
+    // the 4th-order equilibrium at the temperature temp (host layout, as
+    // the macros)
+    int32_t thermal;
+    const double* temp;
 };
 
 struct HaloParams {

@@ -53,8 +53,13 @@ struct DevBuf {
 struct mplbhip_ctx {
     int device = -1;
     int dim = 3;
+    int lat = 3;  // lattice key of the kernels: 2 D2Q9, 3 D3Q19, 15 D3Q15
     int storage = MPLBHIP_STORE_F64;
     int Q = 19;
+    // Collision_BGKThermal4th: per-node relaxation and the 4th-order
+    // equilibrium at the Variable_T field temp (host layout, FP64)
+    bool thermal = false;
+    bool temp_set = false;
     Plan plan;
     double tau = 1.0, dt = 1.0;
     int force_type = 0, force_macros = 0;
@@ -123,6 +128,7 @@ struct mplbhip_ctx {
     // at the first step (the collision needs the time step, which the Apps
     // set afterwards); a macro refresh at t = 0 must not change it
     DevBuf macro0[4];
+    DevBuf temp;  // Variable_T of the thermal collision
     std::vector<HaloLink> halos;
     std::vector<double> host_init_f;  // f given by mplbhip_set_f, per block
     std::vector<float> scale, inv_scale;
@@ -262,6 +268,8 @@ static KParams base_params(mplbhip_ctx* c) {
     }
     kp.force_type = c->force_type;
     kp.force_macros = c->force_macros;
+    kp.thermal = c->thermal ? 1 : 0;
+    kp.temp = static_cast<const double*>(c->temp.p);
     for (int i = 0; i < 2; i++) {
         kp.fs_buf[i] = c->fs[i].p;
         kp.per_buf[i] = c->per[i].p;
@@ -306,24 +314,24 @@ static int launch_step_kernels(mplbhip_ctx* c, const KParams& kp, int mode,
                                hipEvent_t* ev = nullptr) {
     if (!use_side(c, kp)) {
         if (!ev) {
-            HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->stream));
+            HIPCHK(c, launch_lattice(c->lat, c->storage, mode, kp, c->stream));
             return 0;
         }
         HIPCHK(c, hipEventRecord(ev[0], c->stream));
-        HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->stream, 1));
+        HIPCHK(c, launch_lattice(c->lat, c->storage, mode, kp, c->stream, 1));
         HIPCHK(c, hipEventRecord(ev[1], c->stream));
         HIPCHK(c, hipEventRecord(ev[2], c->stream));
-        HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->stream, 2));
+        HIPCHK(c, launch_lattice(c->lat, c->storage, mode, kp, c->stream, 2));
         HIPCHK(c, hipEventRecord(ev[3], c->stream));
         return 0;
     }
     HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
     HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
     if (ev) HIPCHK(c, hipEventRecord(ev[2], c->side));
-    HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->side, 2));
+    HIPCHK(c, launch_lattice(c->lat, c->storage, mode, kp, c->side, 2));
     if (ev) HIPCHK(c, hipEventRecord(ev[3], c->side));
     if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
-    HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->stream, 1));
+    HIPCHK(c, launch_lattice(c->lat, c->storage, mode, kp, c->stream, 1));
     if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
     HIPCHK(c, hipEventRecord(c->ev_join, c->side));
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
@@ -442,7 +450,7 @@ static int materialize(mplbhip_ctx* c, bool want_f, bool residual,
         } else {
             KParams ki = base_params(c);
             ki.fhost = static_cast<double*>(c->fhost.p);
-            HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_INIT_FEQ, ki, c->stream));
+            HIPCHK(c, launch_lattice(c->lat, c->storage, MODE_INIT_FEQ, ki, c->stream));
         }
         kp.fhost = static_cast<double*>(c->fhost.p);
         kp.fhost_in = 1;
@@ -450,7 +458,7 @@ static int materialize(mplbhip_ctx* c, bool want_f, bool residual,
         kp.fhost = want_f ? static_cast<double*>(c->fhost.p) : nullptr;
     }
     kp.want_residual = residual ? 1 : 0;
-    HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_MATERIALIZE, kp, c->stream));
+    HIPCHK(c, launch_lattice(c->lat, c->storage, MODE_MATERIALIZE, kp, c->stream));
     if (acc_out) {
         HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
                                 static_cast<const BlockParams*>(c->blocks.p),
@@ -472,6 +480,26 @@ const char* mplbhip_version(void) {
     return "mplbhip 0.1 gfx950";
 }
 
+// the lattice of a context (key 2 D2Q9, 3 D3Q19, 15 D3Q15) and its FP16
+// population scales
+static void set_lattice(mplbhip_ctx* c, int lat) {
+    const LatticeTab L = lattice_tab(lat);
+    c->lat = lat;
+    c->plan.lat = lat;
+    c->Q = L.Q;
+    c->scale.assign(c->Q, 1.f);
+    c->inv_scale.assign(c->Q, 1.f);
+    if (c->storage == MPLBHIP_STORE_F16) {
+        // per-population scale: g/w_i is O(1e-1) for |u| ~ 0.05, so
+        // s_i = w_i/2^10 keeps h = g/s_i in the normal FP16 range with ~11
+        // significant bits
+        for (int q = 0; q < c->Q; q++) {
+            c->scale[q] = (float)(L.W[q] / 1024.0);
+            c->inv_scale[q] = (float)(1024.0 / L.W[q]);
+        }
+    }
+}
+
 int mplbhip_create(mplbhip_ctx** out, int device, int space_dim, int storage) {
     if (!out) return MPLBHIP_ERR_ARG;
     *out = nullptr;
@@ -482,21 +510,9 @@ int mplbhip_create(mplbhip_ctx** out, int device, int space_dim, int storage) {
     c->device = device;
     c->dim = space_dim;
     c->storage = storage;
-    c->Q = space_dim == 2 ? 9 : 19;
     c->plan.dim = space_dim;
     c->es = storage == MPLBHIP_STORE_F64 ? 8 : (storage == MPLBHIP_STORE_F32 ? 4 : 2);
-    c->scale.assign(c->Q, 1.f);
-    c->inv_scale.assign(c->Q, 1.f);
-    if (storage == MPLBHIP_STORE_F16) {
-        // per-population scale: g/w_i is O(1e-1) for |u| ~ 0.05, so
-        // s_i = w_i/2^10 keeps h = g/s_i in the normal FP16 range with ~11
-        // significant bits
-        for (int q = 0; q < c->Q; q++) {
-            const double w = space_dim == 2 ? kD2Q9W[q] : kD3Q19W[q];
-            c->scale[q] = (float)(w / 1024.0);
-            c->inv_scale[q] = (float)(1024.0 / w);
-        }
-    }
+    set_lattice(c, space_dim);
     if (device >= 0) {
         int n = 0;
         if (hipGetDeviceCount(&n) != hipSuccess || device >= n) {
@@ -635,6 +651,60 @@ int mplbhip_define_scheme(mplbhip_ctx* c, int scheme) {
                     "scheme_kernel.inc:957-1398) with FP64 storage");
     c->plan.scheme = scheme;
     c->swap = scheme == SCHEME_Swap;
+    return 0;
+}
+
+int mplbhip_define_lattice(mplbhip_ctx* c, int q) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    if (c->prepared) return fail(c, MPLBHIP_ERR_ARG, "the lattice is defined before prepare");
+    int lat = 0;
+    if (c->dim == 2 && q == 9) lat = 2;
+    if (c->dim == 3 && q == 19) lat = 3;
+    if (c->dim == 3 && q == 15) lat = 15;
+    if (!lat)
+        return fail(c, MPLBHIP_ERR_UNSUPPORTED,
+                    "lattice with %d velocities in %dD: d2q9, d3q19 and d3q15 are defined "
+                    "(model.cpp:99-150)",
+                    q, c->dim);
+    set_lattice(c, lat);
+    return 0;
+}
+
+int mplbhip_define_collision_type(mplbhip_ctx* c, int type) {
+    if (!c) return MPLBHIP_ERR_ARG;
+    if (c->prepared) return fail(c, MPLBHIP_ERR_ARG, "the collision type is defined before prepare");
+    if (type != 0 && type != 1)
+        return fail(c, MPLBHIP_ERR_UNSUPPORTED,
+                    "collision type %d: Collision_BGKIsothermal2nd (0) and "
+                    "Collision_BGKThermal4th (1)",
+                    type);
+    if (type == 1 && c->storage != MPLBHIP_STORE_F64)
+        return fail(c, MPLBHIP_ERR_UNSUPPORTED,
+                    "Collision_BGKThermal4th runs with FP64 storage (its equilibrium is "
+                    "restated in the reference's FP64 arithmetic)");
+    c->thermal = type == 1;
+    return 0;
+}
+
+int mplbhip_set_temperature(mplbhip_ctx* c, int block, const double* T) {
+    if (!c || !T) return MPLBHIP_ERR_ARG;
+    if (int rc = need_device(c)) return rc;
+    if (!c->prepared) return fail(c, MPLBHIP_ERR_ARG, "call mplbhip_prepare first");
+    if (!c->thermal)
+        return fail(c, MPLBHIP_ERR_ARG, "the temperature is read by Collision_BGKThermal4th only");
+    if (block < 0 || block >= (int)c->bp.size())
+        return fail(c, MPLBHIP_ERR_ARG, "undefined block");
+    const BlockParams& B = c->bp[block];
+    const size_t nn = (size_t)B.n[0] * B.n[1] * B.n[2];
+    for (size_t i = 0; i < nn; i++)
+        if (!(T[i] > 0.0))
+            return fail(c, MPLBHIP_ERR_ARG, "temperature %g at node %zu of block %d", T[i], i,
+                        block);
+    drop_graphs(c);
+    HIPCHK(c, hipMemcpyAsync(static_cast<double*>(c->temp.p) + B.mbase, T, nn * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->temp_set = true;
     return 0;
 }
 
@@ -808,7 +878,7 @@ static int setup_halos(mplbhip_ctx* c) {
             for (int j = 0; j < 3; j++)
                 if (a != m.af && m.R[j][a] != 0) h.hp.dstep[a] = m.R[j][a] * stride[j];
         }
-        const int (*cv)[3] = c->dim == 3 ? kD3Q19 : kD2Q9;
+        const int (*cv)[3] = lattice_tab(c->lat).C;
         for (int q = 0; q < c->Q; q++) {
             int v[3] = {0, 0, 0};
             for (int j = 0; j < 3; j++)
@@ -890,7 +960,7 @@ static int build_obstacles(mplbhip_ctx* c) {
                 }
     }
     c->obst.assign((size_t)c->interior_nodes, 0u);
-    const int (*cv)[3] = c->dim == 3 ? kD3Q19 : kD2Q9;
+    const int (*cv)[3] = lattice_tab(c->lat).C;
     for (int b = 0; b < nb; b++) {
         const int* n = c->plan.blocks[b].n;
         bool has = false;
@@ -993,6 +1063,10 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     if (!c) return MPLBHIP_ERR_ARG;
     if (c->prepared) return fail(c, MPLBHIP_ERR_ARG, "case already prepared");
     if (c->plan.blocks.empty()) return fail(c, MPLBHIP_ERR_ARG, "no blocks defined");
+    if (c->swap && (c->lat != 3 || c->thermal))
+        return fail(c, MPLBHIP_ERR_UNSUPPORTED,
+                    "the swap scheme has D3Q19 Collision_BGKIsothermal2nd_Swap kernels only "
+                    "(scheme_kernel.inc:957-1398)");
     if (!c->plan.build()) return fail(c, MPLBHIP_ERR_UNSUPPORTED, "%s", c->plan.error.c_str());
     // HBM layout: one halo node per axis, x = 0 and every row aligned to a
     // 128-byte cache line, so a wave's 64-node row segment spans whole lines
@@ -1041,7 +1115,7 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         B.base = off + hz * sz + sx + ox;
         off += (elems + 255) / 256 * 256;
         {
-            const int (*cv)[3] = c->dim == 3 ? kD3Q19 : kD2Q9;
+            const int (*cv)[3] = lattice_tab(c->lat).C;
             const int64_t shift = (c->dim == 3 ? sz : 0) + sx + 1;
             B.pull_shift = (int32_t)shift;
             for (int q = 0; q < c->Q; q++) {
@@ -1119,6 +1193,11 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         c->fused = std::atoi(e) != 0;
         c->fused_env = true;
     }
+    if (c->lat == 15) {
+        // D3Q15 has whole-lattice kernels only: every step is one launch
+        c->fused = true;
+        c->fused_env = true;
+    }
     c->fused_full = c->fused;
     if (const char* e = std::getenv("MPLBHIP_BATCH_COMPACT")) c->batch_compact = std::atoi(e) != 0;
     if (const char* e = std::getenv("MPLBHIP_XGROUP")) {
@@ -1136,7 +1215,7 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     if (c->swap && !c->obst.empty())
         return fail(c, MPLBHIP_ERR_UNSUPPORTED, "obstacles are not supported by the swap scheme");
     // the two-node FP16 kernel has no obstacle path
-    c->h2_ok = c->storage == MPLBHIP_STORE_F16 && c->obst.empty();
+    c->h2_ok = c->storage == MPLBHIP_STORE_F16 && c->obst.empty() && c->lat != 15;
     c->h2 = c->h2_ok;
     if (const char* e = std::getenv("MPLBHIP_H2")) {
         c->h2 = c->h2_ok && std::atoi(e) != 0;
@@ -1162,6 +1241,8 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         HIPCHK(c, launch_fill(c->macro[v].p, (int64_t)mb, c->stream));
         HIPCHK(c, launch_fill(c->macro_copy[v].p, (int64_t)mb, c->stream));
     }
+    if (c->thermal)
+        if (int rc = dalloc(c, c->temp, (size_t)c->interior_nodes * sizeof(double))) return rc;
     if (int rc = dalloc(c, c->progs, c->plan.progs.size() * sizeof(RegionProg))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->progs.p, c->plan.progs.data(), c->progs.bytes,
                              hipMemcpyHostToDevice, c->stream));
@@ -1265,6 +1346,9 @@ int mplbhip_init_feq(mplbhip_ctx* c) {
 int mplbhip_init_lift(mplbhip_ctx* c) {
     if (c && c->swap)
         return fail(c, MPLBHIP_ERR_UNSUPPORTED, "the swap scheme starts from feq or a given f");
+    if (c && (c->lat == 15 || c->thermal))
+        return fail(c, MPLBHIP_ERR_UNSUPPORTED,
+                    "the lifted start is built for D2Q9/D3Q19 with the isothermal collision");
     if (int rc = mplbhip_init_feq(c)) return rc;
     c->init_lift = true;
     return 0;
@@ -1322,7 +1406,7 @@ int mplbhip_set_accel(mplbhip_ctx* c, int block, const double* g) {
 static int autotune(mplbhip_ctx* c) {
     if (c->tuned) return 0;
     c->tuned = true;
-    if (c->interior_nodes < (1 << 20)) return 0;
+    if (c->interior_nodes < (1 << 20) || c->lat == 15) return 0;
     std::vector<std::array<int, 3>> cand;  // fold, xgroup, h2
     std::vector<int> folds;
     if (c->fold_env) {
@@ -1412,6 +1496,9 @@ static int swap_form(mplbhip_ctx* c) {
 // the collision of f^0 happens here, with the time step then in force
 static int first_collision(mplbhip_ctx* c) {
     if (c->swap) return swap_form(c);
+    if (c->thermal && !c->temp_set)
+        return fail(c, MPLBHIP_ERR_ARG,
+                    "Collision_BGKThermal4th reads Variable_T: call mplbhip_set_temperature");
     // every block starts from buffer 0
     for (int i = 0; i < 2; i++)
         HIPCHK(c, launch_fill(c->parity[i].p, (int64_t)c->parity[i].bytes, c->stream));
@@ -1434,7 +1521,7 @@ static int first_collision(mplbhip_ctx* c) {
         HIPCHK(c, launch_lift(c->dim, kp, c->interior_nodes, c->stream));
         mode = MODE_INIT_F;
     }
-    HIPCHK(c, launch_lattice(c->dim, c->storage, mode, kp, c->stream));
+    HIPCHK(c, launch_lattice(c->lat, c->storage, mode, kp, c->stream));
     if (int rc = exchange(c, 0)) return rc;
     return autotune(c);
 }
@@ -1469,7 +1556,7 @@ static int run_steps(mplbhip_ctx* c, int nsteps, bool timed, double* ms_total,
             if (kp.fused) {
                 // one whole-lattice launch: it is the bulk share
                 HIPCHK(c, hipEventRecord(evs[4 * s], c->stream));
-                HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_STEP, kp, c->stream));
+                HIPCHK(c, launch_lattice(c->lat, c->storage, MODE_STEP, kp, c->stream));
                 HIPCHK(c, hipEventRecord(evs[4 * s + 1], c->stream));
                 HIPCHK(c, hipEventRecord(evs[4 * s + 2], c->stream));
                 HIPCHK(c, hipEventRecord(evs[4 * s + 3], c->stream));
@@ -1634,7 +1721,7 @@ static int restart_at_rest(mplbhip_ctx* c, const std::vector<int>& blocks) {
     HIPCHK(c, hipMemcpyAsync(c->blocks.p, c->restart_bp.data(),
                              c->restart_bp.size() * sizeof(BlockParams), hipMemcpyHostToDevice,
                              c->stream));
-    HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_INIT_FEQ, kp, c->stream));
+    HIPCHK(c, launch_lattice(c->lat, c->storage, MODE_INIT_FEQ, kp, c->stream));
     // the host copy must outlive the upload
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
@@ -1954,7 +2041,7 @@ int mplbhip_mass_noalloc(mplbhip_ctx* c, double* mass) {
     kp.fhost = nullptr;
     kp.want_residual = 0;
     if (c->swap) HIPCHK(c, launch_swap(5, kp, 0, c->stream));
-    else HIPCHK(c, launch_lattice(c->dim, c->storage, MODE_MATERIALIZE, kp, c->stream));
+    else HIPCHK(c, launch_lattice(c->lat, c->storage, MODE_MATERIALIZE, kp, c->stream));
     double* out = c->scratch + (size_t)c->ntiles * kNAcc;
     HIPCHK(c, launch_reduce(c->scratch, static_cast<const BlockParams*>(c->blocks.p),
                             (int)c->bp.size(), c->ntiles, c->nbulk, 0, 0, 0, out, c->stream));

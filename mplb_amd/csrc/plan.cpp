@@ -26,6 +26,22 @@ const double kD3Q19W[19] = {
     1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
 const int kD3Q19Opp[19] = {0,  10, 11, 12, 13, 14, 15, 16, 17, 18,
                            1,  2,  3,  4,  5,  6,  7,  8,  9};
+// Src/model.cpp:133-150 (weights 2/9, 1/9, 1/72); opposites by the same
+// first-match search as FindReverseXi
+const int kD3Q15[15][3] = {
+    {0, 0, 0},   {-1, 0, 0},  {0, -1, 0}, {0, 0, -1}, {-1, -1, -1},
+    {-1, -1, 1}, {-1, 1, -1}, {-1, 1, 1}, {1, 0, 0},  {0, 1, 0},
+    {0, 0, 1},   {1, 1, 1},   {1, 1, -1}, {1, -1, 1}, {1, -1, -1}};
+const double kD3Q15W[15] = {2.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,  1.0 / 72.0,
+                            1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0, 1.0 / 9.0,  1.0 / 9.0,
+                            1.0 / 9.0,  1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0};
+const int kD3Q15Opp[15] = {0, 8, 9, 10, 11, 12, 13, 14, 1, 2, 3, 4, 5, 6, 7};
+
+LatticeTab lattice_tab(int lat) {
+    if (lat == 2) return LatticeTab{9, kD2Q9, kD2Q9W, kD2Q9Opp};
+    if (lat == 15) return LatticeTab{15, kD3Q15, kD3Q15W, kD3Q15Opp};
+    return LatticeTab{19, kD3Q19, kD3Q19W, kD3Q19Opp};
+}
 
 // Block::RangeStart/RangeEnd (block.cpp:14-97), including the "OP_3D"
 // misspelling at block.cpp:35 that leaves TopBack/TopFront on the full y
@@ -259,9 +275,10 @@ static void region_rep(int dim, const int n[3], int r, int p[3]) {
 }
 
 bool Plan::build() {
-    const int (*C)[3] = dim == 2 ? kD2Q9 : kD3Q19;
-    const int* OPP = dim == 2 ? kD2Q9Opp : kD3Q19Opp;
-    Q = dim == 2 ? 9 : 19;
+    const LatticeTab L = lattice_tab(lat ? lat : dim);
+    const int (*C)[3] = L.C;
+    const int* OPP = L.OPP;
+    Q = L.Q;
     nregions = dim == 2 ? 9 : 27;
     const int nb = (int)blocks.size();
     const uint32_t all = (Q == 32) ? 0xffffffffu : ((1u << Q) - 1u);

@@ -102,6 +102,7 @@ struct BlockDef {
 
 struct Plan {
     int dim = 0, Q = 0;
+    int lat = 0;  // lattice key: 2 D2Q9, 3 D3Q19, 15 D3Q15 (0: by dim)
     int scheme = SCHEME_StreamCollision;
     std::vector<BlockDef> blocks;
     std::vector<LinkDesc> links;
@@ -128,6 +129,19 @@ extern const int kD3Q19[19][3];
 extern const double kD3Q19W[19];
 extern const int kD2Q9Opp[9];
 extern const int kD3Q19Opp[19];
+// D3Q15, Src/model.cpp:133-150
+extern const int kD3Q15[15][3];
+extern const double kD3Q15W[15];
+extern const int kD3Q15Opp[15];
+
+// the tables of a lattice key (2 D2Q9, 3 D3Q19, 15 D3Q15)
+struct LatticeTab {
+    int Q;
+    const int (*C)[3];
+    const double* W;
+    const int* OPP;
+};
+LatticeTab lattice_tab(int lat);
 
 // helpers exposed for tests of the plan
 void surface_range(int dim, const int n[3], int surface, int r[6]);

@@ -2,10 +2,12 @@
 // the same call sequence as the reference Apps' simulate(config), on one
 // GPU.  Build with DIM=2 or DIM=3.  Initial state: rest, density 1.
 //   case_driver Config=<file.json> [Dump=<file>] [Force=<g0>] [ForceSchedule=<g0>]
+//               [Temperature=<T0>]
 // Force gives the body-force hook a constant acceleration g0 along x;
 // ForceSchedule a time-dependent one, g0 (1 + floor(k / 7) mod 3) at step k
 // (time k dt); either is written into g_MacroBodyforce as an App's hook
-// would.
+// would.  Temperature gives a Variable_T macro, when the case defines one,
+// the uniform value T0 (default 1).
 #include <cmath>
 #include <iostream>
 #include <string>
@@ -29,12 +31,28 @@ void KerRestInitial(ACC<Real>& rho, ACC<Real>& u, ACC<Real>& v) {
 }
 #endif
 
+static double g_temperature = 1.0;
+
+#ifdef OPS_3D
+void KerSetTemperature(ACC<Real>& T, const Real* val) { T(0, 0, 0) = val[0]; }
+#else
+void KerSetTemperature(ACC<Real>& T, const Real* val) { T(0, 0) = val[0]; }
+#endif
+
 void SetInitialMacrosVars() {
     for (auto idBlock : g_Block()) {
         Block& block{idBlock.second};
         std::vector<int> rng(block.WholeRange().begin(), block.WholeRange().end());
         for (auto& idCompo : g_Components()) {
             const Component& c{idCompo.second};
+            // Variable_T (read by Collision_BGKThermal4th): Temperature=<T0>
+            if (c.macroVars.count(Variable_T))
+                ops_par_loop(KerSetTemperature, "KerSetTemperature", block.Get(), SpaceDim(),
+                             rng.data(),
+                             ops_arg_dat(g_MacroVars().at(c.macroVars.at(Variable_T).id)
+                                             .at(block.ID()),
+                                         1, LOCALSTENCIL, "double", OPS_RW),
+                             ops_arg_gbl(&g_temperature, 1, "double", OPS_READ));
             const int rhoId{c.macroVars.at(Variable_Rho).id};
             ops_par_loop(KerRestInitial, "KerRestInitial", block.Get(), SpaceDim(), rng.data(),
                          ops_arg_dat(g_MacroVars().at(rhoId).at(block.ID()), 1, LOCALSTENCIL,
@@ -139,6 +157,7 @@ int main(int argc, const char** argv) {
     ops_timers(&c0, &e0);
     ReadConfiguration(cfg);
     g_force_amp = OptD(o, "Force", 0.0);
+    g_temperature = OptD(o, "Temperature", 1.0);
     if (o.count("ForceSchedule")) {
         g_force_amp = OptD(o, "ForceSchedule", 0.0);
         g_force_schedule = true;

@@ -432,6 +432,8 @@ void TransferHalos() {}  // inside the device step
 
 void Partition() { PrepareFlowField(); }
 
+static void upload_temperature();
+
 void PrepareFlowField() {
     State& s = S();
     if (!s.ctx) die("Error! please call DefineComponents before Partition");
@@ -454,9 +456,15 @@ void PrepareFlowField() {
             die("Error! the swap scheme needs BodyForce_None_Swap or BodyForce_1st_Swap");
         ck(mplbhip_define_scheme(s.ctx, (int)Scheme_StreamCollision_Swap), "DefineScheme");
     } else {
-        if (c.collisionType != Collision_BGKIsothermal2nd)
-            die("Error! only Collision_BGKIsothermal2nd is implemented for "
-                "Scheme_StreamCollision");
+        if (c.collisionType != Collision_BGKIsothermal2nd &&
+            c.collisionType != Collision_BGKThermal4th)
+            die("Error! only Collision_BGKIsothermal2nd and Collision_BGKThermal4th are "
+                "implemented for Scheme_StreamCollision");
+        if (c.collisionType == Collision_BGKThermal4th) {
+            if (!c.macroVars.count(Variable_T))
+                die("Error! Collision_BGKThermal4th reads the macro variable Variable_T");
+            ck(mplbhip_define_collision_type(s.ctx, 1), "DefineCollision (thermal)");
+        }
         if (c.bodyForceType != BodyForce_None && c.bodyForceType != BodyForce_1st)
             die("Error! only BodyForce_None and BodyForce_1st are implemented for "
                 "Scheme_StreamCollision");
@@ -526,6 +534,7 @@ void PrepareFlowField() {
                         for (int q = 0; q < NUMXI; q++) f[q * nn + n] = p[d->offset(i, j, k) + q];
             ck(mplbhip_set_f(s.ctx, b, f.data()), "restart f");
         }
+        upload_temperature();
     }
 }
 
@@ -607,23 +616,30 @@ void DefineComponents(const std::vector<std::string>& compoNames,
     if (compoNames.size() != 1)
         die("Error! this build runs one component per case (got %zu)", compoNames.size());
     const std::string latt = lattNames.at(0);
-    if ((s.spaceDim == 2 && latt != "d2q9") || (s.spaceDim == 3 && latt != "d3q19"))
-        die("Error! lattice %s is not implemented (d2q9 in 2D, d3q19 in 3D)", latt.c_str());
+    if ((s.spaceDim == 2 && latt != "d2q9") ||
+        (s.spaceDim == 3 && latt != "d3q19" && latt != "d3q15"))
+        die("Error! lattice %s is not implemented (d2q9 in 2D, d3q19 or d3q15 in 3D)",
+            latt.c_str());
+    const bool q15 = latt == "d3q15";
     Component comp;
     comp.id = compoId.at(0);
     comp.name = compoNames.at(0);
     comp.latticeName = latt;
     comp.tauRef = tauRef.at(0);
     NUMCOMPONENTS = 1;
-    NUMXI = s.spaceDim == 2 ? 9 : 19;
+    NUMXI = s.spaceDim == 2 ? 9 : (q15 ? 15 : 19);
     LATTDIM = s.spaceDim;
     CS = std::sqrt(3.0);
     comp.index[0] = 0;
     comp.index[1] = NUMXI - 1;
     s.components.emplace(comp.id, comp);
-    // host copies of the tables (model.cpp:99-131)
+    // host copies of the tables (model.cpp:99-150)
     static const int c2[9][2] = {{0, 0},  {-1, 1}, {-1, 0}, {-1, -1}, {0, -1},
                                  {1, -1}, {1, 0},  {1, 1},  {0, 1}};
+    static const int c15[15][3] = {
+        {0, 0, 0},   {-1, 0, 0},  {0, -1, 0}, {0, 0, -1}, {-1, -1, -1},
+        {-1, -1, 1}, {-1, 1, -1}, {-1, 1, 1}, {1, 0, 0},  {0, 1, 0},
+        {0, 0, 1},   {1, 1, 1},   {1, 1, -1}, {1, -1, 1}, {1, -1, -1}};
     static const int c3[19][3] = {
         {0, 0, 0},  {-1, 0, 0},  {0, -1, 0}, {0, 0, -1}, {-1, -1, 0},
         {-1, 1, 0}, {-1, 0, -1}, {-1, 0, 1}, {0, -1, -1}, {0, -1, 1},
@@ -636,9 +652,12 @@ void DefineComponents(const std::vector<std::string>& compoNames,
     WEIGHTS = new Real[NUMXI];
     OPP = new int[NUMXI];
     for (int q = 0; q < NUMXI; q++) {
-        for (int a = 0; a < LATTDIM; a++) XI[q * LATTDIM + a] = s.spaceDim == 2 ? c2[q][a] : c3[q][a];
+        for (int a = 0; a < LATTDIM; a++)
+            XI[q * LATTDIM + a] = s.spaceDim == 2 ? c2[q][a] : (q15 ? c15[q][a] : c3[q][a]);
         if (s.spaceDim == 2) {
             WEIGHTS[q] = q == 0 ? 4.0 / 9.0 : (q % 2 == 1 ? 1.0 / 36.0 : 1.0 / 9.0);
+        } else if (q15) {
+            WEIGHTS[q] = q == 0 ? 2.0 / 9.0 : ((q <= 3 || (q >= 8 && q <= 10)) ? 1.0 / 9.0 : 1.0 / 72.0);
         } else {
             WEIGHTS[q] = q == 0 ? 1.0 / 3.0 : ((q <= 3 || (q >= 10 && q <= 12)) ? 1.0 / 18.0 : 1.0 / 36.0);
         }
@@ -654,6 +673,7 @@ void DefineComponents(const std::vector<std::string>& compoNames,
     const int device = dev ? std::atoi(dev) : 0;
     ck(mplbhip_create(&s.ctx, device, s.spaceDim, storage_from_env()), "mplbhip_create");
     if (!s.ctx) die("Error! no GPU context");
+    if (q15) ck(mplbhip_define_lattice(s.ctx, 15), "DefineComponents (d3q15)");
     for (const auto& idBlock : s.blocks) {
         int size[3] = {1, 1, 1};
         for (int a = 0; a < s.spaceDim; a++) size[a] = idBlock.second.Size()[a];
@@ -732,6 +752,20 @@ void DefineInitialCondition(std::vector<InitialType> types, std::vector<int> com
         S().components.at(compoId[i]).initialType = types[i];
 }
 
+// the Variable_T field of every block, which the thermal collision reads
+// (KerCollideBGKThermal[3D] takes it as it is: nothing updates it)
+static void upload_temperature() {
+    State& s = S();
+    const Component& c = the_component();
+    if (c.collisionType != Collision_BGKThermal4th) return;
+    const int id = c.macroVars.at(Variable_T).id;
+    for (const auto& idBlock : s.blocks) {
+        const int b = idBlock.first;
+        const std::vector<double> t = interior(s.macroVars.at(id).at(b));
+        ck(mplbhip_set_temperature(s.ctx, b, t.data()), "Variable_T upload");
+    }
+}
+
 static void initial_condition() {
     State& s = S();
     const Component& c = the_component();
@@ -753,6 +787,7 @@ static void initial_condition() {
         for (int k = 0; k < 1 + s.spaceDim; k++) s.macroVars.at(ids[k]).at(b)->dirty = false;
     }
     upload_force_if_dirty();
+    upload_temperature();
     ck(mplbhip_init_feq(s.ctx), "PreDefinedInitialCondition");
 }
 

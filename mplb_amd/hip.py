@@ -58,6 +58,9 @@ def load():
         "mplbhip_define_obstacles": (I, [P, I, ctypes.c_void_p]),
         "mplbhip_define_collision": (I, [P, D, I, I]),
         "mplbhip_define_scheme": (I, [P, I]),
+        "mplbhip_define_lattice": (I, [P, I]),
+        "mplbhip_define_collision_type": (I, [P, I]),
+        "mplbhip_set_temperature": (I, [P, I, DP]),
         "mplbhip_define_boundary": (I, [P, I, I, I, DP, I, I]),
         "mplbhip_prepare": (I, [P]),
         "mplbhip_node_info": (I, [P, I, I, IP]),
@@ -112,7 +115,9 @@ class LatticeCase:
     """One lattice case (one component) on one GPU, or a plan-only case
     with device=-1 (node typing and boundary programs, no compute)."""
 
-    def __init__(self, space_dim, storage=STORE_F64, device=0):
+    def __init__(self, space_dim, storage=STORE_F64, device=0, lattice=None):
+        """lattice: the number of velocities, 9 (D2Q9), 19 (D3Q19, the 3D
+        default) or 15 (D3Q15)."""
         self.lib = load()
         self.dim = space_dim
         self.Q = 9 if space_dim == 2 else 19
@@ -124,6 +129,9 @@ class LatticeCase:
                             % (rc, device))
         self.h = h
         self.sizes = {}
+        if lattice is not None:
+            self._ck(self.lib.mplbhip_define_lattice(self.h, int(lattice)))
+            self.Q = int(lattice)
 
     def close(self):
         if getattr(self, "h", None):
@@ -164,6 +172,11 @@ class LatticeCase:
         Scheme_StreamCollision_Swap (D3Q19, FP64 storage)."""
         self._ck(self.lib.mplbhip_define_scheme(self.h, int(scheme)))
 
+    def set_collision_type(self, ctype):
+        """DefineCollision: 0 Collision_BGKIsothermal2nd (default), 1
+        Collision_BGKThermal4th (FP64 storage)."""
+        self._ck(self.lib.mplbhip_define_collision_type(self.h, int(ctype)))
+
     def set_collision(self, tau_ref, force_type=0, force_macros=False):
         self._ck(self.lib.mplbhip_define_collision(self.h, tau_ref, int(force_type),
                                                    int(bool(force_macros))))
@@ -194,6 +207,12 @@ class LatticeCase:
         arrs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), shp))
                 for a in (rho, u, v, w if w is not None else np.zeros(shp))]
         self._ck(self.lib.mplbhip_set_macros(self.h, block, *[_dp(a) for a in arrs]))
+
+    def set_temperature(self, block, T):
+        """Variable_T of one block, interior [k][j][i] (thermal collision)."""
+        T = np.ascontiguousarray(np.broadcast_to(np.asarray(T, dtype=np.float64),
+                                                 self._shape(block)))
+        self._ck(self.lib.mplbhip_set_temperature(self.h, block, _dp(T)))
 
     def init_feq(self):
         self._ck(self.lib.mplbhip_init_feq(self.h))

@@ -58,6 +58,8 @@ enum { BS_LEFT = 0, BS_RIGHT = 1, BS_TOP = 2, BS_BOTTOM = 3, BS_FRONT = 4,
 enum { SCH_EXTRAPOLPRESSURE1ST = 16, SCH_MDPERIODIC = 18,
        SCH_FDPERIODIC = 19, SCH_BOUNCEBACK = 20, SCH_EQMDIFFUSEREFL = 24,
        SCH_NONE = -1 };
+/* CollisionType (Src/model.h:94-99) */
+enum { COLL_ISOTHERMAL2ND = 0, COLL_THERMAL4TH = 1 };
 /* SchemeType (Src/scheme.h:69-72) */
 enum { SCHEME_STREAM_COLLISION = 10, SCHEME_SWAP = 11 };
 /* BndryDvType (Src/boundary_host_device.h:34-39) */
@@ -79,6 +81,7 @@ typedef struct {
     int *nt, *vg;      /* node type, geometry property */
     unsigned char* solid; /* obstacle nodes (extension), NULL when none */
     double g[3];       /* uniform acceleration */
+    double* temp;      /* Variable_T (the thermal collision reads it) */
 } OBlock;
 
 typedef struct { int from, fsurf, to, tsurf, type, rot; } OLink;
@@ -100,6 +103,7 @@ struct orc_case {
     double tau, dt;
     int force_type, force_macros;
     int scheme;            /* SCHEME_STREAM_COLLISION or SCHEME_SWAP */
+    int collision;         /* COLL_ISOTHERMAL2ND or COLL_THERMAL4TH */
     int prepared;
     char err[256];
 };
@@ -144,6 +148,29 @@ static void setup_lattice(orc_case* c, int lattice) {
             c->W[l] = t[l];
             c->OPP[l] = op[l];
         }
+    } else if (lattice == 2) {
+        /* SetupD3Q15Latt, model.cpp:133-150 */
+        const double t000 = 2 / ((double)9), t001 = 1 / ((double)9),
+                     t111 = 1 / ((double)72);
+        const double t[15] = {t000, t001, t001, t001, t111, t111, t111, t111,
+                              t001, t001, t001, t111, t111, t111, t111};
+        const int cx[15] = {0, -1, 0, 0, -1, -1, -1, -1, 1, 0, 0, 1, 1, 1, 1};
+        const int cy[15] = {0, 0, -1, 0, -1, -1, 1, 1, 0, 1, 0, 1, 1, -1, -1};
+        const int cz[15] = {0, 0, 0, -1, -1, 1, -1, 1, 0, 0, 1, 1, -1, 1, -1};
+        c->Q = 15;
+        for (int l = 0; l < 15; l++) {
+            c->XI[l * 3] = cx[l];
+            c->XI[l * 3 + 1] = cy[l];
+            c->XI[l * 3 + 2] = cz[l];
+            c->W[l] = t[l];
+        }
+        for (int i = 0; i < 15; i++)
+            for (int j = 0; j < 15; j++) {
+                int rev = 1;
+                for (int k = 0; k < 3; k++)
+                    rev = rev && (c->XI[i * 3 + k] + c->XI[j * 3 + k] == 0.0);
+                if (rev) { c->OPP[i] = j; break; }
+            }
     } else {
         const double t000 = 1 / ((double)3), t001 = 1 / ((double)18),
                      t011 = 1 / ((double)36);
@@ -217,9 +244,61 @@ static inline double feq(const orc_case* c, int l, double rho, double u,
     return c->dim == 2 ? feq2d(c, l, rho, u, v) : feq3d(c, l, rho, u, v, w);
 }
 
+/* ---- CalcBGKFeq with temperature and polyOrder, model_host_device.h:34-57
+ * (2D) and 58-83 (3D), every term and its operation order; LATTDIM is an
+ * int in the reference, so c2 - LATTDIM etc. convert it to double ------- */
+static inline double feq_t2d(const orc_case* c, int l, double rho, double u,
+                             double v, double T, int polyOrder) {
+    const double CS = c->CS;
+    const double* XI = c->XI;
+    const int LATTDIM = 2;
+    double cu = (CS * XI[l * LATTDIM] * u + CS * XI[l * LATTDIM + 1] * v);
+    double c2 = (CS * XI[l * LATTDIM] * CS * XI[l * LATTDIM] +
+                 CS * XI[l * LATTDIM + 1] * CS * XI[l * LATTDIM + 1]);
+    double cu2 = cu * cu;
+    double u2 = u * u + v * v;
+    double res = 1.0 + cu + 0.5 * (cu2 - u2 + (T - 1.0) * (c2 - LATTDIM));
+    if (polyOrder >= 3)
+        res = res + cu * (cu2 - 3.0 * u2 + 3.0 * (T - 1.0) * (c2 - LATTDIM - 2.0)) / 6.0;
+    if (polyOrder >= 4)
+        res = res + (cu2 * cu2 - 6.0 * cu2 * u2 + 3.0 * u2 * u2) / 24.0 +
+              (T - 1.0) * ((c2 - (LATTDIM + 2)) * (cu2 - u2) - 2.0 * cu2) / 4.0 +
+              (T - 1.0) * (T - 1.0) *
+                  (c2 * c2 - 2.0 * (LATTDIM + 2) * c2 + LATTDIM * (LATTDIM + 2)) / 8.0;
+    return c->W[l] * rho * res;
+}
+static inline double feq_t3d(const orc_case* c, int l, double rho, double u,
+                             double v, double w, double T, int polyOrder) {
+    const double CS = c->CS;
+    const double* XI = c->XI;
+    const int LATTDIM = 3;
+    double cu = (CS * XI[l * LATTDIM] * u + CS * XI[l * LATTDIM + 1] * v +
+                 CS * XI[l * LATTDIM + 2] * w);
+    double c2 = (CS * XI[l * LATTDIM] * CS * XI[l * LATTDIM] +
+                 CS * XI[l * LATTDIM + 1] * CS * XI[l * LATTDIM + 1] +
+                 CS * XI[l * LATTDIM + 2] * CS * XI[l * LATTDIM + 2]);
+    double cu2 = cu * cu;
+    double u2 = u * u + v * v + w * w;
+    double res = 1.0 + cu + 0.5 * (cu2 - u2 + (T - 1.0) * (c2 - LATTDIM));
+    if (polyOrder >= 3)
+        res = res + cu * (cu2 - 3.0 * u2 + 3.0 * (T - 1.0) * (c2 - LATTDIM - 2.0)) / 6.0;
+    if (polyOrder >= 4)
+        res = res + (cu2 * cu2 - 6.0 * cu2 * u2 + 3.0 * u2 * u2) / 24.0 +
+              (T - 1.0) * ((c2 - (LATTDIM + 2)) * (cu2 - u2) - 2.0 * cu2) / 4.0 +
+              (T - 1.0) * (T - 1.0) *
+                  (c2 * c2 - 2.0 * (LATTDIM + 2) * c2 + LATTDIM * (LATTDIM + 2)) / 8.0;
+    return c->W[l] * rho * res;
+}
+double orc_feq_t(orc_case* c, int l, double rho, double u, double v, double w,
+                 double T, int poly_order) {
+    return c->dim == 2 ? feq_t2d(c, l, rho, u, v, T, poly_order)
+                       : feq_t3d(c, l, rho, u, v, w, T, poly_order);
+}
+
 /* ---- creation -------------------------------------------------------- */
 orc_case* orc_create(int space_dim, int lattice) {
-    if (!((space_dim == 2 && lattice == 0) || (space_dim == 3 && lattice == 1)))
+    if (!((space_dim == 2 && lattice == 0) ||
+          (space_dim == 3 && (lattice == 1 || lattice == 2))))
         return NULL;
     orc_case* c = (orc_case*)calloc(1, sizeof(orc_case));
     c->dim = space_dim;
@@ -235,7 +314,7 @@ void orc_destroy(orc_case* c) {
     if (!c) return;
     for (int ib = 0; ib < c->nb; ib++) {
         OBlock* b = &c->b[ib];
-        free(b->f); free(b->fs); free(b->nt); free(b->vg); free(b->solid);
+        free(b->f); free(b->fs); free(b->nt); free(b->vg); free(b->solid); free(b->temp);
         for (int v = 0; v < 4; v++) { free(b->m[v]); free(b->mc[v]); }
     }
     free(c);
@@ -262,6 +341,9 @@ int orc_add_block(orc_case* c, int id, int nx, int ny, int nz) {
         b->m[v] = (double*)calloc(b->nalloc, sizeof(double));
         b->mc[v] = (double*)calloc(b->nalloc, sizeof(double));
     }
+    /* Variable_T: the thermal collision's temperature, 1 until set */
+    b->temp = (double*)malloc(b->nalloc * sizeof(double));
+    for (size_t n = 0; n < b->nalloc; n++) b->temp[n] = 1.0;
     return 0;
 }
 
@@ -371,6 +453,23 @@ int orc_set_collision(orc_case* c, double tau_ref, int force_type,
     c->tau = tau_ref;
     c->force_type = force_type;
     c->force_macros = force_macros;
+    return 0;
+}
+
+int orc_set_collision_type(orc_case* c, int type) {
+    if (type != COLL_ISOTHERMAL2ND && type != COLL_THERMAL4TH)
+        return fail(c, "collision type %d is not restated", type);
+    c->collision = type;
+    return 0;
+}
+
+int orc_set_temperature(orc_case* c, int block, const double* T) {
+    if (block < 0 || block >= c->nb) return fail(c, "undefined block");
+    OBlock* b = &c->b[block];
+    size_t p = 0;
+    for (int k = 0; k < b->n[2]; k++)
+        for (int j = 0; j < b->n[1]; j++)
+            for (int i = 0; i < b->n[0]; i++) b->temp[nidx(b, i, j, k)] = T[p++];
     return 0;
 }
 
@@ -766,8 +865,49 @@ static void body_force(orc_case* c) {
     }
 }
 
+/* KerCollideBGKThermal[3D], model_kernel.inc:267-310, 457-502: per node
+ * tau = tauRef / (rho sqrt(T)), the 4th-order equilibrium at temperature T,
+ * and the update f - w (f - feq) (+ tau w F at Fluid and MDPeriodic) */
+static void collide_thermal(orc_case* c) {
+    const int Q = c->Q;
+    const double dt = c->dt;
+    for (int ib = 0; ib < c->nb; ib++) {
+        OBlock* b = &c->b[ib];
+        OMP_FOR
+        for (int k = 0; k < b->n[2]; k++)
+            for (int j = 0; j < b->n[1]; j++)
+                for (int i = 0; i < b->n[0]; i++) {
+                    size_t n = nidx(b, i, j, k);
+                    int vt = b->nt[n];
+                    if (vt == VT_IMMERSEDSOLID) continue;
+                    const double rho = b->m[0][n], u = b->m[1][n],
+                                 v = b->m[2][n], w = b->m[3][n];
+                    const double T = b->temp[n];
+                    const int polyOrder = 4;
+                    const double tau = c->tau / (rho * sqrt(T));
+                    const double dtOvertauPlusdt = dt / (tau + 0.5 * dt);
+                    double* fs = &b->fs[n * Q];
+                    const double* f = &b->f[n * Q];
+                    for (int q = 0; q < Q; q++) {
+                        const double fe = c->dim == 2
+                                              ? feq_t2d(c, q, rho, u, v, T, polyOrder)
+                                              : feq_t3d(c, q, rho, u, v, w, T, polyOrder);
+                        if (vt == VT_FLUID || vt == VT_MDPERIODIC)
+                            fs[q] = f[q] - dtOvertauPlusdt * (f[q] - fe) +
+                                    tau * dtOvertauPlusdt * fs[q];
+                        else
+                            fs[q] = f[q] - dtOvertauPlusdt * (f[q] - fe);
+                    }
+                }
+    }
+}
+
 /* KerCollideBGKIsothermal[3D], model_kernel.inc:221-265, 410-455 */
 static void collide(orc_case* c) {
+    if (c->collision == COLL_THERMAL4TH) {
+        collide_thermal(c);
+        return;
+    }
     const int Q = c->Q;
     const double tau = c->tau, dt = c->dt;
     const double dtOvertauPlusdt = dt / (tau + 0.5 * dt);

@@ -21,7 +21,8 @@ extern "C" {
 
 typedef struct orc_case orc_case;
 
-/* lattice: 0 = D2Q9 (model.cpp:99-112), 1 = D3Q19 (model.cpp:114-131) */
+/* lattice: 0 = D2Q9 (model.cpp:99-112), 1 = D3Q19 (model.cpp:114-131),
+ * 2 = D3Q15 (model.cpp:133-150) */
 orc_case* orc_create(int space_dim, int lattice);
 void orc_destroy(orc_case* c);
 const char* orc_error(orc_case* c);
@@ -46,6 +47,15 @@ int orc_set_obstacles(orc_case* c, int block, const unsigned char* solid);
  * (model_kernel.inc:665-759), 0 the plain ones (model_kernel.inc:593-663). */
 int orc_set_collision(orc_case* c, double tau_ref, int force_type,
                       int force_macros);
+/* DefineCollision (model.cpp:406-445): 0 = Collision_BGKIsothermal2nd,
+ * 1 = Collision_BGKThermal4th (model_kernel.inc:267-310, 457-502) */
+int orc_set_collision_type(orc_case* c, int type);
+/* Variable_T of one block, interior [k][j][i] (the thermal collision reads
+ * it; nothing updates it, as in the reference); 1 until set */
+int orc_set_temperature(orc_case* c, int block, const double* T);
+/* CalcBGKFeq with temperature T and polyOrder (model_host_device.h:34-83) */
+double orc_feq_t(orc_case* c, int l, double rho, double u, double v, double w,
+                 double T, int poly_order);
 /* DefineBlockBoundary (boundary.cpp:61-130); call order is kept. */
 int orc_add_boundary(orc_case* c, int block, int surface, int scheme,
                      const double* vars, int nvars, int vertex_type);

@@ -49,6 +49,9 @@ def load():
         "orc_step": (I, [P, I]),
         "orc_apply_boundaries": (I, [P]),
         "orc_set_scheme": (I, [P, I]),
+        "orc_set_collision_type": (I, [P, I]),
+        "orc_set_temperature": (I, [P, I, DP]),
+        "orc_feq_t": (D, [P, I, D, D, D, D, D, I]),
         "orc_update_macros": (None, [P]),
         "orc_get_f": (I, [P, I, DP]),
         "orc_set_f": (I, [P, I, DP]),
@@ -76,12 +79,14 @@ class OracleCase:
     mplb_amd.LatticeCase so a test can drive either with the same code."""
 
     def __init__(self, space_dim, lattice=None):
+        """lattice: the number of velocities, 9 (D2Q9), 19 (D3Q19, the 3D
+        default) or 15 (D3Q15)."""
         self.lib = load()
         if lattice is None:
-            lattice = 0 if space_dim == 2 else 1
+            lattice = 9 if space_dim == 2 else 19
         self.dim = space_dim
-        self.Q = 9 if lattice == 0 else 19
-        self.h = self.lib.orc_create(space_dim, lattice)
+        self.Q = int(lattice)
+        self.h = self.lib.orc_create(space_dim, {9: 0, 19: 1, 15: 2}.get(self.Q, -1))
         if not self.h:
             raise ValueError("unsupported dimension/lattice pair")
         self.sizes = {}
@@ -161,6 +166,19 @@ class OracleCase:
     def set_scheme(self, scheme):
         """SchemeType: 10 StreamCollision (default), 11 StreamCollision_Swap."""
         self._ck(self.lib.orc_set_scheme(self.h, int(scheme)))
+
+    def set_collision_type(self, ctype):
+        """0 Collision_BGKIsothermal2nd, 1 Collision_BGKThermal4th"""
+        self._ck(self.lib.orc_set_collision_type(self.h, int(ctype)))
+
+    def set_temperature(self, bid, T):
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        assert T.shape == tuple(reversed(self.sizes[bid])), T.shape
+        self._ck(self.lib.orc_set_temperature(self.h, bid, _dp(T)))
+
+    def feq_t(self, q, rho, u, v, w, T, order):
+        """CalcBGKFeq at temperature T and polynomial order `order`"""
+        return self.lib.orc_feq_t(self.h, int(q), rho, u, v, w, T, int(order))
 
     def apply_boundaries(self):
         """The boundary stage of a step alone, on the current f."""

@@ -159,3 +159,29 @@ def test_launch_shape_counts_bulk_nodes(monkeypatch, fold):
         assert nx * (ny - 2) * (nz - 2) <= nodes <= nx * ny * (nz - 2)
     else:
         assert nodes == nx * ny * nz - 8
+
+
+def test_lattice_and_collision_type_arguments():
+    """define_lattice takes 9 in 2D and 19 or 15 in 3D; the thermal
+    collision needs FP64 storage; the D3Q15 plan tags nodes as the oracle
+    does."""
+    with pytest.raises(hip.MplbError, match="d3q15"):
+        hip.LatticeCase(2, device=-1, lattice=19)
+    with pytest.raises(hip.MplbError, match="d3q15"):
+        hip.LatticeCase(3, device=-1, lattice=27)
+    r = hip.LatticeCase(3, storage=hip.STORE_F16, device=-1, lattice=15)
+    with pytest.raises(hip.MplbError, match="FP64"):
+        r.set_collision_type(1)
+    with pytest.raises(hip.MplbError, match="collision type 2"):
+        r.set_collision_type(2)
+
+    class PlanOnly15(PlanOnly):
+        def __init__(self, dim):
+            hip.LatticeCase.__init__(self, dim, device=-1, lattice=15)
+
+    p = PlanOnly15(3)
+    cases.cavity3d(p, 7, noise=False)
+    o = OracleCase(3, lattice=15)
+    cases.cavity3d(o, 7, noise=False)
+    for which in (0, 1):
+        assert np.array_equal(p.node_info(0, which), o.get_int(0, which))

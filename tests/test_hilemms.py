@@ -40,7 +40,8 @@ def test_example_drivers_build():
                                       ("case_cavity3d_10_toporder.json", 3),
                                       ("case_lchannel_9.json", 3),
                                       ("case_cavity3d_swap_12.json", 3),
-                                      ("case_channel3d_fdperiodic.json", 3)])
+                                      ("case_channel3d_fdperiodic.json", 3),
+                                      ("case_cavity3d_d3q15_thermal.json", 3)])
 def test_json_reader_matches_python(name, dim):
     path = os.path.join(GOLDEN, name)
     cfg = json.load(open(path))

@@ -20,9 +20,9 @@ from oracle.oracle import OracleCase
 CS = math.sqrt(3.0)
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-def test_lattice_identities(dim):
-    c, w = lattice.lattice(dim)
+@pytest.mark.parametrize("dim,q", [(2, 9), (3, 19), (3, 15)])
+def test_lattice_identities(dim, q):
+    c, w = lattice.lattice(dim, q)
     assert abs(w.sum() - 1.0) < 1e-15
     assert np.allclose((w[:, None] * c).sum(0), 0.0, atol=1e-16)
     second = np.einsum("q,qa,qb->ab", w, c, c)
@@ -34,8 +34,10 @@ def test_lattice_identities(dim):
     assert np.allclose(fourth, iso, atol=1e-16)
     opp = lattice.opposite(c)
     assert np.all(c[opp] == -c)
-    if dim == 3:
+    if q == 19:
         assert np.all(opp[1:10] == np.arange(10, 19))
+    if q == 15:
+        assert np.all(opp[1:8] == np.arange(8, 15))
 
 
 def _random_macros(rng, shape, dim, amp=0.08):
@@ -44,14 +46,14 @@ def _random_macros(rng, shape, dim, amp=0.08):
     return rho, u
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-def test_equilibrium_moments(dim):
+@pytest.mark.parametrize("dim,q", [(2, 9), (3, 19), (3, 15)])
+def test_equilibrium_moments(dim, q):
     """Init writes f = feq(rho, u): density, momentum and the momentum flux
     rho (delta_ab + u_a u_b) in code units (CS^2/3 = 1) must come back."""
     rng = np.random.default_rng(1)
     size = (7, 5) if dim == 2 else (5, 4, 3)
     shp = tuple(reversed(size))
-    o = OracleCase(dim)
+    o = OracleCase(dim, lattice=q)
     o.add_block(0, size)
     o.set_collision(0.1)
     o.prepare()
@@ -59,7 +61,7 @@ def test_equilibrium_moments(dim):
     o.set_macros(0, rho, *u)
     o.init_feq()
     f = o.get_f(0)
-    c, _ = lattice.lattice(dim)
+    c, _ = lattice.lattice(dim, q)
     cc = CS * c.astype(float)
     assert np.allclose(f.sum(0), rho, rtol=0, atol=2e-15)
     for a in range(dim):
@@ -68,28 +70,28 @@ def test_equilibrium_moments(dim):
             p = np.tensordot(cc[:, a] * cc[:, b], f, 1)
             exp = rho * ((a == b) + u[a] * u[b])
             assert np.allclose(p, exp, atol=5e-15)
-    assert np.allclose(f, lattice.feq(dim, rho, np.array(u)), atol=1e-15)
+    assert np.allclose(f, lattice.feq(dim, rho, np.array(u), q), atol=1e-15)
 
 
-def _periodic(dim, size, tau_l, rng, amp=0.05, force=None):
+def _periodic(dim, size, tau_l, rng, amp=0.05, force=None, q=None):
     shp = tuple(reversed(size))
     rho, u = _random_macros(rng, shp, dim, amp)
-    o = OracleCase(dim)
+    o = OracleCase(dim, lattice=q)
     cases.periodic_box(o, size, tau_l, rho, *u, **({"force": force} if force else {}))
     return o
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-def test_periodic_conserves_mass_and_momentum(dim):
+@pytest.mark.parametrize("dim,q", [(2, 9), (3, 19), (3, 15)])
+def test_periodic_conserves_mass_and_momentum(dim, q):
     """Without force the BGK collision conserves rho and rho u per node and a
     fully periodic stream is a permutation: totals are kept to round-off."""
     rng = np.random.default_rng(2)
     size = (12, 10) if dim == 2 else (8, 7, 6)
-    o = _periodic(dim, size, 0.8, rng)
+    o = _periodic(dim, size, 0.8, rng, q=q)
     # start off-equilibrium so the collision is not the identity
     f0 = o.get_f(0) * (1 + 0.01 * rng.standard_normal((o.Q,) + o._shape(0)))
     o.set_f(0, f0)
-    c, _ = lattice.lattice(dim)
+    c, _ = lattice.lattice(dim, q)
     m0 = f0.sum()
     j0 = np.tensordot(c.T.astype(float), f0.reshape(o.Q, -1).sum(1), 1)
     o.step(25)
@@ -130,6 +132,115 @@ def test_shear_wave_decay_rate():
     k = 2 * np.pi / n
     expected = U * math.exp(-nu * k * k * steps)
     assert abs(amp / expected - 1) < 2e-3
+
+
+def test_shear_wave_decay_rate_d3q15():
+    """The same decay on D3Q15 (its second and fourth moments are those of
+    D3Q19, so the viscosity is (tau_L - 1/2)/3 again), with the wave along
+    z and the flow along x: exercises every diagonal velocity."""
+    n, tau_l, steps = 48, 0.8, 300
+    size = (3, 3, n)
+    shp = (n, 3, 3)
+    z = np.arange(n)[:, None, None] * np.ones((1, 3, 3))
+    U = 0.01 * CS
+    o = OracleCase(3, lattice=15)
+    zero = np.zeros(shp)
+    cases.periodic_box(o, size, tau_l, np.ones(shp), U * np.sin(2 * np.pi * z / n), zero, zero)
+    o.step(steps)
+    o.update_macros()
+    u = o.get_macro(0, 1)[:, 0, 0]
+    amp = 2 * np.mean(u * np.sin(2 * np.pi * np.arange(n) / n))
+    nu = (tau_l - 0.5) / 3.0
+    k = 2 * np.pi / n
+    assert abs(amp / (U * math.exp(-nu * k * k * steps)) - 1) < 2e-3
+
+
+def test_d3q15_cavity_rest_and_mirror_symmetry():
+    """D3Q15 inside EQMDiffuseRefl walls: rest is a fixed point, and the
+    noiseless lid-driven cavity (lid along x) stays mirror symmetric in z
+    population by population (f_q(x, y, z) = f_m(q)(x, y, n-1-z) with m the
+    z mirror), which fixes the wall masks of the diagonal velocities."""
+    o = OracleCase(3, lattice=15)
+    cases.cavity3d(o, 7, ulid=0.0, noise=False)
+    f0 = o.get_f(0)
+    o.step(5)
+    assert np.max(np.abs(o.get_f(0) - f0)) < 1e-15
+    c, _ = lattice.lattice(3, 15)
+    mirror = [int(np.nonzero((c == c[q] * np.array([1, 1, -1])).all(1))[0][0]) for q in range(15)]
+    o = OracleCase(3, lattice=15)
+    cases.cavity3d(o, 8, re=100.0, ulid=0.05, noise=False)
+    o.step(40)
+    f = o.get_f(0)
+    assert np.max(np.abs(f - f[mirror][:, ::-1])) < 1e-13
+    o.update_macros()
+    assert np.max(np.abs(o.get_macro(0, 1))) > 1e-3
+
+
+# ---- Collision_BGKThermal4th (model_kernel.inc:267-310, 457-502) ---------------
+@pytest.mark.parametrize("dim,q", [(2, 9), (3, 19), (3, 15)])
+def test_thermal_equilibrium_mass_and_momentum(dim, q):
+    """CalcBGKFeq at polyOrder 4 and T != 1: every term beyond the first
+    two is a Hermite polynomial of degree 2-4 in c, whose weighted sum
+    vanishes on a lattice with isotropic moments up to fourth order, so
+    sum_q feq = rho and sum_q CS c_q feq = rho u exactly; a wrong constant
+    in any term (D for D + 2, a sign, a factor) breaks one of them."""
+    o = OracleCase(dim, lattice=q)
+    c, w = lattice.lattice(dim, q)
+    rng = np.random.default_rng(5)
+    for T in (0.7, 1.0, 1.45):
+        for _ in range(3):
+            rho = 1 + 0.1 * rng.standard_normal()
+            u = 0.05 * rng.standard_normal(3)
+            if dim == 2:
+                u[2] = 0.0
+            fe = np.array([o.feq_t(l, rho, u[0], u[1], u[2], T, 4) for l in range(q)])
+            assert abs(fe.sum() - rho) < 1e-14
+            for a in range(dim):
+                assert abs((CS * c[:, a] * fe).sum() - rho * u[a]) < 1e-14
+            # polyOrder 2 at T = 1 is the isothermal equilibrium
+            f2 = np.array([o.feq_t(l, rho, u[0], u[1], u[2], 1.0, 2) for l in range(q)])
+            assert np.allclose(f2, lattice.feq(dim, rho, u[:dim].reshape(dim, 1), q)[:, 0],
+                               atol=1e-15)
+
+
+@pytest.mark.parametrize("dim,q", [(2, 9), (3, 19), (3, 15)])
+def test_thermal_uniform_relaxation_closed_form(dim, q):
+    """A uniform periodic box at rest with density rho0 and temperature T0:
+    streaming is the identity, mass and (zero) momentum are kept, so the
+    collision relaxes each population geometrically to the 4th-order
+    equilibrium, f^n = feq4 + (1 - w)^n (f^0 - feq4) with
+    w = dt / (tau + dt/2), tau = tauRef / (rho0 sqrt(T0)).  Fixes the
+    per-node relaxation time (a tauRef/rho or missing sqrt fails it)."""
+    size = (5, 4) if dim == 2 else (4, 3, 3)
+    shp = tuple(reversed(size))
+    rho0, T0, tau_ref, dt = 1.07, 1.3, 0.6, 0.8
+    o = OracleCase(dim, lattice=q)
+    zero = np.zeros(shp)
+    o.add_block(0, size)
+    faces = [(BS.Left, BS.Right), (BS.Bottom, BS.Top)] + ([(BS.Back, BS.Front)] if dim == 3 else [])
+    for lo, hi in faces:
+        o.add_connection(0, lo, 0, hi, VT.MDPeriodic)
+        o.add_connection(0, hi, 0, lo, VT.MDPeriodic)
+    o.set_collision(tau_ref)
+    o.set_collision_type(1)
+    for lo, hi in faces:
+        for s_ in (lo, hi):
+            o.add_boundary(0, s_, BSch.None_, (), VT.MDPeriodic)
+    o.prepare()
+    o.set_macros(0, np.full(shp, rho0), zero, zero, zero if dim == 3 else None)
+    o.set_temperature(0, np.full(shp, T0))
+    o.init_feq()
+    o.set_dt(dt)
+    f0 = o.get_f(0)
+    fe4 = np.array([o.feq_t(l, rho0, 0.0, 0.0, 0.0, T0, 4) for l in range(q)])
+    tau = tau_ref / (rho0 * math.sqrt(T0))
+    om = dt / (tau + 0.5 * dt)
+    n = 7
+    o.step(n)
+    f = o.get_f(0)
+    exp = fe4.reshape((q,) + (1,) * dim) + (1 - om) ** n * (f0 - fe4.reshape((q,) + (1,) * dim))
+    assert np.max(np.abs(f - exp)) < 1e-14
+    assert np.max(np.abs(f0 - fe4.reshape((q,) + (1,) * dim))) > 1e-3  # not trivially equal
 
 
 def test_rest_state_with_walls_is_fixed_point():
@@ -625,17 +736,26 @@ _REF_D3Q19 = (list(zip([0, -1, 0, 0, -1, -1, -1, -1, 0, 0, 1, 0, 0, 1, 1, 1, 1, 
               [1 / 3] + [1 / 18] * 3 + [1 / 36] * 6 + [1 / 18] * 3 + [1 / 36] * 6)
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-def test_oracle_lattice_tables(dim):
+_REF_D3Q15 = (list(zip([0, -1, 0, 0, -1, -1, -1, -1, 1, 0, 0, 1, 1, 1, 1],
+                       [0, 0, -1, 0, -1, -1, 1, 1, 0, 1, 0, 1, 1, -1, -1],
+                       [0, 0, 0, -1, -1, 1, -1, 1, 0, 0, 1, 1, -1, 1, -1])),
+              [2 / 9] + [1 / 9] * 3 + [1 / 72] * 4 + [1 / 9] * 3 + [1 / 72] * 4)
+
+
+@pytest.mark.parametrize("dim,q", [(2, 9), (3, 19), (3, 15)])
+def test_oracle_lattice_tables(dim, q):
     """The oracle's own velocity and weight tables, read back through its
     equilibrium: f(rho = 1, u = 0) is the weight list, and the odd part
     (f(s e_a) - f(-s e_a)) / (2 CS s w_q) is the velocity component c_qa,
-    both in the reference's order."""
-    cref, wref = _REF_D2Q9 if dim == 2 else _REF_D3Q19
+    both in the reference's order (and equal to the tables whose lattice
+    identities test_lattice_identities checks)."""
+    cref, wref = {9: _REF_D2Q9, 19: _REF_D3Q19, 15: _REF_D3Q15}[q]
+    c, w = lattice.lattice(dim, q)
+    assert np.array_equal(c, np.array(cref)) and np.array_equal(w, np.array(wref))
     s = 1e-3
     size = (2 * dim + 1, 3) if dim == 2 else (2 * dim + 1, 3, 3)
     shp = tuple(reversed(size))
-    o = OracleCase(dim)
+    o = OracleCase(dim, lattice=q)
     o.add_block(0, size)
     o.set_collision(0.1)
     o.prepare()
