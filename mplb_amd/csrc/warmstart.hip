@@ -173,33 +173,52 @@ __global__ __launch_bounds__(64 * kWaves) void k_conv_fwd(const float* __restric
             }
         }
         // weights of the chunk in B-fragment order, zero past cout: thread
-        // -> channel co = tid % CG and the (tap, ci) pairs j = tid / CG +
-        // k (256 / CG), loaded kBatch at a time
+        // -> four output channels co4 .. co4 + 3 (one 16-byte load when
+        // they lie inside cout and cout is a multiple of 4) and the (tap,
+        // ci) pairs j = tid / (CG / 4) + k (256 / (CG / 4)); every load of
+        // the chunk is issued before any is converted, so they are in
+        // flight together
         {
-            const int co = (int)threadIdx.x % CG;
-            const int oc = co0 + co;
-            constexpr int step = 64 * kWaves / CG;
-            constexpr int kBatch = 8;
+            constexpr int groups = CG / 4;
+            constexpr int step = 64 * kWaves / groups;
+            constexpr int kMax = (9 * kCC + step - 1) / step;  // loads per thread
+            const int co4 = ((int)threadIdx.x % groups) * 4;
+            const int oc = co0 + co4;
             const int pairs = 9 * cc;
+            const int j0 = (int)threadIdx.x / groups;
             const FastDiv dcc = fast_div(cc);
-            for (int j0 = (int)threadIdx.x / CG; j0 < pairs; j0 += step * kBatch) {
-                float v[kBatch];
+            const bool vec = (cout & 3) == 0 && oc + 3 < cout;
+            float4 v[kMax];
 #pragma unroll
-                for (int k = 0; k < kBatch; k++) {
-                    const int j = j0 + k * step;
+            for (int k = 0; k < kMax; k++) {
+                const int j = j0 + k * step;
+                v[k] = float4{0.f, 0.f, 0.f, 0.f};
+                if (j < pairs && oc < cout) {
                     int ci;
                     const int tap = fdiv(j, dcc, &ci);
-                    v[k] = j < pairs && oc < cout ? w[((int64_t)tap * cin + c0 + ci) * cout + oc] : 0.f;
+                    const float* src = w + ((int64_t)tap * cin + c0 + ci) * cout + oc;
+                    if (vec) {
+                        v[k] = *reinterpret_cast<const float4*>(src);
+                    } else {
+                        v[k].x = src[0];
+                        if (oc + 1 < cout) v[k].y = src[1];
+                        if (oc + 2 < cout) v[k].z = src[2];
+                        if (oc + 3 < cout) v[k].w = src[3];
+                    }
                 }
+            }
 #pragma unroll
-                for (int k = 0; k < kBatch; k++) {
-                    const int j = j0 + k * step;
-                    if (j >= pairs) break;
-                    int ci;
-                    const int tap = fdiv(j, dcc, &ci);
-                    const int ks = ci >> 3, gg = (ci >> 2) & 1, tt = ci & 3;
-                    s_w[((((tap * (kCC / 8) + ks) * 2 + gg) * CG) + co) * 4 + tt] = to_bf16(v[k]);
-                }
+            for (int k = 0; k < kMax; k++) {
+                const int j = j0 + k * step;
+                if (j >= pairs) break;
+                int ci;
+                const int tap = fdiv(j, dcc, &ci);
+                const int ks = ci >> 3, gg = (ci >> 2) & 1, tt = ci & 3;
+                short* dst = &s_w[((((tap * (kCC / 8) + ks) * 2 + gg) * CG) + co4) * 4 + tt];
+                dst[0] = to_bf16(v[k].x);
+                dst[4] = to_bf16(v[k].y);
+                dst[8] = to_bf16(v[k].z);
+                dst[12] = to_bf16(v[k].w);
             }
         }
         __syncthreads();
