@@ -326,6 +326,51 @@ def bench_unet(args):
     }
 
 
+def secondary(args):
+    """The other configurations, measured in the same process after the
+    headline line's own timed region: the 2D cavity (FP64, 10,000 steps),
+    the three-block L-channel (FP16, 500 steps), the 256 batched cavities
+    (FP32, to the residual 1e-8 per instance) and the learned warm start's
+    time-to-residual on the 512 held-out cases with the committed network
+    (velocity against FP64 runs of the first 64)."""
+    import copy
+    from mplb_amd import warmstart
+    out = {}
+
+    def sub(**kw):
+        a = copy.copy(args)
+        for k, v in kw.items():
+            setattr(a, k, v)
+        return a
+
+    t0 = time.perf_counter()
+    c1 = bench_cavity2d(sub(steps=10000, warmup=100, storage="f64"))
+    out["cavity2d_128_f64"] = {k: c1[k] for k in ("value", "unit", "ms_per_step", "wall_s")}
+    c3 = bench_lchannel(sub(steps=500, warmup=20, storage="f16", lc_a=128, no_error=True,
+                            no_cpu=True))
+    out["lchannel_f16"] = {"value": c3["value"], "unit": c3["unit"],
+                           "ms_per_step": c3["ms_per_step"],
+                           "bulk_frac": c3["roofline"]["frac"],
+                           "mass_drift_max_rel": c3["mass_drift_max_rel"]}
+    c4 = bench_batch(sub(steps=1000, warmup=50, storage="f32", batch=256, converge=True,
+                         check_period=1000, max_steps=400000))
+    conv = dict(c4.get("convergence") or {})
+    conv.pop("per_instance_steps", None)
+    out["batch256_f32"] = {"value": c4["value"], "unit": c4["unit"],
+                           "ms_per_step": c4["ms_per_step"], "convergence": conv}
+    weights = os.path.join(ROOT, "mplb_amd", "data", "unet128_c5.npz")
+    if os.path.exists(weights):
+        ev = warmstart.evaluate_saved(weights, n_test=512, fp64_cases=64, log=log)
+        keep = ("test_cases", "rest_seconds", "learned_seconds", "speedup",
+                "per_case_step_ratio_median", "work_ratio_sum_steps", "rest_steps_max",
+                "learned_steps_max", "inference_seconds", "fallback_cases", "restarted_cases",
+                "prediction_rel_l2_median", "fp64_reference")
+        out["warmstart_time_to_residual"] = {k: ev[k] for k in keep if k in ev}
+        out["warmstart_time_to_residual"]["weights"] = "mplb_amd/data/unet128_c5.npz"
+    out["seconds"] = round(time.perf_counter() - t0, 1)
+    return out
+
+
 def bench_warmstart(args):
     """Learned warm start on 128^2 cavities with 0-3 rectangular obstacles:
     training data from the batched solver, U-Net training on the matrix
@@ -367,6 +412,8 @@ def main():
                          "the three-block L-channel's configured form)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the other configurations measured after the headline")
     ap.add_argument("--no-error", action="store_true",
                     help="skip the FP64 comparison run")
     ap.add_argument("--ws-n", type=int, default=128)
@@ -527,6 +574,8 @@ def main():
     }
     if err is not None:
         out["error_vs_fp64"] = err
+    if not args.no_secondary and args.gpus == 1 and n == 256 and st == "f32":
+        out["secondary"] = secondary(args)
     print(json.dumps(out), flush=True)
 
 

@@ -648,6 +648,34 @@ def run(n=128, n_train=4096, n_test=512, epochs=200, criteria=1e-8, check_period
     t_train = time.perf_counter() - t0
     # 3. evaluation on the held-out cases
     tr, tq = res[n_train:], rects[n_train:]
+    ev = evaluate(net, n, tr, tq, criteria, check_period, storage, restart, diagnose, log)
+    out = {
+        "n": n, "train_cases": n_train, "test_cases": n_test, "criteria": criteria,
+        "check_period": check_period, "params": param_count(layer_specs(enc, bott)),
+        "epochs": epochs,
+        "loss_first": hist[0], "loss_last": hist[-1],
+        "data_seconds": round(t_data, 2), "data_steps_median": float(np.median(data_steps)),
+        "train_seconds": round(t_train, 2),
+        "network": {"encoder": list(enc), "bottleneck": bott, "coarse_loss": list(coarse),
+                    "params": param_count(layer_specs(enc, bott))},
+    }
+    out.update(ev)
+    out["total_seconds"] = round(time.perf_counter() - t_all, 1)
+    return out, net
+
+
+def evaluate(net, n, tr, tq, criteria=1e-8, check_period=1000, storage=hip.STORE_F32,
+             restart=1e3, diagnose=False, log=print, fp64_cases=0):
+    """Time-to-residual of the held-out cases (Reynolds numbers tr,
+    rectangles tq) from rest and from the network's prediction lifted to
+    populations (wall times include set-up; the learned one includes its
+    inputs and inference).  Fallback: an instance whose residual rises above
+    `restart` times its lowest restarts at rest inside the batch; one still
+    unconverged when the budget ends is re-run from rest and that time is
+    added.  fp64_cases > 0: the first that many cases are also solved from
+    rest in FP64 storage (bit-exact with the CPU oracle) to the same
+    criterion, and both starts' converged velocities are compared with it."""
+    n_test = len(tr)
     log("evaluation: %d held-out cases from rest" % n_test)
     tm_rest, tm_warm = {}, {}
     g_rest, s_rest, r_rest, t_rest = solve(n, tr, tq, criteria, check_period, storage=storage,
@@ -704,12 +732,7 @@ def run(n=128, n_train=4096, n_test=512, epochs=200, criteria=1e-8, check_period
                               float(np.median(s_rest / np.maximum(sd, 1)))}
             log("diagnosis %s: %s" % (name, diag[name]))
     out = {
-        "n": n, "train_cases": n_train, "test_cases": n_test, "criteria": criteria,
-        "check_period": check_period, "params": param_count(layer_specs(enc, bott)),
-        "epochs": epochs,
-        "loss_first": hist[0], "loss_last": hist[-1],
-        "data_seconds": round(t_data, 2), "data_steps_median": float(np.median(data_steps)),
-        "train_seconds": round(t_train, 2),
+        "test_cases": n_test, "criteria": criteria, "check_period": check_period,
         "rest_seconds": round(t_rest, 3), "learned_seconds": round(t_warm + t_fb, 3),
         "rest_phases_seconds": {k: round(v, 3) for k, v in tm_rest.items()},
         "learned_phases_seconds": {k: round(v, 3) for k, v in tm_warm.items()},
@@ -723,11 +746,8 @@ def run(n=128, n_train=4096, n_test=512, epochs=200, criteria=1e-8, check_period
         "work_ratio_sum_steps": float(s_rest.sum() / max(s_warm.sum(), 1)),
         "prediction_rel_l2_median": float(np.median(pred_err)),
         "prediction_density_rel_l2_median": float(np.median(rho_err)),
-        "network": {"encoder": list(enc), "bottleneck": bott, "coarse_loss": list(coarse),
-                    "params": param_count(layer_specs(enc, bott))},
         "velocity_rel_l2_learned_vs_rest_median": float(np.median(l2)),
         "velocity_rel_l2_learned_vs_rest_max": float(l2.max()),
-        "total_seconds": round(time.perf_counter() - t_all, 1),
         # the cases that set the batch times
         "slowest_cases": [{"Re": round(tr[b], 1), "rects": len(tq[b]),
                            "rest_steps": int(s_rest[b]), "learned_steps": int(s_warm[b]),
@@ -737,6 +757,34 @@ def run(n=128, n_train=4096, n_test=512, epochs=200, criteria=1e-8, check_period
     }
     if diag:
         out["diagnosis"] = diag
+    if fp64_cases > 0:
+        k = min(fp64_cases, n_test)
+        g64, s64, _, t64 = solve(n, tr[:k], tq[:k], criteria, check_period,
+                                 storage=hip.STORE_F64)
+        g64.update_macros()
+        lw = velocity_l2(g_warm, g64, k)
+        lr = velocity_l2(g_rest, g64, k)
+        out["fp64_reference"] = {
+            "cases": k, "seconds": round(t64, 3),
+            "velocity_rel_l2_learned_vs_fp64_median": float(np.median(lw)),
+            "velocity_rel_l2_learned_vs_fp64_max": float(lw.max()),
+            "velocity_rel_l2_rest_vs_fp64_median": float(np.median(lr)),
+            "velocity_rel_l2_rest_vs_fp64_max": float(lr.max())}
+        log("FP64 reference on %d cases: %s" % (k, out["fp64_reference"]))
+        g64.close()
     g_rest.close()
     g_warm.close()
-    return out, net
+    return out
+
+
+def evaluate_saved(path, n_test=512, seed=0x4D504C42, log=print, fp64_cases=0):
+    """The held-out evaluation of the network saved at `path` (trained on
+    the first n_train cases of the seeded stream, as run() draws them): the
+    n_test cases after them."""
+    net, n_train = load_weights(path)
+    n = net.n
+    res, rects = case_params(n_train + n_test, seed, n=n)
+    out = evaluate(net, n, res[n_train:], rects[n_train:], log=log, fp64_cases=fp64_cases)
+    out.update({"n": n, "train_cases": n_train, "weights": os.path.basename(path)})
+    return out
+
