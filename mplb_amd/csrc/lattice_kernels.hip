@@ -1295,6 +1295,45 @@ __global__ __launch_bounds__(256) void k_halo(HaloParams hp) {
     static_cast<E*>(hp.dst)[di] = src[si];
 }
 
+// Several block connections in one launch: workgroup w belongs to the link
+// whose range [wg_start[l], wg_start[l+1]) holds it; within a link the work
+// is k_halo's.  The host merges only links that neither read a halo cell
+// another merged link writes in this exchange nor write a cell another
+// link wrote before them, and gives each cell the target populations a
+// later merged link writes there (sup), so no two threads write one element
+// and every read sees what the one-link-at-a-time order would show.
+template <typename E>
+__global__ __launch_bounds__(256) void k_halo_multi(const HaloParams* __restrict__ links,
+                                                    const int32_t* __restrict__ wg_start,
+                                                    int nlinks) {
+    const int wg = (int)blockIdx.x;
+    int lo = 0, hi = nlinks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (wg_start[mid] <= wg) lo = mid; else hi = mid - 1;
+    }
+    const HaloParams& hp = links[lo];
+    const int ncell = hp.ext[0] * hp.ext[1] * hp.ext[2];
+    const int cell = (wg - wg_start[lo]) * 256 + (int)threadIdx.x;
+    if (cell >= ncell) return;
+    const int q = (int)blockIdx.y;
+    if (q >= hp.Q) return;
+    const uint8_t fl = hp.other_flag ? hp.other_flag[cell] : 0;
+    if (!((hp.inward >> q) & 1u) && !(fl & 2)) return;
+    if (hp.sup && ((hp.sup[cell] >> hp.qmap[q]) & 1u)) return;
+    const int a = cell % hp.ext[0];
+    const int rest = cell / hp.ext[0];
+    const int bb = rest % hp.ext[1];
+    const int cc = rest / hp.ext[1];
+    const E* src = static_cast<const E*>((fl & 1) ? hp.src_other : hp.src_cur);
+    const int64_t si = hp.src_base + (int64_t)q * hp.src_sq +
+                       (int64_t)(hp.org_src[2] + cc) * hp.src_sz +
+                       (int64_t)(hp.org_src[1] + bb) * hp.src_sx + hp.org_src[0] + a;
+    const int64_t di = hp.dst_base + (int64_t)hp.qmap[q] * hp.dst_sq + hp.dst_org +
+                       a * hp.dstep[0] + bb * hp.dstep[1] + cc * hp.dstep[2];
+    static_cast<E*>(hp.dst)[di] = src[si];
+}
+
 // Per-block sums of the per-workgroup partials, one workgroup per block:
 // the threads stride over the block's slots (its bulk tiles, then its
 // boundary workgroups; or its whole-lattice tiles), and an LDS tree adds
@@ -1489,6 +1528,19 @@ hipError_t launch_halo(const HaloParams& hp, hipStream_t s) {
         hipLaunchKernelGGL(k_halo<float>, grid, dim3(256), 0, s, hp);
     else
         hipLaunchKernelGGL(k_halo<__half>, grid, dim3(256), 0, s, hp);
+    return hipGetLastError();
+}
+
+hipError_t launch_halo_multi(const HaloParams* links, const int32_t* wg_start, int nlinks,
+                             int nwg, int Q, int elem_bytes, hipStream_t s) {
+    if (nlinks < 1 || nwg < 1 || Q < 1 || Q > kMaxQ) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)nwg, (unsigned)Q);
+    if (elem_bytes == 8)
+        hipLaunchKernelGGL(k_halo_multi<double>, grid, dim3(256), 0, s, links, wg_start, nlinks);
+    else if (elem_bytes == 4)
+        hipLaunchKernelGGL(k_halo_multi<float>, grid, dim3(256), 0, s, links, wg_start, nlinks);
+    else
+        hipLaunchKernelGGL(k_halo_multi<__half>, grid, dim3(256), 0, s, links, wg_start, nlinks);
     return hipGetLastError();
 }
 

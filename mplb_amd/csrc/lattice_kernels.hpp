@@ -184,6 +184,10 @@ struct HaloParams {
     // populations (source index) that stream from the target halo into the
     // target block: the only ones a cell without bit 1 needs
     uint32_t inward;
+    // per slab cell (or null): target populations a later connection of the
+    // same merged launch writes at this cell's target, which this one
+    // leaves to it
+    const uint32_t* sup;
 };
 
 }  // namespace mplbhip

@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "lattice_kernels.hpp"
@@ -22,6 +23,8 @@ namespace mplbhip {
 hipError_t launch_lattice(int dim, int storage, int mode, const KParams& kp,
                           hipStream_t s, int part = 0);
 hipError_t launch_halo(const HaloParams& hp, hipStream_t s);
+hipError_t launch_halo_multi(const HaloParams* links, const int32_t* wg_start, int nlinks,
+                             int nwg, int Q, int elem_bytes, hipStream_t s);
 hipError_t launch_reduce(const double* partials, const BlockParams* blocks,
                          int nblocks, int ntiles, int nbulk, int nbnd, int split, int fold,
                          double* out, hipStream_t s);
@@ -41,6 +44,8 @@ struct HaloLink {
     HaloParams hp;       // src/dst buffers filled per launch
     int from, to;
     uint8_t* d_flag = nullptr;
+    uint32_t* d_sup = nullptr;  // merged launch: populations a later link writes
+    bool merged = false;        // copied by the merged launch
 };
 
 struct DevBuf {
@@ -130,6 +135,14 @@ struct mplbhip_ctx {
     DevBuf macro0[4];
     DevBuf temp;  // Variable_T of the thermal collision
     std::vector<HaloLink> halos;
+    // the merged halo launch: its links' parameters for each ping-pong
+    // target (into = 0, 1), their workgroup ranges; the other links follow
+    // it one launch each, in order
+    DevBuf halo_links[2], halo_wgstart;
+    int halo_nmerged = 0, halo_nwg = 0;
+    bool halo_merge = true;
+    std::vector<HaloParams> halo_hps;  // the merged links, buffers not yet filled
+    void* halo_fs[2] = {nullptr, nullptr};  // the fStage buffers halo_links point at
     std::vector<double> host_init_f;  // f given by mplbhip_set_f, per block
     std::vector<float> scale, inv_scale;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -285,7 +298,18 @@ static KParams base_params(mplbhip_ctx* c) {
 // its one lattice f in place: a halo cell written later in the exchange is
 // read as it is (last exchanged, then swapped), as in the reference.
 static int exchange(mplbhip_ctx* c, int into) {
+    bool all_active = true;
+    for (const HaloLink& h : c->halos)
+        all_active = all_active && c->bp[h.from].active && c->bp[h.to].active;
+    const bool merged = c->halo_merge && c->halo_nmerged >= 2 && all_active &&
+                        c->halo_fs[0] == c->fs[0].p && c->halo_fs[1] == c->fs[1].p &&
+                        c->halo_links[0].p && c->halo_links[1].p;
+    if (merged)
+        HIPCHK(c, launch_halo_multi(static_cast<const HaloParams*>(c->halo_links[into].p),
+                                    static_cast<const int32_t*>(c->halo_wgstart.p),
+                                    c->halo_nmerged, c->halo_nwg, c->Q, (int)c->es, c->stream));
     for (HaloLink& h : c->halos) {
+        if (merged && h.merged) continue;
         if (!c->bp[h.from].active || !c->bp[h.to].active) continue;
         HaloParams hp = h.hp;
         hp.src_cur = c->fs[into].p;
@@ -559,8 +583,13 @@ void mplbhip_destroy(mplbhip_ctx* c) {
         dfree(c->fhost);
         dfree(c->parity[0]);
         dfree(c->parity[1]);
-        for (HaloLink& h : c->halos)
+        for (HaloLink& h : c->halos) {
             if (h.d_flag) (void)hipFree(h.d_flag);
+            if (h.d_sup) (void)hipFree(h.d_sup);
+        }
+        dfree(c->halo_links[0]);
+        dfree(c->halo_links[1]);
+        dfree(c->halo_wgstart);
         for (int i = 0; i < 2; i++)
             if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
         if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -749,6 +778,152 @@ int mplbhip_define_boundary(mplbhip_ctx* c, int block, int surface, int scheme,
     return 0;
 }
 
+// The merged halo launch.  A link that reads a halo cell an earlier link
+// writes in the same exchange (raw) must see that write: it stays out,
+// copied by its own launch after the merged one; so does every later link
+// that writes a cell such a link reads or writes (the order of those
+// writes matters), to a fixed point.  The merged links never read what
+// another merged link writes (an earlier one: they would be raw; a later
+// one: they read the previous exchange's buffer, as the single launches
+// do), and where two of them write one target element only the later
+// writes (sup: per source cell, the target populations later merged links
+// write at its target).
+struct HaloBox {
+    int b, org[3], ext[3];
+};
+static bool box_overlap(const HaloBox& a, const HaloBox& b) {
+    if (a.b != b.b) return false;
+    for (int k = 0; k < 3; k++)
+        if (a.org[k] + a.ext[k] <= b.org[k] || b.org[k] + b.ext[k] <= a.org[k]) return false;
+    return true;
+}
+
+static int plan_merged_halos(mplbhip_ctx* c, size_t nl, const std::vector<HaloBox>& src,
+                             const std::vector<HaloBox>& dst, const std::vector<LinkMap>& maps,
+                             const std::vector<std::vector<uint8_t>>& flags,
+                             const std::vector<bool>& raw) {
+    c->halo_nmerged = 0;
+    c->halo_nwg = 0;
+    if (const char* e = std::getenv("MPLBHIP_HALO_MERGE")) c->halo_merge = std::atoi(e) != 0;
+    if (nl < 2) return 0;
+    auto box = [](const HaloBox& s) { return s; };
+    std::vector<bool> deferred(raw);
+    for (bool changed = true; changed;) {
+        changed = false;
+        for (size_t i = 0; i < nl; i++) {
+            if (!deferred[i]) continue;
+            for (size_t m = i + 1; m < nl; m++)
+                if (!deferred[m] && (box_overlap(box(dst[m]), box(src[i])) ||
+                                     box_overlap(box(dst[m]), box(dst[i])))) {
+                    deferred[m] = true;
+                    changed = true;
+                }
+        }
+    }
+    std::vector<int> merged;
+    for (size_t i = 0; i < nl; i++)
+        if (!deferred[i]) merged.push_back((int)i);
+    if (merged.size() < 2) return 0;
+    const int (*cv)[3] = lattice_tab(c->lat).C;
+    const uint32_t all = (1u << c->Q) - 1u;
+    // per merged link: target cell -> source cell, and the target
+    // populations it writes there
+    auto key = [](int x, int y, int z) {
+        return ((int64_t)(x + 4) << 42) | ((int64_t)(y + 4) << 21) | (int64_t)(z + 4);
+    };
+    std::vector<std::unordered_map<int64_t, int>> inv(nl);
+    std::vector<uint32_t> inward_t(nl, 0);
+    for (int m : merged) {
+        const HaloBox& s = src[m];
+        const BlockDef& T = c->plan.blocks[dst[m].b];
+        const HaloParams& hp = c->halos[m].hp;
+        for (int q = 0; q < c->Q; q++)
+            if ((hp.inward >> q) & 1u) inward_t[m] |= 1u << hp.qmap[q];
+        int idx = 0;
+        for (int zz = 0; zz < s.ext[2]; zz++)
+            for (int yy = 0; yy < s.ext[1]; yy++)
+                for (int xx = 0; xx < s.ext[0]; xx++, idx++) {
+                    const int p[3] = {s.org[0] + xx, s.org[1] + yy, s.org[2] + zz};
+                    int t[3];
+                    link_target(maps[m], T.n, p, t);
+                    if (c->dim == 2) t[2] = 0;
+                    inv[m][key(t[0], t[1], t[2])] = idx;
+                }
+    }
+    (void)cv;
+    int nwg = 0;
+    std::vector<int32_t> wgstart;
+    c->halo_hps.clear();
+    for (int i : merged) {
+        HaloLink& h = c->halos[i];
+        const HaloBox& s = src[i];
+        const BlockDef& T = c->plan.blocks[dst[i].b];
+        const int64_t ncell = (int64_t)s.ext[0] * s.ext[1] * s.ext[2];
+        std::vector<uint32_t> sup(ncell, 0);
+        bool any = false;
+        int idx = 0;
+        for (int zz = 0; zz < s.ext[2]; zz++)
+            for (int yy = 0; yy < s.ext[1]; yy++)
+                for (int xx = 0; xx < s.ext[0]; xx++, idx++) {
+                    const int p[3] = {s.org[0] + xx, s.org[1] + yy, s.org[2] + zz};
+                    int t[3];
+                    link_target(maps[i], T.n, p, t);
+                    if (c->dim == 2) t[2] = 0;
+                    for (int m : merged) {
+                        if (m <= i || dst[m].b != dst[i].b) continue;
+                        auto it = inv[m].find(key(t[0], t[1], t[2]));
+                        if (it == inv[m].end()) continue;
+                        sup[idx] |= (flags[m][it->second] & 2) ? all : inward_t[m];
+                    }
+                    any = any || sup[idx] != 0;
+                }
+        if (any && c->device >= 0) {
+            HIPCHK(c, hipMalloc(&h.d_sup, ncell * sizeof(uint32_t)));
+            HIPCHK(c, hipMemcpy(h.d_sup, sup.data(), ncell * sizeof(uint32_t),
+                                hipMemcpyHostToDevice));
+        }
+        h.merged = true;
+        wgstart.push_back(nwg);
+        nwg += (int)((ncell + 255) / 256);
+        HaloParams hp = h.hp;
+        hp.other_flag = h.d_flag;
+        hp.sup = h.d_sup;
+        c->halo_hps.push_back(hp);
+    }
+    wgstart.push_back(nwg);
+    c->halo_nmerged = (int)merged.size();
+    c->halo_nwg = nwg;
+    if (c->device >= 0) {
+        if (int rc = dalloc(c, c->halo_wgstart, wgstart.size() * sizeof(int32_t))) return rc;
+        HIPCHK(c, hipMemcpy(c->halo_wgstart.p, wgstart.data(), wgstart.size() * sizeof(int32_t),
+                            hipMemcpyHostToDevice));
+    }
+    if (std::getenv("MPLBHIP_VERBOSE"))
+        std::fprintf(stderr, "mplbhip: halos: %d of %zu links in one launch (%d workgroups)\n",
+                     c->halo_nmerged, nl, nwg);
+    return 0;
+}
+
+// the merged links' parameters with the fStage buffers filled in, one
+// array per ping-pong target; once the buffers exist (end of prepare)
+static int upload_merged_halos(mplbhip_ctx* c) {
+    if (c->halo_nmerged < 2 || c->device < 0) return 0;
+    for (int into = 0; into < 2; into++) {
+        std::vector<HaloParams> hps = c->halo_hps;
+        for (HaloParams& hp : hps) {
+            hp.src_cur = c->fs[into].p;
+            hp.src_other = c->fs[c->swap ? into : 1 - into].p;
+            hp.dst = c->fs[into].p;
+        }
+        if (int rc = dalloc(c, c->halo_links[into], hps.size() * sizeof(HaloParams))) return rc;
+        HIPCHK(c, hipMemcpy(c->halo_links[into].p, hps.data(), hps.size() * sizeof(HaloParams),
+                            hipMemcpyHostToDevice));
+    }
+    c->halo_fs[0] = c->fs[0].p;
+    c->halo_fs[1] = c->fs[1].p;
+    return 0;
+}
+
 static int setup_halos(mplbhip_ctx* c) {
     // ops_halo_transfer order: blocks by id, then surfaces Left, Right, Top,
     // Bottom, Front, Back (std::map<BoundarySurface> order); edge surfaces
@@ -812,11 +987,14 @@ static int setup_halos(mplbhip_ctx* c) {
                z < s.org[2] + s.ext[2];
     };
     c->halos.clear();
+    std::vector<std::vector<uint8_t>> flags(seq.size());
+    std::vector<bool> raw(seq.size(), false);  // reads a halo cell an earlier link wrote
     for (size_t i = 0; i < seq.size(); i++) {
         const Slab& s = src[i];
         const BlockDef& A = c->plan.blocks[s.b];
         const int64_t ncell = (int64_t)s.ext[0] * s.ext[1] * s.ext[2];
-        std::vector<uint8_t> flag(ncell, 0);
+        std::vector<uint8_t>& flag = flags[i];
+        flag.assign(ncell, 0);
         bool anyother = false;
         for (int zz = 0; zz < s.ext[2]; zz++)
             for (int yy = 0; yy < s.ext[1]; yy++)
@@ -830,6 +1008,7 @@ static int setup_halos(mplbhip_ctx* c) {
                         if (covers(dst[j], s.b, x, y, z)) {
                             if (j < i) before = true; else after = true;
                         }
+                    if (before) raw[i] = true;
                     if (!before && after) {
                         flag[((int64_t)zz * s.ext[1] + yy) * s.ext[0] + xx] = 1;
                         anyother = true;
@@ -904,7 +1083,18 @@ static int setup_halos(mplbhip_ctx* c) {
         }
         c->halos.push_back(h);
     }
-    return 0;
+    std::vector<HaloBox> sbox(seq.size()), dbox(seq.size());
+    for (size_t i = 0; i < seq.size(); i++) {
+        sbox[i].b = src[i].b;
+        dbox[i].b = dst[i].b;
+        for (int k = 0; k < 3; k++) {
+            sbox[i].org[k] = src[i].org[k];
+            sbox[i].ext[k] = src[i].ext[k];
+            dbox[i].org[k] = dst[i].org[k];
+            dbox[i].ext[k] = dst[i].ext[k];
+        }
+    }
+    return plan_merged_halos(c, seq.size(), sbox, dbox, maps, flags, raw);
 }
 
 static bool full_pull_type(int vt) {
@@ -1269,6 +1459,7 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         HIPCHK(c, launch_fill(c->parity[i].p, (int64_t)c->parity[i].bytes, c->stream));
     }
     if (int rc = upload_blocks(c)) return rc;
+    if (int rc = upload_merged_halos(c)) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
