@@ -824,7 +824,6 @@ static int plan_merged_halos(mplbhip_ctx* c, size_t nl, const std::vector<HaloBo
     for (size_t i = 0; i < nl; i++)
         if (!deferred[i]) merged.push_back((int)i);
     if (merged.size() < 2) return 0;
-    const int (*cv)[3] = lattice_tab(c->lat).C;
     const uint32_t all = (1u << c->Q) - 1u;
     // per merged link: target cell -> source cell, and the target
     // populations it writes there
@@ -850,7 +849,6 @@ static int plan_merged_halos(mplbhip_ctx* c, size_t nl, const std::vector<HaloBo
                     inv[m][key(t[0], t[1], t[2])] = idx;
                 }
     }
-    (void)cv;
     int nwg = 0;
     std::vector<int32_t> wgstart;
     c->halo_hps.clear();
