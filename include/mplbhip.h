@@ -124,7 +124,10 @@ int mplbhip_define_boundary(mplbhip_ctx* ctx, int block, int surface,
  * typing, geometry, per-region boundary programs, HBM allocation. */
 int mplbhip_prepare(mplbhip_ctx* ctx);
 /* node type (which=0) or geometry property (which=1) of one block, as the
- * reference's NodeType_<compo> / GeometryProperty fields. */
+ * reference's NodeType_<compo> / GeometryProperty fields; which=2 and 3
+ * describe the launch plan (no reference counterpart): 1 where the node's
+ * boundary program may run inside the bulk kernel, and the faces (bit 2a
+ * low, 2a+1 high side of axis a) across which such a node streams nothing. */
 int mplbhip_node_info(mplbhip_ctx* ctx, int block, int which, int* out);
 
 /* SetInitialMacrosVars upload (app hook, e.g. lbm3d_cavity.cpp:44-68);

@@ -18,6 +18,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "../../include/mplbhip.h"
 #include "lattice_kernels.hpp"
 
@@ -197,6 +199,29 @@ __device__ void apply_eqm(T* f, const BCOp& op) {
         if (op.m_out >> q & 1u)
             f[q] = f[opp(D, q)] +
                    T(2) * rhoW * T(wgt(D, q)) * cu_wall<D, T>(q, u, v, w);
+}
+
+// apply_eqm on deviations g = f - w (reduced storage, FP32 arithmetic),
+// with the terms that depend on the wall velocity only computed by the
+// host (BCOp::eq_*): rho_in = sum_in w + sum_in g, rho_w = 2 rho_in /
+// (1 - dRho - rhoPar), parallel g = rho_w w res - w, outgoing g = g_opp +
+// 2 rho_w w c.u (the w of a population and its opposite are equal)
+template <int D, typename OP>
+__device__ __forceinline__ void apply_eqm_dev(float* g, const OP& op) {
+    constexpr int Q = lq(D);
+    const uint32_t m_in = op.m_in, m_out = op.m_out, m_par = op.m_par;
+    // the incoming sum as one multiply-add per population with a uniform
+    // 0/1 factor (a scalar select), not an add and a vector select
+    float rin = op.eq_win;
+#pragma unroll
+    for (int q = 0; q < Q; q++) rin = __builtin_fmaf(g[q], (m_in >> q & 1u) ? 1.f : 0.f, rin);
+    const float rw = rin * op.eq_k;
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        if (m_par >> q & 1u) g[q] = __builtin_fmaf(rw, op.eq_c[q], -float(wgt(D, q)));
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        if (m_out >> q & 1u) g[q] = __builtin_fmaf(rw, op.eq_c[q], g[opp(D, q)]);
 }
 
 // rescale part of KerCutCellExtrapolPressure1ST[3D]
@@ -619,52 +644,94 @@ __device__ __forceinline__ void finish_node(const KParams& kp, const BlockParams
             const float om = B.fomega, om1 = B.fone_minus_omega;
             const bool addf = force_node && kp.force_type == 1;
             const __amdgpu_buffer_rsrc_t rput = buf_rsrc(reinterpret_cast<uint64_t>(next));
-            auto store = [&](int q, float gn) {
+            // hn: the new value in storage units (h = g / s_q for FP16)
+            auto store = [&](int q, float hn) {
                 if constexpr (BUF) {
                     const uint32_t voff = (uint32_t)node * (uint32_t)sizeof(E);
                     if constexpr (ST == MPLBHIP_STORE_F16)
-                        buf_st_nt<E>(__float2half_rn(gn * kp.inv_scale[q]), rput, voff,
-                                     B.put_off[q]);
-                    else buf_st_nt<E>(gn, rput, voff, B.put_off[q]);
+                        buf_st_nt<E>(__float2half_rn(hn), rput, voff, B.put_off[q]);
+                    else buf_st_nt<E>(hn, rput, voff, B.put_off[q]);
                     return;
                 }
                 E* p = at_u(next, (int64_t)q * B.sq, (uint32_t)node);
                 if constexpr (ST == MPLBHIP_STORE_F16)
-                    __builtin_nontemporal_store(
-                        __half_as_ushort(__float2half_rn(gn * kp.inv_scale[q])),
-                        reinterpret_cast<unsigned short*>(p));
-                else __builtin_nontemporal_store(gn, p);
+                    __builtin_nontemporal_store(__half_as_ushort(__float2half_rn(hn)),
+                                                reinterpret_cast<unsigned short*>(p));
+                else __builtin_nontemporal_store(hn, p);
             };
-            // opposite velocities share the even part of the equilibrium:
-            // geq(q), geq(opp q) = A +- Bq with A = w (drho + rho (4.5 cu^2 -
-            // 1.5 u^2)), Bq = 3 w rho cu, and the force term changes sign
             const float c0 = drho + rho * (-1.5f * usq);
-            store(0, om1 * f[0] + om * (float(wgt(D, 0)) * c0));
+            if constexpr (ST == MPLBHIP_STORE_F16) {
+                // g' = (1 - om) g + om geq in storage units h = g / s_q
+                // (s_q = w_q / kF16Scale).  Opposite velocities share the
+                // even part of the equilibrium: geq(q), geq(opp q) = w_q (A
+                // +- B) with A = drho + rho (4.5 cu^2 - 1.5 u^2), B = 3 rho
+                // cu, so h' = m_q g + k (A +- B), m_q = (1 - om) / s_q, k =
+                // om kF16Scale (w_q cancels); the force term changes sign too
+                const float k = om * float(kF16Scale);
+                auto m_of = [&](int q) { return om1 * float(kF16Scale / wgt(D, q)); };
+                store(0, __builtin_fmaf(m_of(0), f[0], k * c0));
+                const float ka = (k * 4.5f) * rho, kb = (k * 3.f) * rho, kc = k * c0;
 #pragma unroll
-            for (int q = 1; q < Q; q++) {
-                const int p = opp(D, q);
-                if (p < q) continue;
-                float cu = 0.f;
+                for (int q = 1; q < Q; q++) {
+                    const int p = opp(D, q);
+                    if (p < q) continue;
+                    // c.u without a leading 0 + (which would not fold away)
+                    float cu = 0.f;
+                    bool any = false;
 #pragma unroll
-                for (int d = 0; d < ldim(D); d++) {
-                    if (cvel(D, q, d) > 0) cu += ul[d];
-                    if (cvel(D, q, d) < 0) cu -= ul[d];
+                    for (int d = 0; d < ldim(D); d++) {
+                        if (cvel(D, q, d) > 0) { cu = any ? cu + ul[d] : ul[d]; any = true; }
+                        if (cvel(D, q, d) < 0) { cu = any ? cu - ul[d] : -ul[d]; any = true; }
+                    }
+                    const float a = __builtin_fmaf(ka, cu * cu, kc);
+                    const float b = kb * cu;
+                    float hq = a + b, hp = a - b;
+                    if (addf) {
+                        float cg = 0.f;
+#pragma unroll
+                        for (int d = 0; d < ldim(D); d++)
+                            cg += float(cvel(D, q, d)) * float(B.g[d]);
+                        const float fq = (B.fforce * float(kF16Scale) * cg) * rho;
+                        hq += fq;
+                        hp -= fq;
+                    }
+                    const float m = m_of(q);
+                    store(q, __builtin_fmaf(m, f[q], hq));
+                    store(p, __builtin_fmaf(m, f[p], hp));
                 }
-                const float wq = float(wgt(D, q));
-                const float a = wq * (c0 + rho * (4.5f * cu * cu));
-                const float bq = wq * rho * (3.f * cu);
-                float gq = om1 * f[q] + om * (a + bq);
-                float gp = om1 * f[p] + om * (a - bq);
-                if (addf) {
-                    float cg = 0.f;
+            } else {
+                // opposite velocities share the even part of the
+                // equilibrium: geq(q), geq(opp q) = A +- Bq with A = w (drho
+                // + rho (4.5 cu^2 - 1.5 u^2)), Bq = 3 w rho cu, and the force
+                // term changes sign
+                store(0, om1 * f[0] + om * (float(wgt(D, 0)) * c0));
 #pragma unroll
-                    for (int d = 0; d < ldim(D); d++) cg += float(cvel(D, q, d)) * float(B.g[d]);
-                    const float fq = B.fforce * wq * rho * cg;
-                    gq += fq;
-                    gp -= fq;
+                for (int q = 1; q < Q; q++) {
+                    const int p = opp(D, q);
+                    if (p < q) continue;
+                    float cu = 0.f;
+#pragma unroll
+                    for (int d = 0; d < ldim(D); d++) {
+                        if (cvel(D, q, d) > 0) cu += ul[d];
+                        if (cvel(D, q, d) < 0) cu -= ul[d];
+                    }
+                    const float wq = float(wgt(D, q));
+                    const float a = wq * (c0 + rho * (4.5f * cu * cu));
+                    const float bq = wq * rho * (3.f * cu);
+                    float gq = om1 * f[q] + om * (a + bq);
+                    float gp = om1 * f[p] + om * (a - bq);
+                    if (addf) {
+                        float cg = 0.f;
+#pragma unroll
+                        for (int d = 0; d < ldim(D); d++)
+                            cg += float(cvel(D, q, d)) * float(B.g[d]);
+                        const float fq = B.fforce * wq * rho * cg;
+                        gq += fq;
+                        gp -= fq;
+                    }
+                    store(q, gq);
+                    store(p, gp);
                 }
-                store(q, gq);
-                store(p, gp);
             }
         }
     }
@@ -939,7 +1006,6 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
         E* next = static_cast<E*>(kp.fs_buf[1 - cur]) + B.base;
         uint32_t om = 0;
         if constexpr (OBST) om = obst_mask(kp, B, mnode);
-        const uint32_t pull = P ? P->pull : ~0u;
         if (OBST && (om & kObstSolid)) {
             // obstacle node: nothing streams, collides or is read
             solid_residual<D, MODE>(kp, mnode, acc);
@@ -955,70 +1021,146 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
             constexpr bool kBufPull = ST != MPLBHIP_STORE_F64 && !OBST;
             const __amdgpu_buffer_rsrc_t rpull =
                 buf_rsrc(reinterpret_cast<uint64_t>(prev) - (uint64_t)B.pull_shift * sizeof(E));
-            int32_t adj[3] = {0, 0, 0};
-            if (FOLD && P) {
-                adj[0] = x == 0 ? 1 : (x == B.n[0] - 1 ? -1 : 0);
-                adj[1] = y == 0 ? B.sx : (y == B.n[1] - 1 ? -B.sx : 0);
-                if (ZALL) adj[2] = z == 0 ? B.sz : (z == B.n[2] - 1 ? -B.sz : 0);
-            }
-#pragma unroll
-            for (int q = 0; q < Q; q++) {
-                const bool bounce = OBST && (om >> q & 1u);
-                uint32_t nd = (uint32_t)node;
-                if (FOLD && P && !(pull >> q & 1u)) {
-                    // only the components that point out of the block
-                    if (cvel(D, q, 0) > 0 && adj[0] > 0) nd += 1;
-                    if (cvel(D, q, 0) < 0 && adj[0] < 0) nd -= 1;
-                    if (cvel(D, q, 1) > 0 && adj[1] > 0) nd += (uint32_t)B.sx;
-                    if (cvel(D, q, 1) < 0 && adj[1] < 0) nd -= (uint32_t)B.sx;
-                    if (ZALL && cvel(D, q, 2) > 0 && adj[2] > 0) nd += (uint32_t)B.sz;
-                    if (ZALL && cvel(D, q, 2) < 0 && adj[2] < 0) nd -= (uint32_t)B.sz;
-                }
-                E v;
-                if constexpr (kBufPull) {
-                    v = buf_ld<E>(rpull, nd * (uint32_t)sizeof(E), B.pull_off[q]);
-                } else {
-                    v = *(bounce ? at_u(prev, (int64_t)opp(D, q) * B.sq, (uint32_t)node)
-                                 : at_u(prev, (int64_t)q * B.sq - pull_delta<D>(B, q), nd));
-                }
-                if constexpr (ST == MPLBHIP_STORE_F16) f[q] = h2f_scaled(v, kp.scale[q]);
-                else f[q] = v;
-            }
-            if (FOLD && P) {
-                // full values; unpulled populations start at zero (the EQM
-                // applications set every one of them), then the boundary ops
-#pragma unroll
-                for (int q = 0; q < Q; q++) {
-                    if constexpr (ST != MPLBHIP_STORE_F64) f[q] += T(wgt(D, q));
-                    f[q] = (pull >> q & 1u) ? f[q] : T(0);
-                }
-                // the folded lanes of a wave (one wall face, or one row of
-                // it) share one or a few region programs: taken one program
-                // at a time, its ops are wave-uniform, read by scalar loads,
-                // and the per-population choices of the EQM application
-                // become scalar branches instead of every alternative
-                // computed under masks.  Every pass retires at least the
-                // first pending lane.
-                bool pending = true;
-                while (pending) {
-                    const RegionProg* Pu = first_lane_ptr(P);
-                    if (P == Pu) {
-                        const int nops = const_view(Pu)->nops;
-                        for (int o = 0; o < nops; o++) {
-                            BCOp op;
-                            __builtin_memcpy(&op, (const BCOp*)&const_view(Pu)->op[o],
-                                             sizeof op);
-                            apply_eqm<D, T>(f, op);
+            // LEAN (FP16): fewer vector instructions per node, which bound the
+            // FP16 kernel (half the bytes per node of FP32).  FP32 and FP64,
+            // bound by HBM, keep the plain form: FP32 ran 1-2% slower on its
+            // fastest launch shapes with the lean one.
+            constexpr bool LEAN = ST == MPLBHIP_STORE_F16;
+            bool force_node = true;
+            if constexpr (LEAN) {
+                // one load per population and lane; for a lane in a simple
+                // boundary region, sources across a face nothing streams
+                // over are moved back inside the block (RegionProg::adj).
+                // Waves without such a lane skip the shifts (a uniform
+                // branch).
+                auto load_all = [&](auto shifted) {
+                    uint32_t sp[3] = {0u, 0u, 0u}, sm[3] = {0u, 0u, 0u};
+                    if constexpr (decltype(shifted)::value) {
+                        const uint32_t a = P ? P->adj : 0u;
+                        sp[0] = (a & 1u) ? 1u : 0u;
+                        sm[0] = (a & 2u) ? ~0u : 0u;
+                        sp[1] = (a & 4u) ? (uint32_t)B.sx : 0u;
+                        sm[1] = (a & 8u) ? (uint32_t)-B.sx : 0u;
+                        if constexpr (D != 2) {
+                            sp[2] = (a & 16u) ? (uint32_t)B.sz : 0u;
+                            sm[2] = (a & 32u) ? (uint32_t)-B.sz : 0u;
                         }
-                        pending = false;
+                    }
+#pragma unroll
+                    for (int q = 0; q < Q; q++) {
+                        uint32_t nd = (uint32_t)node;
+                        if constexpr (decltype(shifted)::value) {
+#pragma unroll
+                            for (int d = 0; d < ldim(D); d++) {
+                                if (cvel(D, q, d) > 0) nd += sp[d];
+                                if (cvel(D, q, d) < 0) nd += sm[d];
+                            }
+                        }
+                        E v;
+                        if constexpr (kBufPull) {
+                            v = buf_ld<E>(rpull, nd * (uint32_t)sizeof(E), B.pull_off[q]);
+                        } else {
+                            const bool bounce = OBST && (om >> q & 1u);
+                            v = *(bounce
+                                      ? at_u(prev, (int64_t)opp(D, q) * B.sq, (uint32_t)node)
+                                      : at_u(prev, (int64_t)q * B.sq - pull_delta<D>(B, q), nd));
+                        }
+                        f[q] = h2f_scaled(v, kp.scale[q]);
+                    }
+                };
+                if (FOLD && __builtin_amdgcn_ballot_w64(P != nullptr) != 0)
+                    load_all(std::true_type{});
+                else
+                    load_all(std::false_type{});
+                if (FOLD && P) {
+                    // the folded lanes of a wave (one wall face, or one row
+                    // of it) share one or a few region programs: taken one
+                    // program at a time, its ops are wave-uniform, read by
+                    // scalar loads, and the per-population choices of the
+                    // EQM application become scalar branches.  Every pass
+                    // retires at least the first pending lane.  A population
+                    // the program does not pull is written by one of its EQM
+                    // applications before any reads it (a simple program
+                    // keeps no state).  Deviations throughout, the
+                    // wall-velocity terms from the host.
+                    bool pending = true;
+                    while (pending) {
+                        const RegionProg* Pu = first_lane_ptr(P);
+                        if (P == Pu) {
+                            const MPLB_CONST RegionProg& Pc = *const_view(Pu);
+                            force_node = Pc.force != 0;
+                            const int nops = Pc.nops;
+                            for (int o = 0; o < nops; o++) apply_eqm_dev<D>(f, Pc.op[o]);
+                            pending = false;
+                        }
                     }
                 }
+            } else {
+                const uint32_t pull = P ? P->pull : ~0u;
+                // every lane loads all Q populations, so the loads stay one
+                // uniform burst; the pull mask applies afterwards.  A
+                // boundary lane's unpulled source is moved back inside the
+                // block along the axes where it left it: that row is one
+                // the wave (or its workgroup) loads anyway, so no halo line
+                // is fetched for it.
+                int32_t adj[3] = {0, 0, 0};
+                if (FOLD && P) {
+                    adj[0] = x == 0 ? 1 : (x == B.n[0] - 1 ? -1 : 0);
+                    adj[1] = y == 0 ? B.sx : (y == B.n[1] - 1 ? -B.sx : 0);
+                    if (ZALL) adj[2] = z == 0 ? B.sz : (z == B.n[2] - 1 ? -B.sz : 0);
+                }
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const bool bounce = OBST && (om >> q & 1u);
+                    uint32_t nd = (uint32_t)node;
+                    if (FOLD && P && !(pull >> q & 1u)) {
+                        // only the components that point out of the block
+                        if (cvel(D, q, 0) > 0 && adj[0] > 0) nd += 1;
+                        if (cvel(D, q, 0) < 0 && adj[0] < 0) nd -= 1;
+                        if (cvel(D, q, 1) > 0 && adj[1] > 0) nd += (uint32_t)B.sx;
+                        if (cvel(D, q, 1) < 0 && adj[1] < 0) nd -= (uint32_t)B.sx;
+                        if (ZALL && cvel(D, q, 2) > 0 && adj[2] > 0) nd += (uint32_t)B.sz;
+                        if (ZALL && cvel(D, q, 2) < 0 && adj[2] < 0) nd -= (uint32_t)B.sz;
+                    }
+                    E v;
+                    if constexpr (kBufPull) {
+                        v = buf_ld<E>(rpull, nd * (uint32_t)sizeof(E), B.pull_off[q]);
+                    } else {
+                        v = *(bounce ? at_u(prev, (int64_t)opp(D, q) * B.sq, (uint32_t)node)
+                                     : at_u(prev, (int64_t)q * B.sq - pull_delta<D>(B, q), nd));
+                    }
+                    f[q] = v;
+                }
+                if (FOLD && P) {
+                    // full values; unpulled populations start at zero (the
+                    // EQM applications set every one of them), then the
+                    // boundary ops, one wave-uniform program at a time
+#pragma unroll
+                    for (int q = 0; q < Q; q++) {
+                        if constexpr (ST != MPLBHIP_STORE_F64) f[q] += T(wgt(D, q));
+                        f[q] = (pull >> q & 1u) ? f[q] : T(0);
+                    }
+                    bool pending = true;
+                    while (pending) {
+                        const RegionProg* Pu = first_lane_ptr(P);
+                        if (P == Pu) {
+                            const int nops = const_view(Pu)->nops;
+                            for (int o = 0; o < nops; o++) {
+                                BCOp op;
+                                __builtin_memcpy(&op, (const BCOp*)&const_view(Pu)->op[o],
+                                                 sizeof op);
+                                apply_eqm<D, T>(f, op);
+                            }
+                            pending = false;
+                        }
+                    }
+                }
+                force_node = P ? P->force != 0 : true;
             }
             // buffer stores for FP32 only: 2-byte buffer stores ran 7%
             // slower than global ones (FP16 47.7k against 51.3k MLUPS)
             finish_node<D, ST, MODE, ST == MPLBHIP_STORE_F32>(
-                kp, B, f, P != nullptr, P ? P->force != 0 : true, node, mnode, next, x, y, z,
-                acc);
+                kp, B, f, !LEAN && P != nullptr, force_node, node, mnode, next, x, y, z, acc);
         }
     }
     if constexpr (MODE == MODE_STEP_CHECK) reduce_store(acc, kp.partials + (int64_t)wg * kNAcc);

@@ -10,6 +10,10 @@ namespace mplbhip {
 constexpr int kTX = 64;   // one wavefront along x
 constexpr int kTY = 4;    // four wavefronts per workgroup along y
 constexpr int kNAcc = 9;  // residual: 4 squared differences, 4 squares, mass
+// FP16 storage holds h = g / s_q with s_q = w_q / kF16Scale: g / w_q is
+// O(1e-1) for |u| ~ 0.05, so h stays in FP16's normal range with ~11
+// significant bits
+constexpr double kF16Scale = 1024.0;
 // doubles in the file-scope scratch array of the allocation-free entry
 // point (8 MiB per GPU): kNAcc partials per workgroup of a whole-lattice
 // launch plus kNAcc totals per block and for the case

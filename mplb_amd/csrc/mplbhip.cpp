@@ -514,12 +514,11 @@ static void set_lattice(mplbhip_ctx* c, int lat) {
     c->scale.assign(c->Q, 1.f);
     c->inv_scale.assign(c->Q, 1.f);
     if (c->storage == MPLBHIP_STORE_F16) {
-        // per-population scale: g/w_i is O(1e-1) for |u| ~ 0.05, so
-        // s_i = w_i/2^10 keeps h = g/s_i in the normal FP16 range with ~11
-        // significant bits
+        // per-population scale s_i = w_i / kF16Scale (the bulk update
+        // folds the same constant into its coefficients)
         for (int q = 0; q < c->Q; q++) {
-            c->scale[q] = (float)(L.W[q] / 1024.0);
-            c->inv_scale[q] = (float)(1024.0 / L.W[q]);
+            c->scale[q] = (float)(L.W[q] / kF16Scale);
+            c->inv_scale[q] = (float)(kF16Scale / L.W[q]);
         }
     }
 }
@@ -1467,6 +1466,7 @@ int mplbhip_node_info(mplbhip_ctx* c, int block, int which, int* out) {
     if (!c->prepared) return fail(c, MPLBHIP_ERR_ARG, "call mplbhip_prepare first");
     if (block < 0 || block >= (int)c->plan.blocks.size())
         return fail(c, MPLBHIP_ERR_ARG, "undefined block");
+    if (which < 0 || which > 3) return fail(c, MPLBHIP_ERR_ARG, "which must be 0 .. 3");
     const int* n = c->plan.blocks[block].n;
     const int R = c->plan.nregions;
     size_t p = 0;
@@ -1476,9 +1476,15 @@ int mplbhip_node_info(mplbhip_ctx* c, int block, int which, int* out) {
                 const int r = c->plan.region_of(block, i, j, k);
                 const bool obst = !c->obst.empty() &&
                                   (c->obst[(size_t)c->bp[block].mbase + p] & kObstSolid);
-                out[p] = which == 0 ? (obst ? (int)VT_ImmersedSolid
-                                            : c->plan.region_vt[(size_t)block * R + r])
-                                    : c->plan.region_vg[(size_t)block * R + r];
+                const RegionProg& P = c->plan.progs[(size_t)block * R + r];
+                if (which == 0)
+                    out[p] = obst ? (int)VT_ImmersedSolid : c->plan.region_vt[(size_t)block * R + r];
+                else if (which == 1)
+                    out[p] = c->plan.region_vg[(size_t)block * R + r];
+                else if (which == 2)
+                    out[p] = P.simple;
+                else
+                    out[p] = (int)P.adj;
                 p++;
             }
     return 0;

@@ -1,6 +1,8 @@
 // plan.cpp -- node typing, geometry and per-region boundary programs.
 #include "plan.hpp"
 
+#include <cmath>
+
 #include <cstdio>
 #include <cstring>
 
@@ -522,11 +524,65 @@ bool Plan::build() {
         error = "persisted boundary state too large";
         return false;
     }
-    for (RegionProg& P : progs) {
-        bool eqm_only = true;
-        for (int o = 0; o < P.nops; o++) eqm_only = eqm_only && P.op[o].type == OP_EQM;
-        P.simple = (P.kind == RK_FULL || P.kind == RK_MASKED) && P.persist < 0 && eqm_only;
-    }
+    const double cs = std::sqrt(3.0);
+    for (int b = 0; b < nb; b++)
+        for (int r = 0; r < nregions; r++) {
+            RegionProg& P = progs[(size_t)b * nregions + r];
+            bool eqm_only = true;
+            for (int o = 0; o < P.nops; o++) eqm_only = eqm_only && P.op[o].type == OP_EQM;
+            P.simple = (P.kind == RK_FULL || P.kind == RK_MASKED) && P.persist < 0 && eqm_only;
+            for (int o = 0; o < P.nops; o++) {
+                BCOp& op = P.op[o];
+                if (op.type != OP_EQM) continue;
+                double win = 0.0, drho = 0.0, rpar = 0.0;
+                const double u[3] = {op.vars[0], op.vars[1], op.vars[2]};
+                const double u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+                for (int q = 0; q < Q; q++) {
+                    const double cu = cs * (C[q][0] * u[0] + C[q][1] * u[1] + C[q][2] * u[2]);
+                    op.eq_c[q] = 0.f;
+                    if (op.m_in >> q & 1u) {
+                        win += L.W[q];
+                    } else if (op.m_out >> q & 1u) {
+                        drho += 2.0 * L.W[q] * cu;
+                        op.eq_c[q] = (float)(2.0 * L.W[q] * cu);
+                    } else if (op.m_par >> q & 1u) {
+                        const double f1 = L.W[q] * (1.0 + cu + 0.5 * (cu * cu - u2));
+                        rpar += f1;
+                        op.eq_c[q] = (float)f1;
+                    }
+                }
+                op.eq_win = (float)win;
+                op.eq_k = (float)(2.0 / (1.0 - drho - rpar));
+            }
+            // faces the region does not stream across: the population that
+            // points straight in from such a face is not pulled
+            P.adj = 0u;
+            if (!P.simple || P.kind != RK_MASKED) continue;
+            int code[3] = {r % 3, (r / 3) % 3, r / 9};
+            for (int a = 0; a < dim; a++) {
+                if (code[a] == 0) continue;
+                const int in = code[a] == 1 ? 1 : -1;  // c_a of a source outside
+                for (int q = 1; q < Q; q++) {
+                    bool unit = C[q][a] == in;
+                    for (int d = 0; d < 3; d++) unit = unit && (d == a || C[q][d] == 0);
+                    if (unit && !(P.pull >> q & 1u)) P.adj |= 1u << (2 * a + (code[a] - 1));
+                }
+            }
+            // a lane shifts every population crossing a flagged face: each
+            // such population must be one the region does not stream
+            for (int q = 0; q < Q; q++) {
+                bool crosses = false;
+                for (int a = 0; a < dim; a++) {
+                    if ((P.adj >> (2 * a) & 1u) && C[q][a] > 0) crosses = true;
+                    if ((P.adj >> (2 * a + 1) & 1u) && C[q][a] < 0) crosses = true;
+                }
+                if (crosses && (P.pull >> q & 1u)) {
+                    P.simple = 0;  // the boundary kernel runs it
+                    P.adj = 0u;
+                    break;
+                }
+            }
+        }
     return true;
 }
 

@@ -55,6 +55,13 @@ struct BCOp {
     int32_t nb_nops;              // neighbour ops applied before the read
     int32_t pad;
     double vars[3];               // wall velocity, or rho given in vars[0]
+    // EQMDiffuseRefl in deviation form (g = f - w), the terms that depend
+    // on the wall velocity only, for the reduced-storage bulk kernel:
+    // rho_in = eq_win + sum_in g, rho_w = rho_in eq_k, then g = rho_w
+    // eq_c - w (parallel) and g = g_opp + rho_w eq_c (outgoing)
+    float eq_win;                 // sum of w over m_in
+    float eq_k;                   // 2 / (1 - dRho - rhoPar)
+    float eq_c[kMaxQ];            // parallel: w res(u); outgoing: 2 w c.u
 };
 
 struct RegionProg {
@@ -68,6 +75,12 @@ struct RegionProg {
     int32_t vt, vg;      // node type and geometry of the region (the swap
                          // scheme's local swap and swap stream decode vg)
     int32_t nops;
+    // faces of the region across which nothing streams, bits 2a (low
+    // face of axis a) and 2a+1 (high face): a bulk-kernel lane moves the
+    // source of a population that crosses one back inside the block (that
+    // population is never streamed, and the row it then reads is one its
+    // wave loads anyway).  Simple regions only.
+    uint32_t adj;
     BCOp op[kMaxOps];
 };
 

@@ -185,3 +185,38 @@ def test_lattice_and_collision_type_arguments():
     cases.cavity3d(o, 7, noise=False)
     for which in (0, 1):
         assert np.array_equal(p.node_info(0, which), o.get_int(0, which))
+
+
+def test_folded_region_shift_faces():
+    """The faces across which a node of a simple region streams nothing
+    (node_info which=3: bit 2a low side, 2a+1 high side of axis a), whose
+    sources the bulk kernel moves back inside the block: every wall face
+    of a closed cavity at its face, edge and corner nodes; none inside or
+    on periodic faces; and a region whose program keeps state (the cavity's
+    corners persist populations) is not simple."""
+    p = PlanOnly(3)
+    cases.cavity3d(p, 7, noise=False)
+    simple, adj = p.node_info(0, 2), p.node_info(0, 3)   # [k][j][i]
+    n = 7
+    for k in range(n):
+        for j in range(n):
+            for i in range(n):
+                want = 0
+                for a, c in enumerate((i, j, k)):
+                    if c == 0:
+                        want |= 1 << (2 * a)
+                    elif c == n - 1:
+                        want |= 1 << (2 * a + 1)
+                corner = sum(c in (0, n - 1) for c in (i, j, k)) == 3
+                if corner:
+                    assert simple[k, j, i] == 0 and adj[k, j, i] == 0, (i, j, k)
+                else:
+                    assert simple[k, j, i] == 1 and adj[k, j, i] == want, (i, j, k)
+
+    q = PlanOnly(3)
+    cases.periodic_box(q, (5, 4, 6), 0.8, np.ones((6, 4, 5)), np.zeros((6, 4, 5)),
+                       np.zeros((6, 4, 5)), np.zeros((6, 4, 5)))
+    assert np.all(q.node_info(0, 3) == 0)
+    assert np.all(q.node_info(0, 2) == 1)
+    with pytest.raises(hip.MplbError, match="which"):
+        q.node_info(0, 4)
