@@ -233,9 +233,10 @@ int mplbhip_mass(mplbhip_ctx* ctx, double* mass);
 /* No counterpart in the reference: mplbhip_mass for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving
  * loops): it calls no hipMalloc or hipFree.  Its partial sums live in a
- * file-scope device array of 1 Mi doubles (8 MiB) per GPU, so a case may
- * have at most about 116,000 workgroups of 256 nodes (every block's
- * nodes, 64 x 4 per z plane); larger cases are refused with
+ * file-scope device array of 1 Mi doubles (8 MiB) per GPU, one slot per
+ * workgroup of 256 nodes of the launches that measure it (the same as
+ * mplbhip_mass's), so a case may have at most about 116,000 such
+ * workgroups; larger cases are refused with
  * MPLBHIP_ERR_UNSUPPORTED.  Needs at least one step taken (f^0 is formed on
  * the device by the first step).  Calls on one GPU share the array: calls
  * that overlap there race.  Same value as mplbhip_mass, bit for bit. */

@@ -980,7 +980,12 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
         y = ty * kTY + threadIdx.y;
     }
     const int cur = load_parity(kp, b);
-    if (first && threadIdx.x == 0 && threadIdx.y == 0) kp.cur_out[b] = B.active ? 1 - cur : cur;
+    // a step reads the latest fStage and flips the block's parity;
+    // MATERIALIZE re-streams from the fStage before the latest (as the
+    // whole-lattice kernel does) and changes nothing
+    if (MODE != MODE_MATERIALIZE && first && threadIdx.x == 0 && threadIdx.y == 0)
+        kp.cur_out[b] = B.active ? 1 - cur : cur;
+    const int src = MODE == MODE_MATERIALIZE ? 1 - cur : cur;
     double acc[kNAcc];
 #pragma unroll
     for (int a = 0; a < kNAcc; a++) acc[a] = 0.0;
@@ -1002,12 +1007,21 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
     if (active) {
         const int32_t node = z * B.sz + y * B.sx + x;
         const int64_t mnode = B.mbase + ((int64_t)z * B.n[1] + y) * B.n[0] + x;
-        const E* prev = static_cast<const E*>(kp.fs_buf[cur]) + B.base;
+        const E* prev = static_cast<const E*>(kp.fs_buf[src]) + B.base;
         E* next = static_cast<E*>(kp.fs_buf[1 - cur]) + B.base;
         uint32_t om = 0;
         if constexpr (OBST) om = obst_mask(kp, B, mnode);
         if (OBST && (om & kObstSolid)) {
-            // obstacle node: nothing streams, collides or is read
+            // obstacle node: nothing streams, collides or is read (its f is
+            // zero when materialized)
+            if constexpr (MODE == MODE_MATERIALIZE) {
+                if (kp.fhost && !kp.fhost_in) {
+                    const int64_t nn = (int64_t)B.n[0] * B.n[1] * B.n[2];
+                    const int64_t h = B.fbase + ((int64_t)z * B.n[1] + y) * B.n[0] + x;
+#pragma unroll
+                    for (int q = 0; q < Q; q++) kp.fhost[h + q * nn] = 0.0;
+                }
+            }
             solid_residual<D, MODE>(kp, mnode, acc);
         } else {
             T f[Q];
@@ -1163,7 +1177,8 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
                 kp, B, f, !LEAN && P != nullptr, force_node, node, mnode, next, x, y, z, acc);
         }
     }
-    if constexpr (MODE == MODE_STEP_CHECK) reduce_store(acc, kp.partials + (int64_t)wg * kNAcc);
+    if constexpr (MODE == MODE_STEP_CHECK || MODE == MODE_MATERIALIZE)
+        reduce_store(acc, kp.partials + (int64_t)wg * kNAcc);
 }
 
 // ---- warm-start initial state ---------------------------------------------------
@@ -1478,13 +1493,13 @@ __global__ __launch_bounds__(256) void k_halo_multi(const HaloParams* __restrict
 
 // Per-block sums of the per-workgroup partials, one workgroup per block:
 // the threads stride over the block's slots (its bulk tiles, then its
-// boundary workgroups; or its whole-lattice tiles), and an LDS tree adds
-// the threads' sums; both orders are fixed, so the sums are reproducible.
-__global__ __launch_bounds__(256) void k_reduce_blocks(const double* __restrict__ partials,
-                                                       const BlockParams* __restrict__ blocks,
-                                                       int nblocks, int ntiles, int nbulk,
-                                                       int nbnd, int split, int fold,
-                                                       double* __restrict__ out) {
+// boundary workgroups; or its whole-lattice tiles), then the waves' sums
+// (shuffles) and the workgroup's (LDS) are added; every order is fixed, so
+// the sums are reproducible.
+constexpr int kRedThreads = 1024;
+__global__ __launch_bounds__(kRedThreads) void k_reduce_blocks(
+    const double* __restrict__ partials, const BlockParams* __restrict__ blocks, int nblocks,
+    int ntiles, int nbulk, int nbnd, int split, int fold, double* __restrict__ out) {
     const int b = blockIdx.x;
     const bool last = b + 1 == nblocks;
     int lo0, hi0, lo1 = 0, hi1 = 0;
@@ -1500,21 +1515,30 @@ __global__ __launch_bounds__(256) void k_reduce_blocks(const double* __restrict_
         hi0 = last ? ntiles : blocks[b + 1].tile_start;
     }
     const int n0 = hi0 - lo0, count = n0 + (hi1 - lo1);
-    __shared__ double red[256];
+    // one pass over the slots, all kNAcc sums per thread (a slot's kNAcc
+    // doubles are contiguous), then waves and workgroup in a fixed order
+    double s[kNAcc];
+#pragma unroll
+    for (int a = 0; a < kNAcc; a++) s[a] = 0.0;
+    for (int i = threadIdx.x; i < count; i += kRedThreads) {
+        const int slot = i < n0 ? lo0 + i : lo1 + (i - n0);
+        const double* p = partials + (int64_t)slot * kNAcc;
+#pragma unroll
+        for (int a = 0; a < kNAcc; a++) s[a] += p[a];
+    }
+    constexpr int kWaves = kRedThreads / 64;
+    __shared__ double red[kWaves][kNAcc];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
     for (int a = 0; a < kNAcc; a++) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < count; i += 256) {
-            const int slot = i < n0 ? lo0 + i : lo1 + (i - n0);
-            s += partials[(int64_t)slot * kNAcc + a];
-        }
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[(int64_t)b * kNAcc + a] = red[0];
-        __syncthreads();
+        const double w = wave_sum(s[a]);
+        if (lane == 0) red[wv][a] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < kNAcc) {
+        double t = 0.0;
+        for (int k = 0; k < kWaves; k++) t += red[k][threadIdx.x];
+        out[(int64_t)b * kNAcc + threadIdx.x] = t;
     }
 }
 
@@ -1608,6 +1632,23 @@ static hipError_t launch_mode(int mode, int part, const KParams& kp, hipStream_t
     return hipGetLastError();
 }
 
+// MATERIALIZE through the step's bulk and boundary kernels (their launch
+// shape and partial-sum slots); the whole-lattice launch remains for t = 0
+// (f^0 from a host array), fused cases and D3Q15
+hipError_t launch_materialize_split(int dim, int storage, const KParams& kp, hipStream_t s) {
+    if (kp.fused || kp.fhost_in || (dim != 2 && dim != 3)) return hipErrorInvalidValue;
+    if (dim == 2) {
+        if (storage == MPLBHIP_STORE_F64) launch_step<2, MPLBHIP_STORE_F64, MODE_MATERIALIZE>(kp, 0, s);
+        else if (storage == MPLBHIP_STORE_F32) launch_step<2, MPLBHIP_STORE_F32, MODE_MATERIALIZE>(kp, 0, s);
+        else launch_step<2, MPLBHIP_STORE_F16, MODE_MATERIALIZE>(kp, 0, s);
+    } else {
+        if (storage == MPLBHIP_STORE_F64) launch_step<3, MPLBHIP_STORE_F64, MODE_MATERIALIZE>(kp, 0, s);
+        else if (storage == MPLBHIP_STORE_F32) launch_step<3, MPLBHIP_STORE_F32, MODE_MATERIALIZE>(kp, 0, s);
+        else launch_step<3, MPLBHIP_STORE_F16, MODE_MATERIALIZE>(kp, 0, s);
+    }
+    return hipGetLastError();
+}
+
 // lattices that run every step as one whole-lattice launch (D3Q15): only
 // the SCOPE_ALL kernels are instantiated for them
 template <int D, int ST>
@@ -1689,7 +1730,7 @@ hipError_t launch_halo_multi(const HaloParams* links, const int32_t* wg_start, i
 hipError_t launch_reduce(const double* partials, const BlockParams* blocks, int nblocks,
                          int ntiles, int nbulk, int nbnd, int split, int fold, double* out,
                          hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_blocks, dim3(nblocks), dim3(256), 0, s, partials, blocks,
+    hipLaunchKernelGGL(k_reduce_blocks, dim3(nblocks), dim3(kRedThreads), 0, s, partials, blocks,
                        nblocks, ntiles, nbulk, nbnd, split, fold, out);
     hipLaunchKernelGGL(k_reduce_total, dim3(1), dim3(256), 0, s, out, nblocks);
     return hipGetLastError();

@@ -22,6 +22,7 @@
 namespace mplbhip {
 hipError_t launch_lattice(int dim, int storage, int mode, const KParams& kp,
                           hipStream_t s, int part = 0);
+hipError_t launch_materialize_split(int dim, int storage, const KParams& kp, hipStream_t s);
 hipError_t launch_halo(const HaloParams& hp, hipStream_t s);
 hipError_t launch_halo_multi(const HaloParams* links, const int32_t* wg_start, int nlinks,
                              int nwg, int Q, int elem_bytes, hipStream_t s);
@@ -482,12 +483,18 @@ static int materialize(mplbhip_ctx* c, bool want_f, bool residual,
         kp.fhost = want_f ? static_cast<double*>(c->fhost.p) : nullptr;
     }
     kp.want_residual = residual ? 1 : 0;
-    HIPCHK(c, launch_lattice(c->lat, c->storage, MODE_MATERIALIZE, kp, c->stream));
+    // after the first step, through the step's bulk and boundary kernels
+    // (the whole-lattice kernel is the general one, several times slower)
+    const bool split = c->tstep > 0 && !c->fused && (c->lat == 2 || c->lat == 3);
+    if (split)
+        HIPCHK(c, launch_materialize_split(c->lat, c->storage, kp, c->stream));
+    else
+        HIPCHK(c, launch_lattice(c->lat, c->storage, MODE_MATERIALIZE, kp, c->stream));
     if (acc_out) {
         HIPCHK(c, launch_reduce(static_cast<const double*>(c->partials.p),
                                 static_cast<const BlockParams*>(c->blocks.p),
-                                (int)c->bp.size(), c->ntiles, c->nbulk, 0, 0, 0,
-                                static_cast<double*>(c->red.p), c->stream));
+                                (int)c->bp.size(), c->ntiles, kp.nbulk, kp.nbnd, split ? 1 : 0,
+                                kp.fold, static_cast<double*>(c->red.p), c->stream));
         acc_out->resize((c->bp.size() + 1) * kNAcc);
         HIPCHK(c, hipMemcpyAsync(acc_out->data(), c->red.p,
                                  acc_out->size() * sizeof(double),
@@ -2215,12 +2222,17 @@ int mplbhip_mass(mplbhip_ctx* c, double* mass) {
 int mplbhip_mass_noalloc(mplbhip_ctx* c, double* mass) {
     if (!c || !mass) return MPLBHIP_ERR_ARG;
     if (!c->prepared) return fail(c, MPLBHIP_ERR_ARG, "call mplbhip_prepare first");
-    const long long need = ((long long)c->ntiles + (long long)c->bp.size() + 1) * kNAcc;
+    // the same launches and partial-sum slots as mplbhip_mass after the
+    // first step: the bulk and boundary kernels, or the whole-lattice one
+    const bool split = !c->swap && !c->fused && (c->lat == 2 || c->lat == 3);
+    KParams kp = base_params(c);
+    const long long slots = split ? (long long)kp.nbulk + kp.nbnd : (long long)c->ntiles;
+    const long long need = (slots + (long long)c->bp.size() + 1) * kNAcc;
     if (need > kScratchDoubles)
         return fail(c, MPLBHIP_ERR_UNSUPPORTED,
                     "mplbhip_mass_noalloc: the case needs %lld doubles of scratch, the limit is "
-                    "%lld (%d workgroups of 256 nodes); use mplbhip_mass",
-                    need, (long long)kScratchDoubles, c->ntiles);
+                    "%lld (%lld workgroups of 256 nodes); use mplbhip_mass",
+                    need, (long long)kScratchDoubles, slots);
     if (int rc = need_device(c)) return rc;
     if (int rc = ensure_init(c)) return rc;
     if (c->tstep == 0)
@@ -2229,17 +2241,18 @@ int mplbhip_mass_noalloc(mplbhip_ctx* c, double* mass) {
                     "by the first step); use mplbhip_mass at t = 0");
     if (!c->scratch) return fail(c, MPLBHIP_ERR_HIP, "no scratch address for this device");
     HIPCHK(c, hipSetDevice(c->device));
-    // every workgroup of the whole-lattice launch writes its partial slot
-    // and the block sums overwrite theirs, so nothing needs clearing
-    KParams kp = base_params(c);
+    // every workgroup of the launches writes its partial slot and the block
+    // sums overwrite theirs, so nothing needs clearing
     kp.partials = c->scratch;
     kp.fhost = nullptr;
     kp.want_residual = 0;
     if (c->swap) HIPCHK(c, launch_swap(5, kp, 0, c->stream));
+    else if (split) HIPCHK(c, launch_materialize_split(c->lat, c->storage, kp, c->stream));
     else HIPCHK(c, launch_lattice(c->lat, c->storage, MODE_MATERIALIZE, kp, c->stream));
-    double* out = c->scratch + (size_t)c->ntiles * kNAcc;
+    double* out = c->scratch + (size_t)slots * kNAcc;
     HIPCHK(c, launch_reduce(c->scratch, static_cast<const BlockParams*>(c->blocks.p),
-                            (int)c->bp.size(), c->ntiles, c->nbulk, 0, 0, 0, out, c->stream));
+                            (int)c->bp.size(), c->ntiles, kp.nbulk, kp.nbnd, split ? 1 : 0,
+                            kp.fold, out, c->stream));
     HIPCHK(c, hipMemcpyAsync(mass, out + c->bp.size() * kNAcc + 8, sizeof(double),
                              hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -54,3 +54,20 @@ def test_noalloc_mass_matches_mass_on_two_cases():
     log("  24^2: noalloc %.17g, mass %.17g" % (a, b))
     assert a == b
     h.close()
+
+
+@pytest.mark.gpu
+def test_noalloc_mass_matches_mass_in_the_split_step_form(monkeypatch):
+    """Cases of 2^18 nodes and more step (and measure their mass) through
+    the bulk and boundary kernels: forced here on a 48^3 FP16 cavity, then a
+    smaller 36^3 FP64 one, the two entry points agree bit for bit."""
+    monkeypatch.setenv("MPLBHIP_FUSED", "0")
+    for n, st in ((48, hip.STORE_F16), (36, hip.STORE_F64)):
+        log("no-alloc mass, split step form: 3D cavity %d^3 storage %d" % (n, st))
+        g = hip.LatticeCase(3, storage=st, device=0)
+        cases.cavity3d(g, n, re=300.0, noise=True)
+        g.step(9)
+        a, b = g.mass_noalloc(), g.mass()
+        log("  %d^3: noalloc %.17g, mass %.17g" % (n, a, b))
+        assert a == b and np.isfinite(a)
+        g.close()
