@@ -213,13 +213,20 @@ def bench_batch(args):
     cases.batched_cavities(g, sizes, res)
     g.step(args.warmup)
     g.sync()
+    # the value: K steps as graph replays (the path iterate_batch runs);
+    # then the kernel split from K more steps with events around each kernel
     t0 = time.perf_counter()
-    _, ker = g.step_timed(args.steps)
+    g.step(args.steps)
+    g.sync()
     wall = time.perf_counter() - t0
     mlups = nodes * args.steps / wall / 1e6
+    _, ker = g.step_timed(args.steps)
     b, d = g.kernel_times()
-    log("bench: batch %.0f MLUPS (wall), kernels %.4f ms/step (bulk %.4f, boundary %.4f)"
-        % (mlups, ker / args.steps, b / args.steps, d / args.steps))
+    fold, xgroup, h2, bulk_nodes = g.launch_shape()
+    achieved = bulk_nodes * 2 * 9 * ELEM[st] / (b / args.steps / 1e3) / 1e9
+    log("bench: batch %.0f MLUPS (wall, graph replays), event-timed kernels %.4f ms/step "
+        "(bulk %.4f = %.0f GB/s, boundary %.4f)"
+        % (mlups, ker / args.steps, b / args.steps, achieved, d / args.steps))
     conv = None
     if args.converge:
         t0 = time.perf_counter()
@@ -245,6 +252,15 @@ def bench_batch(args):
                                "[100,1000], lid 0.05, %s storage" % (len(sizes), st),
                    "nodes": nodes},
         "kernel_ms_per_step": round(ker / args.steps, 5),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "traffic_source": "no committed counter summary for the batch",
+                     "kernel": "k_bulk (D2Q9, %s, step)" % st,
+                     "bulk_nodes_per_launch": bulk_nodes,
+                     "launch_shape": {"fold": fold, "xgroup": xgroup, "h2": h2},
+                     "bulk_ms_per_step": round(b / args.steps, 5)},
+        "timing": "value: K steps of graph replays (the path iterate_batch runs); kernel times: "
+                  "HIP events around every kernel of K more steps launched one by one",
         "convergence": conv,
     }
 

@@ -1035,11 +1035,14 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
             constexpr bool kBufPull = ST != MPLBHIP_STORE_F64 && !OBST;
             const __amdgpu_buffer_rsrc_t rpull =
                 buf_rsrc(reinterpret_cast<uint64_t>(prev) - (uint64_t)B.pull_shift * sizeof(E));
-            // LEAN (FP16): fewer vector instructions per node, which bound the
-            // FP16 kernel (half the bytes per node of FP32).  FP32 and FP64,
-            // bound by HBM, keep the plain form: FP32 ran 1-2% slower on its
-            // fastest launch shapes with the lean one.
-            constexpr bool LEAN = ST == MPLBHIP_STORE_F16;
+            // LEAN: fewer vector instructions per node, which bound the FP16
+            // kernel (half the bytes per node of FP32) and the 2D FP32 one
+            // (9 populations per node, boundary lanes in every wave of a
+            // small block): 256 batched 2D cavities ran 10% faster.  3D FP32
+            // and FP64, bound by HBM, keep the plain form: 3D FP32 ran 1-2%
+            // slower on its fastest launch shapes with the lean one.
+            constexpr bool LEAN =
+                ST == MPLBHIP_STORE_F16 || (ST == MPLBHIP_STORE_F32 && D == 2);
             bool force_node = true;
             if constexpr (LEAN) {
                 // one load per population and lane; for a lane in a simple
@@ -1079,7 +1082,8 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
                                       ? at_u(prev, (int64_t)opp(D, q) * B.sq, (uint32_t)node)
                                       : at_u(prev, (int64_t)q * B.sq - pull_delta<D>(B, q), nd));
                         }
-                        f[q] = h2f_scaled(v, kp.scale[q]);
+                        if constexpr (ST == MPLBHIP_STORE_F16) f[q] = h2f_scaled(v, kp.scale[q]);
+                        else f[q] = v;
                     }
                 };
                 if (FOLD && __builtin_amdgcn_ballot_w64(P != nullptr) != 0)
