@@ -371,13 +371,26 @@ __global__ __launch_bounds__(64 * kWaves) void k_conv_wgrad(const float* __restr
 }
 
 // dw = sum over chunks (chunk order: deterministic)
-__global__ void k_sum_partials(const float* __restrict__ partial, float* __restrict__ dst,
-                               int64_t count, int nchunks) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        float s = 0.f;
-        for (int c = 0; c < nchunks; c++) s += partial[(int64_t)c * count + i];
-        dst[i] = s;
+// dst[i] = sum over the chunks of partial[c][i]: a workgroup takes 64
+// consecutive i, its four waves the chunks c = w, w + 4, ... (so each wave
+// has a quarter of the dependent adds), and the waves' sums are added in
+// wave order through LDS: a fixed order, reproducible sums.
+constexpr int kSumWaves = 4;
+__global__ __launch_bounds__(64 * kSumWaves) void k_sum_partials(const float* __restrict__ partial,
+                                                                 float* __restrict__ dst,
+                                                                 int64_t count, int nchunks) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+    float s = 0.f;
+    if (i < count)
+        for (int c = w; c < nchunks; c += kSumWaves) s += partial[(int64_t)c * count + i];
+    __shared__ float red[kSumWaves][64];
+    red[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && i < count) {
+        float t = red[0][lane];
+        for (int v = 1; v < kSumWaves; v++) t += red[v][lane];
+        dst[i] = t;
     }
 }
 
@@ -616,11 +629,11 @@ int mplbws_conv3x3_wgrad(const float* in, const float* dout, float* dw, float* d
     float* bp = dbias ? workspace + chunks * cnt : nullptr;
     hipLaunchKernelGGL(k_conv_wgrad, grd, dim3(64 * kWaves), 0, 0, in, dout, workspace, bp, h,
                        wd, cin, cout, rows);
-    hipLaunchKernelGGL(k_sum_partials, dim3(grid_for(cnt)), dim3(256), 0, 0, workspace, dw, cnt,
-                       (int)chunks);
+    hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)((cnt + 63) / 64)), dim3(64 * kSumWaves), 0,
+                       0, workspace, dw, cnt, (int)chunks);
     if (dbias)
-        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, 0, bp, dbias, (int64_t)cout,
-                           (int)chunks);
+        hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)((cout + 63) / 64)),
+                           dim3(64 * kSumWaves), 0, 0, bp, dbias, (int64_t)cout, (int)chunks);
     return status();
 }
 
