@@ -224,6 +224,8 @@ def bench_batch(args):
     b, d = g.kernel_times()
     fold, xgroup, h2, bulk_nodes = g.launch_shape()
     achieved = bulk_nodes * 2 * 9 * ELEM[st] / (b / args.steps / 1e3) / 1e9
+    tname = traffic_profile_name("batch%d" % len(sizes), st, fold, xgroup)
+    traffic = traffic_from_profiles(tname)
     log("bench: batch %.0f MLUPS (wall, graph replays), event-timed kernels %.4f ms/step "
         "(bulk %.4f = %.0f GB/s, boundary %.4f)"
         % (mlups, ker / args.steps, b / args.steps, achieved, d / args.steps))
@@ -253,8 +255,12 @@ def bench_batch(args):
                    "nodes": nodes},
         "kernel_ms_per_step": round(ker / args.steps, 5),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "traffic_source": "no committed counter summary for the batch",
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": ("profiles/%s (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE "
+                                        "for this launch shape, scaled by the copy-kernel "
+                                        "calibration; not measured in this run)" % tname)
+                     if traffic else
+                     "no committed counter summary for this launch shape (%s)" % tname,
                      "kernel": "k_bulk (D2Q9, %s, step)" % st,
                      "bulk_nodes_per_launch": bulk_nodes,
                      "launch_shape": {"fold": fold, "xgroup": xgroup, "h2": h2},
