@@ -265,6 +265,16 @@ long long advance(SizeType iter, SizeType n, double criteria, SizeType period, d
 
 }  // namespace
 
+// An App that never steps (Tests/ConservationTest3D/conservation3d.cpp)
+// defines no body-force hook (flowfield.h:114-115).  The reference links it
+// because its library is an archive whose evolution object such an App
+// never pulls in; this weak definition lets it link here, and an App that
+// does step without defining the hook stops with a message.
+__attribute__((weak)) void UpdateMacroscopicBodyForce(const Real time) {
+    die("Error! the application defines no UpdateMacroscopicBodyForce(Real), which stepping "
+        "calls at every step (time %g)", (double)time);
+}
+
 // ---------------------------------------------------------------------------
 // Block (block.cpp:14-148)
 Block::Block(const int blockId, const std::string& blockName,

@@ -16,18 +16,20 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_APPS), reason="reference Apps not mounted")
-@pytest.mark.parametrize("app,dim", [("2DCavity/lbm2d_cavity.cpp", 2),
-                                     ("3DCavity/lbm3d_cavity.cpp", 3),
-                                     ("3DLChannel/lbm3d_L.cpp", 3),
-                                     ("3DCavity-Swap/lbm3d_cavity_swap.cpp", 3)])
+@pytest.mark.parametrize("app,dim", [("Apps/2DCavity/lbm2d_cavity.cpp", 2),
+                                     ("Apps/3DCavity/lbm3d_cavity.cpp", 3),
+                                     ("Apps/3DLChannel/lbm3d_L.cpp", 3),
+                                     ("Apps/3DCavity-Swap/lbm3d_cavity_swap.cpp", 3),
+                                     ("Tests/ConservationTest3D/conservation3d.cpp", 3)])
 def test_reference_apps_link_unchanged(app, dim, tmp_path):
-    """The unmodified App sources build against the HiLeMMS headers and link
-    against libmplbhip.so (compile and link only: nothing of the reference
-    is run or shipped)."""
+    """The unmodified App sources (and the conservation test program, which
+    never steps and defines no body-force hook) build against the HiLeMMS
+    headers and link against libmplbhip.so (compile and link only: nothing
+    of the reference is run or shipped)."""
     out = tmp_path / "app"
     subprocess.check_call(["make", "-s", "-B", "-C", HILEMMS,
-                           "APP=" + os.path.join(REF_APPS, app), "DIM=%d" % dim,
-                           "OUT=" + str(out)])
+                           "APP=" + os.path.join(os.path.dirname(REF_APPS), app),
+                           "DIM=%d" % dim, "OUT=" + str(out)])
     assert out.exists()
 
 
