@@ -4,8 +4,10 @@ Re 400, lid 0.05, seeded density noise, 2,000 steps (62 replays of the
 the FP64 oracle's bit for bit; the oracle's state is committed as a digest
 (tests/golden/c2_fp64_digest.json, written by make_c2_digest.py in the same
 directory): SHA-256 of f in [q][k][j][i] order and probe values, bit for
-bit, and the mass to 1e-11 relative (the two sides add the same 3.2e8
-populations in different orders; measured 1.2e-13)."""
+bit, and the mass to 1e-11 relative (the committed mass is a plain
+running sum of the 3.2e8 populations, written before the oracle's mass
+became a compensated sum; the device's tree of partial sums is 1.2e-13
+from it)."""
 import json
 import os
 import sys

@@ -942,3 +942,19 @@ def test_fdperiodic_copies_from_the_halo_cell(dim):
                     else:
                         assert np.array_equal(got, f[(slice(None),) + pos]), (fd, pos)
         assert copied > 0
+
+
+def test_oracle_mass_is_the_exact_sum():
+    """The oracle's total mass equals the correctly rounded sum of its
+    populations (math.fsum) on a 163,840-node closed L-channel after three
+    steps, where a plain running sum in the same order is already 1e-11
+    relative off: the 1e-13 mass comparisons with the GPU need the exact
+    sum, not one rounding order."""
+    o = OracleCase(3)
+    cases.lchannel_periodic(o, a=32, length=64, tau_l=0.8, g=1e-6)
+    o.step(3)
+    vals = np.concatenate([np.moveaxis(o.get_f(b), 0, -1).ravel() for b in range(3)])
+    exact = math.fsum(vals)
+    assert abs(o.mass() - exact) <= 2e-16 * abs(exact)
+    plain = float(np.cumsum(vals)[-1])
+    assert abs(plain - exact) > 1e-12 * abs(exact)

@@ -230,3 +230,30 @@ def test_gpu_lchannel_periodic_bitexact():
     assert g.mass() == pytest.approx(o.mass(), rel=1e-13)
     # the loop carries flow through the rotated link
     assert np.max(np.abs(g.get_macro(0, 2))) > 0
+
+
+@pytest.mark.gpu
+def test_gpu_c3_lchannel_full_size_fp64_few_steps():
+    """The L-channel at its benchmark size (blocks 128 x 256 x 128, 128^3,
+    256 x 128 x 128: 10.5M nodes, the merged halo launch, the autotuned
+    launch shape, the boundary kernel beside the bulk kernel) in FP64 for as
+    many steps as the oracle finishes quickly: every population and macro
+    bit for bit."""
+    from mplb_amd import cases
+    steps = 4
+    print("closed L-channel a=128 FP64, %d steps against the oracle" % steps, flush=True)
+    o = OracleCase(3)
+    cases.lchannel_periodic(o, a=128, length=256, tau_l=0.8, g=1e-6)
+    g = _gpu(3)
+    cases.lchannel_periodic(g, a=128, length=256, tau_l=0.8, g=1e-6)
+    g.step(steps)
+    o.step(steps)
+    for b in range(3):
+        assert np.array_equal(g.get_f(b), o.get_f(b)), b
+    g.update_macros()
+    o.update_macros()
+    for b in range(3):
+        for v in range(4):
+            assert np.array_equal(g.get_macro(b, v), o.get_macro(b, v)), (b, v)
+    assert g.mass() == pytest.approx(o.mass(), rel=1e-13)
+    g.close()
