@@ -1,5 +1,5 @@
 """Benchmark configurations at their full size against the FP64 oracle, in
-the launch shape the autotuner picks: the first steps of the 256 batched
+the launch shape the autotuner picks: the first 40 steps of the 256 batched
 2D cavities (7.3M nodes in one launch per step), each instance bit for bit
 against a single oracle run of its cavity."""
 import numpy as np
@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 
 def test_c4_batch256_fp64_first_steps_bitexact():
-    steps = 20
+    steps = 40  # one replay of the 32-step graph, then single steps
     sizes, res = cases.batch_spec(256)
     print("C4: 256 cavities, %d nodes, FP64, %d steps" % (sum(n * n for n in sizes), steps),
           flush=True)

@@ -240,7 +240,7 @@ def test_gpu_c3_lchannel_full_size_fp64_few_steps():
     many steps as the oracle finishes quickly: every population and macro
     bit for bit."""
     from mplb_amd import cases
-    steps = 4
+    steps = 40  # one replay of the 32-step graph, then single steps
     print("closed L-channel a=128 FP64, %d steps against the oracle" % steps, flush=True)
     o = OracleCase(3)
     cases.lchannel_periodic(o, a=128, length=256, tau_l=0.8, g=1e-6)
