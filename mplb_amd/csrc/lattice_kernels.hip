@@ -1575,7 +1575,8 @@ __global__ void k_fill_bytes(uint32_t* p, int64_t n, uint32_t v) {
 template <int D, int ST, int MODE>
 static void launch_step(const KParams& kp, int part, hipStream_t s) {
     dim3 blk(kTX, kTY, 1);
-    if (ST == MPLBHIP_STORE_F16 && MODE == MODE_STEP && kp.h2 && !kp.fused) {
+    // the two-node FP16 kernel exists for D2Q9 and D3Q19
+    if (ST == MPLBHIP_STORE_F16 && MODE == MODE_STEP && D != 15 && kp.h2 && !kp.fused) {
         if (part != 2) (void)launch_bulk_h2(D, kp, s);
         if (part != 1 && kp.nbnd > 0)
             hipLaunchKernelGGL((k_lattice<D, ST, MODE, SCOPE_BND>), dim3(kp.nbnd), blk, 0, s, kp);
@@ -1638,10 +1639,14 @@ static hipError_t launch_mode(int mode, int part, const KParams& kp, hipStream_t
 
 // MATERIALIZE through the step's bulk and boundary kernels (their launch
 // shape and partial-sum slots); the whole-lattice launch remains for t = 0
-// (f^0 from a host array), fused cases and D3Q15
+// (f^0 from a host array) and fused cases
 hipError_t launch_materialize_split(int dim, int storage, const KParams& kp, hipStream_t s) {
-    if (kp.fused || kp.fhost_in || (dim != 2 && dim != 3)) return hipErrorInvalidValue;
-    if (dim == 2) {
+    if (kp.fused || kp.fhost_in || (dim != 2 && dim != 3 && dim != 15)) return hipErrorInvalidValue;
+    if (dim == 15) {
+        if (storage == MPLBHIP_STORE_F64) launch_step<15, MPLBHIP_STORE_F64, MODE_MATERIALIZE>(kp, 0, s);
+        else if (storage == MPLBHIP_STORE_F32) launch_step<15, MPLBHIP_STORE_F32, MODE_MATERIALIZE>(kp, 0, s);
+        else launch_step<15, MPLBHIP_STORE_F16, MODE_MATERIALIZE>(kp, 0, s);
+    } else if (dim == 2) {
         if (storage == MPLBHIP_STORE_F64) launch_step<2, MPLBHIP_STORE_F64, MODE_MATERIALIZE>(kp, 0, s);
         else if (storage == MPLBHIP_STORE_F32) launch_step<2, MPLBHIP_STORE_F32, MODE_MATERIALIZE>(kp, 0, s);
         else launch_step<2, MPLBHIP_STORE_F16, MODE_MATERIALIZE>(kp, 0, s);
@@ -1653,46 +1658,13 @@ hipError_t launch_materialize_split(int dim, int storage, const KParams& kp, hip
     return hipGetLastError();
 }
 
-// lattices that run every step as one whole-lattice launch (D3Q15): only
-// the SCOPE_ALL kernels are instantiated for them
-template <int D, int ST>
-static hipError_t launch_mode_fused(int mode, int part, const KParams& kp, hipStream_t s) {
-    const dim3 blk(kTX, kTY, 1), grd(kp.ntiles, 1, 1);
-    if (mode == MODE_STEP || mode == MODE_STEP_CHECK) {
-        if (!kp.fused) return hipErrorInvalidValue;
-        if (part == 2) return hipSuccess;  // the one launch is the "bulk" part
-    }
-    switch (mode) {
-        case MODE_STEP:
-            hipLaunchKernelGGL((k_lattice<D, ST, MODE_STEP, SCOPE_ALL>), grd, blk, 0, s, kp);
-            break;
-        case MODE_STEP_CHECK:
-            hipLaunchKernelGGL((k_lattice<D, ST, MODE_STEP_CHECK, SCOPE_ALL>), grd, blk, 0, s, kp);
-            break;
-        case MODE_MATERIALIZE:
-            hipLaunchKernelGGL((k_lattice<D, ST, MODE_MATERIALIZE, SCOPE_ALL>), grd, blk, 0, s, kp);
-            break;
-        case MODE_INIT_FEQ:
-            hipLaunchKernelGGL((k_lattice<D, ST, MODE_INIT_FEQ, SCOPE_ALL>), grd, blk, 0, s, kp);
-            break;
-        case MODE_INIT_F:
-            hipLaunchKernelGGL((k_lattice<D, ST, MODE_INIT_F, SCOPE_ALL>), grd, blk, 0, s, kp);
-            break;
-        default:
-            return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
 // dim: the lattice key, 2 (D2Q9), 3 (D3Q19) or 15 (D3Q15)
 hipError_t launch_lattice(int dim, int storage, int mode, const KParams& kp, hipStream_t s,
                           int part) {
     if (dim == 15) {
-        if (storage == MPLBHIP_STORE_F64)
-            return launch_mode_fused<15, MPLBHIP_STORE_F64>(mode, part, kp, s);
-        if (storage == MPLBHIP_STORE_F32)
-            return launch_mode_fused<15, MPLBHIP_STORE_F32>(mode, part, kp, s);
-        return launch_mode_fused<15, MPLBHIP_STORE_F16>(mode, part, kp, s);
+        if (storage == MPLBHIP_STORE_F64) return launch_mode<15, MPLBHIP_STORE_F64>(mode, part, kp, s);
+        if (storage == MPLBHIP_STORE_F32) return launch_mode<15, MPLBHIP_STORE_F32>(mode, part, kp, s);
+        return launch_mode<15, MPLBHIP_STORE_F16>(mode, part, kp, s);
     }
     if (dim == 2) {
         if (storage == MPLBHIP_STORE_F64) return launch_mode<2, MPLBHIP_STORE_F64>(mode, part, kp, s);

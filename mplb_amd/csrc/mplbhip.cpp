@@ -485,7 +485,7 @@ static int materialize(mplbhip_ctx* c, bool want_f, bool residual,
     kp.want_residual = residual ? 1 : 0;
     // after the first step, through the step's bulk and boundary kernels
     // (the whole-lattice kernel is the general one, several times slower)
-    const bool split = c->tstep > 0 && !c->fused && (c->lat == 2 || c->lat == 3);
+    const bool split = c->tstep > 0 && !c->fused;
     if (split)
         HIPCHK(c, launch_materialize_split(c->lat, c->storage, kp, c->stream));
     else
@@ -1387,11 +1387,6 @@ int mplbhip_prepare(mplbhip_ctx* c) {
         c->fused = std::atoi(e) != 0;
         c->fused_env = true;
     }
-    if (c->lat == 15) {
-        // D3Q15 has whole-lattice kernels only: every step is one launch
-        c->fused = true;
-        c->fused_env = true;
-    }
     c->fused_full = c->fused;
     if (const char* e = std::getenv("MPLBHIP_BATCH_COMPACT")) c->batch_compact = std::atoi(e) != 0;
     if (const char* e = std::getenv("MPLBHIP_XGROUP")) {
@@ -1608,7 +1603,7 @@ int mplbhip_set_accel(mplbhip_ctx* c, int block, const double* g) {
 static int autotune(mplbhip_ctx* c) {
     if (c->tuned) return 0;
     c->tuned = true;
-    if (c->interior_nodes < (1 << 20) || c->lat == 15) return 0;
+    if (c->interior_nodes < (1 << 20)) return 0;
     std::vector<std::array<int, 3>> cand;  // fold, xgroup, h2
     std::vector<int> folds;
     if (c->fold_env) {
@@ -2224,7 +2219,7 @@ int mplbhip_mass_noalloc(mplbhip_ctx* c, double* mass) {
     if (!c->prepared) return fail(c, MPLBHIP_ERR_ARG, "call mplbhip_prepare first");
     // the same launches and partial-sum slots as mplbhip_mass after the
     // first step: the bulk and boundary kernels, or the whole-lattice one
-    const bool split = !c->swap && !c->fused && (c->lat == 2 || c->lat == 3);
+    const bool split = !c->swap && !c->fused;
     KParams kp = base_params(c);
     const long long slots = split ? (long long)kp.nbulk + kp.nbnd : (long long)c->ntiles;
     const long long need = (slots + (long long)c->bp.size() + 1) * kNAcc;

@@ -47,8 +47,8 @@ def temperature_field(shape, amp=0.25):
 # ---- D3Q15 ---------------------------------------------------------------------
 @pytest.mark.parametrize("order", [("Left", "Right", "Top", "Bottom", "Front", "Back"),
                                    ("Top", "Bottom", "Left", "Right", "Front", "Back")])
-def test_d3q15_cavity_bitexact(order):
-    log("D3Q15 cavity 11^3 F64 order %s" % (order[0],))
+def test_d3q15_cavity_bitexact(order, step_form):
+    log("D3Q15 cavity 11^3 F64 order %s form %s" % (order[0], step_form))
     g = hip.LatticeCase(3, storage=hip.STORE_F64, device=0, lattice=15)
     cases.cavity3d(g, 11, re=100.0, ulid=0.05, noise=True, order=order)
     o = OracleCase(3, lattice=15)
@@ -58,8 +58,8 @@ def test_d3q15_cavity_bitexact(order):
     compare(g, o, "d3q15 cavity")
 
 
-def test_d3q15_periodic_forced_box_bitexact():
-    log("D3Q15 periodic forced box F64")
+def test_d3q15_periodic_forced_box_bitexact(step_form):
+    log("D3Q15 periodic forced box F64 form %s" % step_form)
     rng = np.random.default_rng(3)
     size = (9, 7, 8)
     shp = tuple(reversed(size))
@@ -93,11 +93,11 @@ def test_d3q15_iterate_residual_matches_oracle():
 
 
 @pytest.mark.parametrize("storage,tol_u", [(hip.STORE_F32, 2e-6), (hip.STORE_F16, 2e-4)])
-def test_d3q15_reduced_storage_vs_oracle(storage, tol_u):
+def test_d3q15_reduced_storage_vs_oracle(storage, tol_u, step_form):
     """FP32 f - w and FP16 (f - w)/s_q on D3Q15 (scales w_q / 2^10 of its
-    own weights) against the FP64 oracle after 60 steps of a 12^3 cavity:
-    max-abs velocity error below tol_u (lid speed 0.087)."""
-    log("D3Q15 cavity storage %d" % storage)
+    own weights) against the FP64 oracle after 60 steps of a 12^3 cavity, in
+    every step form: max-abs velocity error below tol_u (lid speed 0.087)."""
+    log("D3Q15 cavity storage %d form %s" % (storage, step_form))
     g = hip.LatticeCase(3, storage=storage, device=0, lattice=15)
     cases.cavity3d(g, 12, re=100.0, ulid=0.05, noise=True)
     o = OracleCase(3, lattice=15)
@@ -128,8 +128,6 @@ def test_thermal_cavity_bitexact(dim, q, step_form):
     relaxation tau = tauRef / (rho sqrt(T)) and the 4th-order equilibrium,
     bit for bit in every step form (whole-lattice launch, bulk plus boundary
     kernels at each fold level)."""
-    if q == 15 and step_form != "fused":
-        pytest.skip("D3Q15 runs every step as one whole-lattice launch")
     n = 12 if dim == 2 else 10
     log("thermal cavity %dD Q%d form %s" % (dim, q, step_form))
     g = hip.LatticeCase(dim, storage=hip.STORE_F64, device=0, lattice=q)
