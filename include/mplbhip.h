@@ -75,8 +75,7 @@ int mplbhip_define_rotated_connection(mplbhip_ctx* ctx, int from_block,
                                       int to_surface, int vertex_type);
 /* DefineComponents lattNames (model.cpp:208-310; SetupD2Q9Latt :99-112,
  * SetupD3Q19Latt :114-131, SetupD3Q15Latt :133-150), before mplbhip_prepare:
- * the number of velocities, 9 (2D), 19 (3D, the default) or 15 (3D).  D3Q15
- * cases run every step as one whole-lattice launch. */
+ * the number of velocities, 9 (2D), 19 (3D, the default) or 15 (3D). */
 int mplbhip_define_lattice(mplbhip_ctx* ctx, int q);
 /* DefineCollision (model.cpp:406-445), before mplbhip_prepare:
  * 0 Collision_BGKIsothermal2nd (the default), 1 Collision_BGKThermal4th
