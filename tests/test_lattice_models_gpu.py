@@ -74,6 +74,20 @@ def test_d3q15_periodic_forced_box_bitexact(step_form):
     compare(g, o, "d3q15 box")
 
 
+def test_d3q15_lchannel_bitexact(step_form):
+    """Three D3Q15 blocks joined by VirtualBoundary faces and the rotated
+    periodic link, walls on the rest, body force: the halo copies carry
+    D3Q15's inward populations."""
+    log("D3Q15 closed L-channel a=9 form %s" % step_form)
+    g = hip.LatticeCase(3, storage=hip.STORE_F64, device=0, lattice=15)
+    cases.lchannel_periodic(g, a=9, length=14, g=1e-4)
+    o = OracleCase(3, lattice=15)
+    cases.lchannel_periodic(o, a=9, length=14, g=1e-4)
+    g.step(60)
+    o.step(60)
+    compare(g, o, "d3q15 l-channel")
+
+
 def test_d3q15_iterate_residual_matches_oracle():
     log("D3Q15 iterate to 1e-3")
     period, crit = 25, 1e-3
