@@ -225,3 +225,28 @@ def test_gpu_cavity_with_rectangles_bitexact_and_residual():
     rg, _ = g.residual(300)
     assert rg == pytest.approx(ro, rel=1e-9)
     assert p["solid"].sum() > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.usefixtures("step_form")
+@pytest.mark.parametrize("storage,tol_u", [("f32", 2e-6), ("f16", 3e-4)])
+def test_gpu_cavity_with_rectangles_reduced_storage(storage, tol_u):
+    """The warm start's cavities (rectangles inside a lid-driven cavity) with
+    FP32 f - w and FP16 storage, every step form (the bulk kernel's folded
+    wall lanes beside obstacle links): velocity within tol_u of the FP64
+    oracle after 300 steps (lid speed 0.087), mass to 1e-5."""
+    from mplb_amd import cases, hip
+    st = {"f32": hip.STORE_F32, "f16": hip.STORE_F16}[storage]
+    rects = [(5, 6, 9, 4), (18, 20, 8, 9)]
+    print("cavity 32^2 with rectangles, storage %s" % storage, flush=True)
+    o = OracleCase(2)
+    g = hip.LatticeCase(2, storage=st, device=0)
+    cases.cavity2d_obstacles(o, 32, rects, re=50.0)
+    cases.cavity2d_obstacles(g, 32, rects, re=50.0)
+    o.step(300)
+    g.step(300)
+    o.update_macros()
+    g.update_macros()
+    du = max(float(np.max(np.abs(g.get_macro(0, v) - o.get_macro(0, v)))) for v in (1, 2))
+    assert du < tol_u, du
+    assert abs(g.mass() - o.mass()) < 1e-5 * o.mass()
