@@ -591,7 +591,8 @@ def main():
                      "timing": "value: K steps of graph replays (the production path); "
                                "kernel times: HIP events around every kernel of K more "
                                "steps launched one by one on the same streams",
-                     "copy_probe_GBs": round(copy_gbs, 1)},
+                     "copy_probe_GBs": round(copy_gbs, 1),
+                     "frac_of_copy": round(achieved / copy_gbs, 4)},
         "cpu_baseline": cpu,
     }
     if err is not None:
