@@ -163,6 +163,15 @@ struct KParams {
     // the macros)
     int32_t thermal;
     const double* temp;
+    // the block of every logical workgroup, one table per grid (whole
+    // lattice, bulk, bulk over every z plane, two-node FP16 bulk, boundary
+    // at the fold level in force): one scalar load where a binary search
+    // over the blocks' first workgroups took log2(nblocks) dependent ones
+    const int32_t* wg_block_all;
+    const int32_t* wg_block_bulk;
+    const int32_t* wg_block_bulk3;
+    const int32_t* wg_block_h2;
+    const int32_t* wg_block_bnd;
 };
 
 struct HaloParams {

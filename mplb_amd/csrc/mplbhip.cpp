@@ -56,6 +56,9 @@ struct DevBuf {
 
 }  // namespace
 
+// the tables of mplbhip_ctx::wg_block (boundary: kWgBnd + fold level)
+enum WgTable : int { kWgAll = 0, kWgBulk = 1, kWgBulk3 = 2, kWgH2 = 3, kWgBnd = 4 };
+
 struct mplbhip_ctx {
     int device = -1;
     int dim = 3;
@@ -130,6 +133,12 @@ struct mplbhip_ctx {
     hipStream_t stream = nullptr;
     DevBuf fs[2], per[2], macro[4], macro_copy[4], progs, blocks, partials,
         red, fhost, parity[2], dsegs, dobst;
+    // the block of every workgroup of each grid (KParams::wg_block_*):
+    // tables kWgAll, kWgBulk, kWgBulk3, kWgH2, then the boundary grid at
+    // fold levels 0-3, concatenated; offsets into wg_block_host
+    DevBuf wg_block;
+    std::vector<int32_t> wg_block_host;
+    size_t wg_block_off[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     // the macros at PreDefinedInitialCondition: f^0 = feq of these, formed
     // at the first step (the collision needs the time step, which the Apps
     // set afterwards); a macro refresh at t = 0 must not change it
@@ -290,6 +299,14 @@ static KParams base_params(mplbhip_ctx* c) {
     }
     kp.cur_in = static_cast<const int32_t*>(c->parity[c->cur].p);
     kp.cur_out = static_cast<int32_t*>(c->parity[1 - c->cur].p);
+    const int32_t* wgb = static_cast<const int32_t*>(c->wg_block.p);
+    if (wgb) {
+        kp.wg_block_all = wgb + c->wg_block_off[kWgAll];
+        kp.wg_block_bulk = wgb + c->wg_block_off[kWgBulk];
+        kp.wg_block_bulk3 = wgb + c->wg_block_off[kWgBulk3];
+        kp.wg_block_h2 = wgb + c->wg_block_off[kWgH2];
+        kp.wg_block_bnd = wgb + c->wg_block_off[kWgBnd + fold];
+    }
     return kp;
 }
 
@@ -584,6 +601,7 @@ void mplbhip_destroy(mplbhip_ctx* c) {
         dfree(c->blocks);
         dfree(c->partials);
         dfree(c->dsegs);
+        dfree(c->wg_block);
         dfree(c->dobst);
         dfree(c->red);
         dfree(c->fhost);
@@ -1250,7 +1268,38 @@ static int assign_tiles(mplbhip_ctx* c, bool only_active) {
                         : 0;
         B.rowwg = B.tiles_x % kTY == 0 && B.n[1] % kTY == 0 ? 1 : 0;
     }
+    // each grid's workgroup -> block table: block b owns workgroups
+    // [its first, the next block's first) of that grid
+    c->wg_block_host.clear();
+    auto table = [&](int k, int total, auto first) {
+        c->wg_block_off[k] = c->wg_block_host.size();
+        for (int b = 0; b < nb; b++) {
+            const int lo = first(c->bp[b]);
+            const int hi = b + 1 < nb ? first(c->bp[b + 1]) : total;
+            for (int w = lo; w < hi; w++) c->wg_block_host.push_back(b);
+        }
+    };
+    table(kWgAll, tiles, [](const BlockParams& B) { return B.tile_start; });
+    table(kWgBulk, bulk, [](const BlockParams& B) { return B.bulk_tile_start; });
+    table(kWgBulk3, bulk3, [](const BlockParams& B) { return B.bulk_tile_start3; });
+    table(kWgH2, bulk2, [](const BlockParams& B) { return B.h2_tile_start; });
+    for (int v = 0; v < 4; v++)
+        table(kWgBnd + v, bnd_v[v], [v](const BlockParams& B) { return B.bnd_wg_start[v]; });
     return grouped;
+}
+
+// the workgroup -> block tables to the device (after assign_tiles; the
+// buffer holds the tables over every block, compaction only shrinks them)
+static int upload_wg_blocks(mplbhip_ctx* c) {
+    const size_t bytes = c->wg_block_host.size() * sizeof(int32_t);
+    if (bytes > c->wg_block.bytes)
+        return fail(c, MPLBHIP_ERR_HIP, "workgroup tables outgrew their buffer");
+    if (bytes)
+        HIPCHK(c, hipMemcpyAsync(c->wg_block.p, c->wg_block_host.data(), bytes,
+                                 hipMemcpyHostToDevice, c->stream));
+    // the host tables change at the next compaction: the copy must be done
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
 }
 
 int mplbhip_prepare(mplbhip_ctx* c) {
@@ -1448,6 +1497,10 @@ int mplbhip_prepare(mplbhip_ctx* c) {
     if (int rc = dalloc(c, c->partials, nslots * kNAcc * sizeof(double))) return rc;
     if (int rc = dalloc(c, c->dsegs, std::max<size_t>(1, c->segs.size()) * sizeof(BndSeg)))
         return rc;
+    if (int rc = dalloc(c, c->wg_block,
+                        std::max<size_t>(1, c->wg_block_host.size()) * sizeof(int32_t)))
+        return rc;
+    if (int rc = upload_wg_blocks(c)) return rc;
     if (!c->segs.empty())
         HIPCHK(c, hipMemcpyAsync(c->dsegs.p, c->segs.data(), c->segs.size() * sizeof(BndSeg),
                                  hipMemcpyHostToDevice, c->stream));
@@ -1892,6 +1945,7 @@ static int compact_grid(mplbhip_ctx* c, bool on) {
         HIPCHK(c, hipMemcpyAsync(c->parity[1 - c->cur].p, c->parity[c->cur].p,
                                  nb * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
     assign_tiles(c, on);
+    if (int rc = upload_wg_blocks(c)) return rc;
     // a small remainder runs as one launch per step, as small cases do
     c->fused = c->fused_env ? c->fused_full
                             : (on ? c->grid_nodes < (1 << 18) : c->fused_full);
