@@ -481,9 +481,9 @@ __device__ __forceinline__ int load_parity(const KParams& kp, int b) {
 }
 
 // block of logical workgroup wg of a grid, from that grid's table (wg is
-// wave-uniform: a scalar load)
-__device__ __forceinline__ int wg_block(const int32_t* table, int wg) {
-    return const_view(table)[wg];
+// wave-uniform: a scalar load); a one-block case reads nothing
+__device__ __forceinline__ int wg_block(const KParams& kp, const int32_t* table, int wg) {
+    return kp.nblocks == 1 ? 0 : const_view(table)[wg];
 }
 
 // macros (+ residual partials), then the MATERIALIZE output or the BGK
@@ -749,7 +749,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
     bool active;
     if constexpr (SCOPE == SCOPE_ALL) {
         wg = logical_wg(blockIdx.x, gridDim.x);
-        b = wg_block(kp.wg_block_all, wg);
+        b = wg_block(kp, kp.wg_block_all, wg);
         const BlockParams B = load_block(kp, b);
         const int t = wg - B.tile_start;
         const int t1 = fast_div(t, B.div_m[0], B.div_sh[0]);
@@ -766,7 +766,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_lattice(KParams kp) {
     } else {
         wg = blockIdx.x;
         const int v = kp.fold;
-        b = wg_block(kp.wg_block_bnd, wg);
+        b = wg_block(kp, kp.wg_block_bnd, wg);
         // the fold level indexes the block's arrays at run time: read the
         // three fields through scalar loads, not from a private copy of the
         // block (a dynamically indexed struct lives in scratch memory, and
@@ -939,7 +939,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_bulk(KParams kp) {
     // fold level 3 spans every z plane (3D): its grid has its own tile starts
     constexpr bool ZALL = FOLD == 3 && D != 2;
     const int wg = kp.xcd_remap ? logical_wg(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    const int b = wg_block(ZALL ? kp.wg_block_bulk3 : kp.wg_block_bulk, wg);
+    const int b = wg_block(kp, ZALL ? kp.wg_block_bulk3 : kp.wg_block_bulk, wg);
     const BlockParams B = load_block(kp, b);
     int t = wg - (ZALL ? B.bulk_tile_start3 : B.bulk_tile_start);
     // a block whose first tile sits on XCD 0 and whose tile count is a
@@ -1312,7 +1312,7 @@ template <int D>
 __global__ __launch_bounds__(kTX * kTY) void k_bulk_h2(KParams kp) {
     constexpr int Q = lq(D);
     const int wg = (int)blockIdx.x;
-    const int b = wg_block(kp.wg_block_h2, wg);
+    const int b = wg_block(kp, kp.wg_block_h2, wg);
     const BlockParams B = load_block(kp, b);
     int t = wg - B.h2_tile_start;
     const bool first = t == 0;
